@@ -1,0 +1,118 @@
+"""Loader for the native libraries of this package.
+
+``libsr_hip.so`` holds the gfx950 kernels behind the C-ABI declared in
+``include/sr_eval.h``; ``libsr_host.so`` holds the host-side PCFG engine
+declared in ``include/sr_pcfg.h``.  Both are built in-tree by
+``__graft_entry__.build()`` (or ``python -m cudasymbolicregression_amd.build``).
+
+There is no fallback: if a library is missing, importing the module that needs
+it raises immediately, and every GPU entry point refuses tensors that are not
+on a ROCm device.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import c_double, c_int, c_int64, c_void_p, POINTER
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+HIP_LIB_PATH = os.path.join(_HERE, "libsr_hip.so")
+HOST_LIB_PATH = os.path.join(_HERE, "libsr_host.so")
+
+_hip = None
+_host = None
+
+
+class NativeLibraryMissing(ImportError):
+    pass
+
+
+def _load(path: str, what: str) -> ctypes.CDLL:
+    if not os.path.exists(path):
+        raise NativeLibraryMissing(
+            f"{what} not found at {path}. Build it first with "
+            "`python -c 'import __graft_entry__ as g; g.build()'` from the repository root "
+            "(or `python -m cudasymbolicregression_amd.build`)."
+        )
+    return ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+
+
+MISSING_SYMBOLS: list = []
+
+
+def _declare(lib: ctypes.CDLL, name: str, restype, argtypes) -> None:
+    try:
+        fn = getattr(lib, name)
+    except AttributeError:
+        # Reported by tests/test_abi.py; calling it raises AttributeError.
+        MISSING_SYMBOLS.append(name)
+        return
+    fn.restype = restype
+    fn.argtypes = argtypes
+
+
+def hip() -> ctypes.CDLL:
+    """The gfx950 kernel library (loaded once)."""
+    global _hip
+    if _hip is None:
+        lib = _load(HIP_LIB_PATH, "The gfx950 kernel library libsr_hip.so")
+        P, I64, I = c_void_p, c_int64, c_int
+        _declare(lib, "sr_validate_mb", I, [P, P, I64, I64, P, P])
+        for sfx in ("f32", "f64"):
+            _declare(lib, f"sr_forward_mb_{sfx}", I, [P, I64, I64, P, P, P, I64, I64, P, P])
+            _declare(lib, f"sr_backward_mb_{sfx}", I, [P, P, P, P, I64, I64, I64, I64, P, P, P, I64, P, P])
+            _declare(lib, f"sr_fused_root_{sfx}", I,
+                     [P, I64, I64, P, P, P, I64, I64, P, I, I, P, P, P, P, P, P, P, I64, P])
+            _declare(lib, f"sr_multi_forward_mb_{sfx}", I, [P, I64, I64, P, P, P, I64, I64, I64, P, P])
+            _declare(lib, f"sr_multi_backward_mb_{sfx}", I,
+                     [P, P, P, P, I64, I64, I64, I64, I64, P, P, P, I64, P, P])
+        _declare(lib, "sr_backward_mb_workspace_bytes", I64, [I64, I64, I64, I])
+        _declare(lib, "sr_multi_backward_mb_workspace_bytes", I64, [I64, I64, I64, I64, I])
+        _declare(lib, "sr_fused_workspace_bytes", I64, [I64, I64, I64, I, I])
+        _declare(lib, "sr_lbfgs_fit", I,
+                 [P, P, I64, I64, P, P, P, I64, I64, P, P, P, P, P, P, P, P, P, I, I, c_double, I, I, c_double,
+                  P, I64, P])
+        _declare(lib, "sr_lbfgs_workspace_bytes", I64, [I64, I64])
+        _hip = lib
+    return _hip
+
+
+def host() -> ctypes.CDLL:
+    """The host-side PCFG engine (loaded once)."""
+    global _host
+    if _host is None:
+        lib = _load(HOST_LIB_PATH, "The host PCFG library libsr_host.so")
+        P, I64, I = c_void_p, c_int64, c_int
+        _declare(lib, "sr_pcfg_sort_rules", None, [P, I64, P])
+        _declare(lib, "sr_pcfg_sample", None, [P, P, P, P, P, I64, I64, I64, P, P, P, I64])
+        _declare(lib, "sr_pcfg_infix_to_postfix", None, [P, P, P, P, I64, I64, I64, I64])
+        _declare(lib, "sr_pcfg_infix_to_prefix", None, [P, P, P, P, I64, I64, I64, I64])
+        _declare(lib, "sr_pcfg_postfix_to_infix", None, [P, P, P, I64, I64, I64, I64, I64])
+        _declare(lib, "sr_pcfg_prefix_to_infix", None, [P, P, P, I64, I64, I64, I64, I64])
+        _declare(lib, "sr_pcfg_prefix_to_postfix", None, [P, P, P, I64, I64])
+        _declare(lib, "sr_pcfg_postfix_to_prefix", None, [P, P, P, I64, I64])
+        _declare(lib, "sr_pcfg_get_prefix_parent", None, [P, P, P, I64, I64])
+        _declare(lib, "sr_pcfg_get_postfix_children", None, [P, P, P, I64, I64])
+        _host = lib
+    return _host
+
+
+def ptr(t) -> int:
+    """Raw data pointer of a tensor (None -> NULL)."""
+    return 0 if t is None else int(t.data_ptr())
+
+
+def stream_of(device) -> int:
+    """Raw hipStream_t of torch's current stream on `device`."""
+    import torch
+
+    return int(torch.cuda.current_stream(device).cuda_stream)
+
+
+def check_status(code: int, what: str) -> None:
+    if code != 0:
+        if code == -1:
+            raise RuntimeError(f"{what}: invalid arguments (sizes out of the supported range)")
+        if code == -2:
+            raise RuntimeError(f"{what}: workspace too small")
+        raise RuntimeError(f"{what}: HIP error {code}")

@@ -1,0 +1,282 @@
+// Wave64 helpers and the shared per-expression evaluation core.
+//
+// Execution model used by every evaluator kernel in this library: one
+// wavefront owns one expression.  The expression's packed program (one
+// 32-bit word per node, see sr_ops.h) and its constants live in VGPRs with
+// lane k holding node k (and k+64 in a second register for M > 64), so that
+// decoding node k is a wave-uniform v_readlane into SGPRs and the opcode
+// switch is a chain of scalar branches.  Points are mapped to lanes: a tile
+// is 64 consecutive points; the per-node values of the tile (the tape) and
+// the adjoints live in LDS as [node][lane] arrays, which makes every LDS
+// access bank-conflict free.
+#pragma once
+#include "sr_ops.h"
+
+namespace sr {
+
+constexpr int WAVE = 64;
+constexpr int NXREG = 8;   // variables held in registers per point
+constexpr int NCREG = 8;   // constants accumulated in per-lane FP64 registers
+
+__device__ inline int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
+
+__device__ inline uint32_t rl_u32(uint32_t v, int k) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, k);
+}
+__device__ inline float rl(float v, int k) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+__device__ inline double rl(double v, int k) {
+    int64_t b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), k);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double(((int64_t)(uint32_t)lo) | ((int64_t)hi << 32));
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+__device__ inline int32_t wave_max_i32(int32_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        int32_t o = __shfl_xor(v, off, WAVE);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// Program of one expression held in registers (M <= 128).
+template <typename T>
+struct Program {
+    uint32_t w0, w1;   // packed node words, lane k -> node k / node 64 + k
+    T c0, c1;          // constants, same mapping
+    uint64_t cmask0, cmask1;  // LEARNABLE_CONSTANT nodes (wave-uniform)
+    int L;             // effective length: 1 + last non-NO_OP node (0 if none)
+    int root;          // (#non-NO_OP) - 1, clamped at 0
+    int32_t verr;      // validation code of this expression (0, 101, 102)
+
+    __device__ inline uint32_t word(int k) const { return k < WAVE ? rl_u32(w0, k) : rl_u32(w1, k - WAVE); }
+    __device__ inline T cval(int k) const { return k < WAVE ? rl(c0, k) : rl(c1, k - WAVE); }
+    __device__ inline int const_ordinal(int k) const {
+        if (k < WAVE) return __popcll(cmask0 & ((k == 0) ? 0ull : (~0ull >> (WAVE - k))));
+        int kk = k - WAVE;
+        return __popcll(cmask0) + __popcll(cmask1 & ((kk == 0) ? 0ull : (~0ull >> (WAVE - kk))));
+    }
+};
+
+// Validation of one node against the reference's rule
+// (src/evaluation_kernels_cpu.cpp:26-48): every non-null child index must be
+// < k (else 101, checked first), and the number of non-null children must
+// equal get_arity(op) (else 102).
+__device__ inline int32_t validate_node(int64_t op, int64_t c0, int64_t c1, int k) {
+    int actual = 0;
+    if (c0 != -1) { actual++; if (c0 >= k) return ERR_CHILD_GEQ_PARENT; }
+    if (c1 != -1) { actual++; if (c1 >= k) return ERR_CHILD_GEQ_PARENT; }
+    int expected = (op >= SIN && op <= FIFTH) ? 1 : ((op >= ADD && op <= POW) ? 2 : 0);
+    return actual != expected ? ERR_ARITY : 0;
+}
+
+// Loads one expression.  ops/ch/C addresses of node k are given by strides so
+// that both the batch-first (B,M) and the backend's node-first (M,B) layouts
+// are served: node k of expression b is at ops[b*sb + k*sk], its children at
+// ch[(b*sb + k*sk)*2 + i], its constant at C[b*cb + k*ck].
+template <typename T>
+__device__ inline Program<T> load_program(const int64_t *__restrict__ ops, const int64_t *__restrict__ ch,
+                                          const T *__restrict__ C, int64_t b, int64_t sb, int64_t sk,
+                                          int64_t cb, int64_t ck, int M, int n_x) {
+    Program<T> p;
+    const int lane = lane_id();
+    uint32_t w[2] = {0u, 0u};
+    T cv[2] = {T(0), T(0)};
+    int32_t e[2] = {0, 0};
+    bool nonnoop[2] = {false, false};
+    bool isconst[2] = {false, false};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        int k = lane + h * WAVE;
+        if (k < M) {
+            int64_t idx = b * sb + (int64_t)k * sk;
+            int64_t op = ops[idx];
+            int64_t a = ch[idx * 2], c = ch[idx * 2 + 1];
+            w[h] = pack_node(op, a, c, n_x);
+            cv[h] = C ? C[b * cb + (int64_t)k * ck] : T(0);
+            e[h] = validate_node(op, a, c, k);
+            nonnoop[h] = (op != NO_OP);
+            isconst[h] = (op == LEARNABLE_CONSTANT);
+        }
+    }
+    p.w0 = w[0];
+    p.w1 = w[1];
+    p.c0 = cv[0];
+    p.c1 = cv[1];
+    uint64_t nn0 = __ballot(nonnoop[0]), nn1 = __ballot(nonnoop[1]);
+    p.cmask0 = __ballot(isconst[0]);
+    p.cmask1 = __ballot(isconst[1]);
+    int cnt = __popcll(nn0) + __popcll(nn1);
+    p.root = cnt > 0 ? cnt - 1 : 0;
+    p.L = nn1 ? (2 * WAVE - __clzll(nn1)) : (nn0 ? WAVE - __clzll(nn0) : 0);
+    // code of the first failing node in node order
+    uint64_t bad0 = __ballot(e[0] != 0), bad1 = __ballot(e[1] != 0);
+    p.verr = bad0 ? __builtin_amdgcn_readlane(e[0], __ffsll((long long)bad0) - 1)
+                  : (bad1 ? __builtin_amdgcn_readlane(e[1], __ffsll((long long)bad1) - 1) : 0);
+    return p;
+}
+
+// Point-tile context: variable values of this lane's point.
+template <typename T>
+struct PointCtx {
+    T x[NXREG];
+    const T *xrow;   // row pointer for n_x > NXREG
+    int n_x;
+    __device__ inline T var(int v) const {
+        if (n_x <= NXREG) {
+            T r = T(0);
+#pragma unroll
+            for (int q = 0; q < NXREG; ++q) r = (q == v) ? x[q] : r;
+            return r;
+        }
+        return xrow[v];
+    }
+};
+
+template <typename T>
+__device__ inline PointCtx<T> load_point(const T *__restrict__ X, int64_t n, bool active, int n_x) {
+    PointCtx<T> pc;
+    pc.n_x = n_x;
+    pc.xrow = X + (active ? n : 0) * (int64_t)n_x;
+#pragma unroll
+    for (int q = 0; q < NXREG; ++q) pc.x[q] = (active && q < n_x) ? pc.xrow[q] : T(0);
+    return pc;
+}
+
+// Forward sweep of one tile over nodes [0, L): tape[k*64 + lane].
+template <typename T>
+__device__ inline void forward_tile(const Program<T> &p, const PointCtx<T> &pc, T *tape, int L) {
+    const int lane = lane_id();
+    for (int k = 0; k < L; ++k) {
+        uint32_t w = p.word(k);
+        int op = node_op(w);
+        T r;
+        if (op == NO_OP) {
+            r = Mth<T>::qnan();
+        } else if (op == LEARNABLE_CONSTANT) {
+            r = p.cval(k);
+        } else if (op == UNKNOWN_LEAF) {
+            r = T(0);
+        } else if (op >= VAR_START_ID) {
+            r = pc.var(op - VAR_START_ID);
+        } else {
+            int ar = arity_of(op);
+            T a0 = ar >= 1 ? tape[node_c0(w) * WAVE + lane] : T(0);
+            T a1 = ar == 2 ? tape[node_c1(w) * WAVE + lane] : T(0);
+            r = fwd_node<T>(op, a0, a1);
+        }
+        tape[k * WAVE + lane] = r;
+    }
+}
+
+// Per-lane accumulators of one expression over all of its tiles.
+template <typename T>
+struct GradAcc {
+    double cacc[NCREG];  // constants with ordinal < NCREG (FP64 per lane)
+    T gxl[NXREG];        // variable adjoints of the current tile
+};
+
+// Reverse sweep of one tile over nodes [0, L) with adjoints preloaded in adj.
+// `val(k)` returns this lane's forward value of node k (LDS tape or the saved
+// cache).  Constant adjoints go to acc.cacc (first NCREG constants) or, beyond
+// that, to a wave-uniform FP64 LDS slot cextra[k].  Variable adjoints go to
+// acc.gxl[v] (n_x <= NXREG) or straight to gx_global_row with atomics.
+// Inactive lanes (points past N) contribute nothing.  Returns this lane's
+// error code (0, 201 or 204; the larger one if both occur).
+template <typename T, bool WANT_GX, typename Val>
+__device__ inline int32_t backward_tile(const Program<T> &p, Val val, T *adj, int L, bool active,
+                                        GradAcc<T> &acc, double *cextra, int n_x, T *gx_global_row) {
+    const int lane = lane_id();
+    int32_t err = 0;
+    for (int k = L - 1; k >= 0; --k) {
+        uint32_t w = p.word(k);
+        int op = node_op(w);
+        T g = adj[k * WAVE + lane];
+        if (op == NO_OP) {
+            if (active && g != T(0)) err = err > ERR_GRAD_NO_OP ? err : ERR_GRAD_NO_OP;
+            continue;
+        }
+        if (op == LEARNABLE_CONSTANT) {
+            int j = p.const_ordinal(k);
+            double gv = (active && g != T(0)) ? (double)g : 0.0;
+            if (j < NCREG) {
+#pragma unroll
+                for (int q = 0; q < NCREG; ++q)
+                    if (q == j) acc.cacc[q] += gv;
+            } else {
+                double s = wave_sum(gv);
+                if (lane == 0) cextra[k] += s;
+            }
+            continue;
+        }
+        if (op == UNKNOWN_LEAF || op < SIN) continue;  // hard constants and 0-valued leaves
+        if (op >= VAR_START_ID) {
+            if (WANT_GX && active && g != T(0)) {
+                int v = op - VAR_START_ID;
+                if (n_x <= NXREG) {
+#pragma unroll
+                    for (int q = 0; q < NXREG; ++q)
+                        if (q == v) acc.gxl[q] += g;
+                } else {
+                    atomicAdd(&gx_global_row[v], g);
+                }
+            }
+            continue;
+        }
+        int c0 = node_c0(w), c1 = node_c1(w);
+        int ar = arity_of(op);
+        T a0 = val(c0);
+        T a1 = ar == 2 ? val(c1) : T(0);
+        T go0 = T(0), go1 = T(0);
+        bool add0, add1;
+        int32_t e = bwd_node<T>(op, g, a0, a1, go0, add0, go1, add1);
+        if (active && e > err) err = e;
+        if (add0) adj[c0 * WAVE + lane] += go0;
+        if (add1) adj[c1 * WAVE + lane] += go1;
+    }
+    return err;
+}
+
+// Wave-reduces the constant accumulators and returns, in lane k, the FP64
+// gradient of node k (0 for non-constant nodes); `hi` selects nodes 64..127.
+template <typename T>
+__device__ inline void finish_constants(const Program<T> &p, const GradAcc<T> &acc, const double *cextra,
+                                        double &out_lo, double &out_hi) {
+    const int lane = lane_id();
+    out_lo = 0.0;
+    out_hi = 0.0;
+    int j = 0;
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+        uint64_t m = h == 0 ? p.cmask0 : p.cmask1;
+        while (m) {
+            int bit = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            int k = bit + h * WAVE;
+            double s;
+            if (j < NCREG) {
+                double v = 0.0;
+#pragma unroll
+                for (int q = 0; q < NCREG; ++q)
+                    if (q == j) v = acc.cacc[q];
+                s = wave_sum(v);
+            } else {
+                s = cextra[k];
+            }
+            if (lane == bit) {
+                if (h == 0) out_lo = s; else out_hi = s;
+            }
+            ++j;
+        }
+    }
+}
+
+}  // namespace sr

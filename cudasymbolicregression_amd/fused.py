@@ -1,0 +1,104 @@
+"""Root-only fused evaluation (forward + reverse sweep in one pass).
+
+New entry point with no counterpart in the reference API: it computes what
+`optimize`'s closure needs (symbolic_torch/optimization.py:72-99) without
+materialising the (M, N, B) cache of `evaluate`:
+
+    y      (B, N)  value of each expression's root node
+           (root = #non-NO_OP - 1, clamped at 0, as in optimization.py:55-58)
+    loss   (B,)    mean_n (y - Y)^2 in FP64                 [mode 1]
+    grad_C (B, M)  dL/dC summed over the N points in FP64
+    grad_X (N, n_x) dL/dX summed over the batch            [optional]
+    err    (B,)    per-expression code: 101/102 (validation), 201/204 (backward)
+
+mode 0: `dy` is the upstream gradient of y.  mode 1: `dy` is the target Y
+and the upstream gradient is the MSE closure's (2 * (y - Y)) * (1 / N).
+Inputs are batch-first: X (N, n_x), Ops (B, M), Ch (B, M, 2), C (B, M).
+"""
+from __future__ import annotations
+
+from typing import Dict, Optional
+
+import torch
+
+from . import _lib
+from ._C import _require_gpu, get_error_message
+
+
+def fused_root(
+    X: torch.Tensor,
+    Ops: torch.Tensor,
+    Ch: torch.Tensor,
+    C: torch.Tensor,
+    dy: Optional[torch.Tensor],
+    mode: int = 0,
+    want_grad_X: bool = False,
+    fwd_only: bool = False,
+    raise_errors: bool = True,
+    out: Optional[Dict[str, torch.Tensor]] = None,
+) -> Dict[str, torch.Tensor]:
+    _require_gpu(X, "X")
+    dev = X.device
+    dt = X.dtype
+    if dt not in (torch.float32, torch.float64):
+        raise RuntimeError("X must be a floating-point tensor (float or double)")
+    B, M = Ops.shape
+    N, n_x = X.shape
+    if Ch.shape != (B, M, 2) or C.shape != (B, M):
+        raise RuntimeError("Ops (B,M), Ch (B,M,2) and C (B,M) shapes disagree")
+    if M > 128:
+        raise RuntimeError("the gfx950 evaluator supports at most 128 nodes per expression")
+    X = X.contiguous()
+    Ops = Ops.to(device=dev, dtype=torch.int64).contiguous()
+    Ch = Ch.to(device=dev, dtype=torch.int64).contiguous()
+    C = C.to(device=dev, dtype=dt).contiguous()
+    if dy is None:
+        if not (fwd_only and mode == 0):
+            raise RuntimeError("dy (upstream gradient or target) is required")
+        dyc = None
+    else:
+        dyc = dy.to(device=dev, dtype=dt).contiguous()
+        if dyc.shape != (B, N):
+            raise RuntimeError("dy must have shape (B, N)")
+    o = {} if out is None else out
+    y = o.get("y")
+    if y is None:
+        y = torch.empty((B, N), dtype=dt, device=dev)
+    loss = o.get("loss") if mode == 1 else None
+    if mode == 1 and loss is None:
+        loss = torch.empty(B, dtype=torch.float64, device=dev)
+    gC = None
+    if not fwd_only:
+        gC = o.get("grad_C")
+        if gC is None:
+            gC = torch.empty((B, M), dtype=dt, device=dev)
+    gX = torch.empty((N, n_x), dtype=dt, device=dev) if (want_grad_X and not fwd_only) else None
+    err = o.get("err")
+    if err is None:
+        err = torch.empty(B, dtype=torch.int32, device=dev)
+    flags = torch.empty(2, dtype=torch.int32, device=dev)
+    lib = _lib.hip()
+    is64 = dt == torch.float64
+    ws_bytes = lib.sr_fused_workspace_bytes(N, n_x, B, int(gX is not None), int(is64))
+    ws = torch.empty(max(ws_bytes, 256), dtype=torch.uint8, device=dev)
+    fn = lib.sr_fused_root_f64 if is64 else lib.sr_fused_root_f32
+    _lib.check_status(
+        fn(_lib.ptr(X), N, n_x, _lib.ptr(Ops), _lib.ptr(Ch), _lib.ptr(C), M, B, _lib.ptr(dyc), int(mode),
+           int(bool(fwd_only)), _lib.ptr(y), _lib.ptr(loss), _lib.ptr(gC), _lib.ptr(gX), _lib.ptr(err),
+           _lib.ptr(flags), _lib.ptr(ws), ws.numel(), _lib.stream_of(dev)),
+        "fused_root",
+    )
+    res = dict(y=y, err=err, flags=flags)
+    if loss is not None:
+        res["loss"] = loss
+    if gC is not None:
+        res["grad_C"] = gC
+    if gX is not None:
+        res["grad_X"] = gX
+    if raise_errors:
+        f = flags.tolist()
+        if f[0]:
+            raise RuntimeError(get_error_message(f[0], "input validation"))
+        if f[1]:
+            raise RuntimeError(get_error_message(f[1], "symbolic evaluation backward pass"))
+    return res

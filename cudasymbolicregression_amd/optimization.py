@@ -1,0 +1,117 @@
+"""Constant fitting (reference symbolic_torch/optimization.py:9-112).
+
+`optimize` fits the LEARNABLE_CONSTANT slots of every expression to its
+targets by minimising the per-expression mean squared error, like the
+reference's closure + LBFGS_batch.  The whole optimisation of a problem runs
+inside one device-resident L-BFGS kernel (libsr_hip.so, `sr_lbfgs_fit`): one
+wavefront per problem, closures evaluated by the fused root-only
+forward/backward, FP64 losses, inner products and Wolfe tests.
+"""
+from __future__ import annotations
+
+from typing import Dict, Literal, Optional, Tuple, Union
+
+import torch
+
+from . import _lib
+from ._C import _require_gpu, get_error_message
+from .fused import fused_root
+
+_LINE_SEARCH = {"strong_wolfe": 0, "backtracking": 1}
+
+
+def _fit(X, Y, Ops, Ch, C_init, max_iter, history_size, tolerance_grad, line_search_fn, stop_loss=None,
+         x_off=None, y_off=None, n_pts=None, line_search_max_iter: int = 10, eps: float = 1e-10):
+    dev = Ops.device
+    B, M = Ops.shape
+    if line_search_fn not in _LINE_SEARCH:
+        raise ValueError(f"Invalid line_search_fn: {line_search_fn}. Choose 'strong_wolfe' or 'backtracking'.")
+    if not 1 <= int(history_size) <= 16:
+        raise ValueError("history_size must be between 1 and 16 for the device L-BFGS")
+    if M > 128:
+        raise RuntimeError("the gfx950 evaluator supports at most 128 nodes per expression")
+    C_out = torch.empty((B, M), dtype=torch.float32, device=dev)
+    loss = torch.empty(B, dtype=torch.float64, device=dev)
+    iters = torch.empty(B, dtype=torch.int32, device=dev)
+    err = torch.empty(B, dtype=torch.int32, device=dev)
+    flags = torch.empty(2, dtype=torch.int32, device=dev)
+    lib = _lib.hip()
+    ws = torch.empty(max(int(lib.sr_lbfgs_workspace_bytes(B, M)), 256), dtype=torch.uint8, device=dev)
+    N = X.shape[0]
+    n_x = X.shape[1]
+    _lib.check_status(
+        lib.sr_lbfgs_fit(
+            _lib.ptr(X), _lib.ptr(Y), N, n_x, _lib.ptr(Ops), _lib.ptr(Ch), _lib.ptr(C_init), M, B,
+            _lib.ptr(x_off), _lib.ptr(y_off), _lib.ptr(n_pts), _lib.ptr(stop_loss),
+            _lib.ptr(C_out), _lib.ptr(loss), _lib.ptr(iters), _lib.ptr(err), _lib.ptr(flags),
+            int(max_iter), int(history_size), float(tolerance_grad), _LINE_SEARCH[line_search_fn],
+            int(line_search_max_iter), float(eps), _lib.ptr(ws), ws.numel(), _lib.stream_of(dev),
+        ),
+        "sr_lbfgs_fit",
+    )
+    return C_out, loss, iters, err, flags
+
+
+def optimize(
+    X: torch.Tensor,
+    Y: torch.Tensor,
+    Ops: torch.Tensor,
+    Ch: torch.Tensor,
+    C_init: torch.Tensor,
+    max_iter: int = 100,
+    history_size: int = 10,
+    tolerance_grad: float = 1e-7,
+    line_search_fn: Literal["strong_wolfe", "backtracking"] = "strong_wolfe",
+    *,
+    stop_rel_loss: Optional[float] = None,
+    raise_errors: bool = True,
+    return_info: bool = False,
+) -> Union[Tuple[torch.Tensor, torch.Tensor], Tuple[torch.Tensor, torch.Tensor, Dict[str, torch.Tensor]]]:
+    """Optimises the constants of a batch of expressions with batched L-BFGS.
+
+    Args (as in the reference):
+        X (N, n_x), Y (B, N), Ops (B, M), Ch (B, M, 2), C_init (B, M);
+        max_iter, history_size, tolerance_grad, line_search_fn.
+    Extensions (keyword-only):
+        stop_rel_loss: also stop a problem once its loss <= stop_rel_loss * Var(Y_b).
+        raise_errors: raise like the reference on validation / backward
+            errors (201, 204); if False, failing problems are reported in
+            info["err"] and the others are still fitted.
+        return_info: also return dict(loss (B,) FP64, iters (B,), err (B,)).
+
+    Returns:
+        C_final (B, M) and Y_final (B, N): the fitted constants (non-constant
+        slots keep C_init's values) and each expression's prediction.
+    """
+    _require_gpu(X, "X")
+    dev = X.device
+    if X.dtype != torch.float32:
+        raise RuntimeError("optimize fits FP32 constants: X must be a float32 tensor")
+    for t, name in ((Y, "Y"), (Ops, "Ops"), (Ch, "Ch"), (C_init, "C_init")):
+        if t.device != dev:
+            raise RuntimeError("All input tensors must be on the same device.")
+    B, M = Ops.shape
+    Xc = X.contiguous()
+    Yc = Y.to(torch.float32).contiguous()
+    Opsc = Ops.to(torch.int64).contiguous()
+    Chc = Ch.to(torch.int64).contiguous()
+    Cc = C_init.to(torch.float32).contiguous()
+    if Yc.shape != (B, X.shape[0]):
+        raise RuntimeError("Y must have shape (B, N)")
+    stop = None
+    if stop_rel_loss is not None:
+        var = Yc.double().var(dim=1, unbiased=False)
+        var = torch.where(var > 0, var, (Yc.double() ** 2).mean(dim=1))
+        stop = (float(stop_rel_loss) * var).contiguous()
+    C_out, loss, iters, err, flags = _fit(Xc, Yc, Opsc, Chc, Cc, max_iter, history_size, tolerance_grad,
+                                          line_search_fn, stop_loss=stop)
+    if raise_errors:
+        f = flags.tolist()
+        if f[0]:
+            raise RuntimeError(get_error_message(f[0], "input validation"))
+        if f[1]:
+            raise RuntimeError(get_error_message(f[1], "symbolic evaluation backward pass"))
+    Y_final = fused_root(Xc, Opsc, Chc, C_out, None, mode=0, fwd_only=True, raise_errors=False)["y"]
+    if return_info:
+        return C_out, Y_final, dict(loss=loss, iters=iters, err=err)
+    return C_out, Y_final

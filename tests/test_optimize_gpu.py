@@ -1,0 +1,124 @@
+"""GPU tests of `optimize` (device-resident L-BFGS) against the reference's
+known answers and against the CPU restatement of LBFGS_batch (oracle).
+
+Stated parity tolerance (FP64 reductions in the product vs FP32 in the
+reference make exact trajectories impossible): per problem
+|MSE_gpu - MSE_oracle| <= 1e-3 * MSE_oracle + 1e-7, required for at least
+90% of the problems where both runs end finite; the fraction is printed."""
+import numpy as np
+import pytest
+import torch
+
+import helpers
+import lbfgs_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev():
+    return torch.device("cuda:0")
+
+
+def _t(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(_dev())
+
+
+@pytest.mark.parametrize("line_search", ["strong_wolfe", "backtracking"])
+def test_optimize_linear_known_answer(line_search):
+    """y = c1*X0 + c2 (reference tests/test_optimization.py:10-80)."""
+    import cudasymbolicregression_amd as st
+
+    torch.manual_seed(0)
+    B, N, M = 2, 100, 5
+    c1 = torch.tensor([2.5, -1.5], device=_dev()).view(B, 1)
+    c2 = torch.tensor([0.5, 3.0], device=_dev()).view(B, 1)
+    X = torch.randn(N, 1, device=_dev())
+    Y = c1 * X[:, 0].T + c2
+    Ops = torch.full((B, M), 0, dtype=torch.int64, device=_dev())
+    Ops[:, 0] = 50
+    Ops[:, 1] = 1
+    Ops[:, 2] = 42
+    Ops[:, 3] = 1
+    Ops[:, 4] = 40
+    Ch = torch.full((B, M, 2), -1, dtype=torch.int64, device=_dev())
+    Ch[:, 2, 0], Ch[:, 2, 1] = 0, 1
+    Ch[:, 4, 0], Ch[:, 4, 1] = 2, 3
+    C0 = st.create_constants(Ops, method="randn")
+    Cf, Yf = st.optimize(X, Y, Ops, Ch, C0, line_search_fn=line_search, max_iter=300)
+    assert torch.allclose(Cf[:, 1], c1.squeeze(), atol=1e-3)
+    assert torch.allclose(Cf[:, 3], c2.squeeze(), atol=1e-3)
+    mask = Ops == 1
+    assert torch.equal(Cf[~mask], C0[~mask])
+    assert torch.allclose(Yf, Y, atol=1e-3)
+
+
+def test_optimize_sin_batch_reduces_loss():
+    """c1*sin(X0) + c2 over 2000 problems (reference tests/test_optimization.py:82-169)."""
+    import cudasymbolicregression_amd as st
+
+    torch.manual_seed(1)
+    B, N, M = 2000, 50, 6
+    c1 = torch.randn(B, 1, device=_dev()) * 2
+    c2 = torch.randn(B, 1, device=_dev())
+    X = torch.rand(N, 1, device=_dev()) * 4 - 2
+    Y = c1 * torch.sin(X[:, 0]).T + c2
+    Ops = torch.zeros((B, M), dtype=torch.int64, device=_dev())
+    Ops[:, :5] = torch.tensor([50, 10, 1, 42, 1], device=_dev())
+    Ops[:, 5] = 40
+    Ch = torch.full((B, M, 2), -1, dtype=torch.int64, device=_dev())
+    Ch[:, 1, 0] = 0
+    Ch[:, 3, 0], Ch[:, 3, 1] = 1, 2
+    Ch[:, 5, 0], Ch[:, 5, 1] = 3, 4
+    C0 = st.create_constants(Ops)
+    y0 = st.evaluation.evaluate(X, Ops, Ch, C0)[:, 5, :]
+    init = ((y0 - Y) ** 2).mean(1)
+    Cf, Yf, info = st.optimize(X, Y, Ops, Ch, C0, return_info=True)
+    final = ((Yf - Y) ** 2).mean(1)
+    assert (final < 0.1 * init + 1e-10).all()
+    assert torch.allclose(Cf[:, 2], c1.squeeze(), atol=1e-3)
+    assert torch.allclose(Cf[:, 4], c2.squeeze(), atol=1e-3)
+    assert (info["err"] == 0).all()
+
+
+@pytest.mark.parametrize("seed,M", [(0, 16), (1, 32), (2, 64)])
+def test_optimize_matches_oracle_lbfgs(seed, M):
+    import cudasymbolicregression_amd as st
+
+    rng = np.random.default_rng(seed)
+    B, N, n_x = 96, 64, 2
+    ops, ch = helpers.random_batch(seed, B, M, n_x, unary=[10, 11, 12, 14], binary=[40, 41, 42],
+                                   leaf_const_p=0.4, depth=5)
+    X = rng.uniform(-2, 2, (N, n_x)).astype(np.float32)
+    C_true = helpers.constants_like(ops, rng, scale=1.0)
+    Y = lbfgs_oracle.oracle.fused_root(X, ops, ch, C_true, np.zeros((B, N), np.float32), mode=0)["y"]
+    keep = np.isfinite(Y).all(1)
+    C0 = helpers.constants_like(ops, rng, scale=1.0)
+    C_o, y_o, f_o, _ = lbfgs_oracle.optimize(X, Y, ops, ch, C0, max_iter=50)
+    Cg, Yg, info = st.optimize(_t(X), _t(Y), _t(ops), _t(ch), _t(C0), max_iter=50, return_info=True,
+                               raise_errors=False)
+    mse_g = ((Yg.cpu().numpy().astype(np.float64) - Y) ** 2).mean(1)
+    mse_o = ((y_o.astype(np.float64) - Y) ** 2).mean(1)
+    both = keep & np.isfinite(mse_g) & np.isfinite(mse_o) & (info["err"].cpu().numpy() == 0)
+    close = np.abs(mse_g - mse_o) <= 1e-3 * mse_o + 1e-7
+    frac = close[both].mean()
+    print(f"M={M}: {both.sum()} finite problems, {frac:.3f} within tolerance; "
+          f"median |dC| {np.nanmedian(np.abs(Cg.cpu().numpy() - C_o)[ops == 1]):.2e}")
+    assert both.sum() > B // 2
+    assert frac >= 0.9
+
+
+def test_optimize_raises_div_by_zero():
+    import cudasymbolicregression_amd as st
+
+    # 1 / (X0 - X0) has an exact zero denominator at every point -> 204
+    X = torch.randn(16, 1, device=_dev())
+    Y = torch.randn(1, 16, device=_dev())
+    Ops = torch.tensor([[1, 50, 50, 41, 43]], dtype=torch.int64, device=_dev())
+    Ch = torch.full((1, 5, 2), -1, dtype=torch.int64, device=_dev())
+    Ch[0, 3, 0], Ch[0, 3, 1] = 1, 2
+    Ch[0, 4, 0], Ch[0, 4, 1] = 0, 3
+    C0 = torch.ones(1, 5, device=_dev())
+    with pytest.raises(RuntimeError, match="a/b where b = 0"):
+        st.optimize(X, Y, Ops, Ch, C0)
+    Cf, Yf, info = st.optimize(X, Y, Ops, Ch, C0, raise_errors=False, return_info=True)
+    assert int(info["err"][0]) == 204
