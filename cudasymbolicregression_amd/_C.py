@@ -192,7 +192,9 @@ class _SymbolicEvaluation(torch.autograd.Function):
             )
         M, B = Ops.shape
         N, n_x = X.shape
-        g = grad_output.to(dtype=cache.dtype).contiguous()
+        # grad_cache: a private copy of the upstream gradient that the kernel
+        # turns into the adjoints of every node in place.
+        g = grad_output.to(dtype=cache.dtype).contiguous().clone()
         grad_C = torch.empty_like(C)
         want_x = ctx.needs_input_grad[0]
         grad_X = torch.empty_like(X) if want_x else None
@@ -296,7 +298,7 @@ class _SymbolicEvaluationMultiple(torch.autograd.Function):
             raise RuntimeError("Cache and grad_output must have the same shape.")
         M, B, K = C.shape
         N, n_x = X.shape
-        g = grad_output.to(dtype=cache.dtype).contiguous()
+        g = grad_output.to(dtype=cache.dtype).contiguous().clone()
         grad_C = torch.empty_like(C)
         want_x = ctx.needs_input_grad[0]
         grad_X = torch.empty_like(X) if want_x else None

@@ -68,7 +68,7 @@ def hip() -> ctypes.CDLL:
                      [P, P, P, P, I64, I64, I64, I64, I64, P, P, P, I64, P, P])
         _declare(lib, "sr_backward_mb_workspace_bytes", I64, [I64, I64, I64, I])
         _declare(lib, "sr_multi_backward_mb_workspace_bytes", I64, [I64, I64, I64, I64, I])
-        _declare(lib, "sr_fused_workspace_bytes", I64, [I64, I64, I64, I, I])
+        _declare(lib, "sr_fused_workspace_bytes", I64, [I64, I64, I64, I64, I, I])
         _declare(lib, "sr_lbfgs_fit", I,
                  [P, P, I64, I64, P, P, P, I64, I64, P, P, P, P, P, P, P, P, P, I, I, c_double, I, I, c_double,
                   P, I64, P])
@@ -115,4 +115,6 @@ def check_status(code: int, what: str) -> None:
             raise RuntimeError(f"{what}: invalid arguments (sizes out of the supported range)")
         if code == -2:
             raise RuntimeError(f"{what}: workspace too small")
+        if code == -3:
+            raise RuntimeError(f"{what}: the launch would need more LDS than one workgroup may use")
         raise RuntimeError(f"{what}: HIP error {code}")

@@ -19,6 +19,7 @@
 // node-first tensors Ops (M,B), Ch (M,B,2), C (M,B), cache (M,N,B).  The fused
 // entry points take batch-first Ops (B,M), Ch (B,M,2), C (B,M), Y/dy (B,N).
 #include "sr_wave.h"
+#include "sr_device_info.h"
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -68,49 +69,55 @@ template <typename T>
 __global__ __launch_bounds__(64) void k_full_forward(const T *__restrict__ X, int64_t N, int n_x,
                                                      const int64_t *__restrict__ ops,
                                                      const int64_t *__restrict__ ch, const T *__restrict__ C,
-                                                     int M, int64_t B, int64_t n_tiles, T *__restrict__ cache) {
+                                                     int M, int64_t B, int64_t K, int64_t n_tiles,
+                                                     T *__restrict__ cache) {
+    // One wave per (tile, expression e = b*K + j): constant set j of expression b.
     extern __shared__ __align__(16) unsigned char smem[];
     T *tape = reinterpret_cast<T *>(smem);
+    const int64_t E = B * K;
     int64_t id = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
-    int64_t tile = id / B, b = id % B;
+    int64_t tile = id / E, e = id % E;
     if (tile >= n_tiles) return;
+    const int64_t b = e / K, j = e % K;
     const int lane = lane_id();
-    Program<T> p = load_program<T>(ops, ch, C, b, 1, B, 1, B, M, n_x);
+    Program<T> p = load_program<T>(ops, ch, C + j, b, 1, B, K, B * K, M, n_x);
     int64_t n = tile * TILE + lane;
     bool active = n < N;
     PointCtx<T> pc = load_point<T>(X, n, active, n_x);
     forward_tile<T>(p, pc, tape, p.L);
     if (!active) return;
-    for (int k = 0; k < p.L; ++k) cache[((int64_t)k * N + n) * B + b] = tape[k * WAVE + lane];
+    for (int k = 0; k < p.L; ++k) cache[(((int64_t)k * N + n) * B + b) * K + j] = tape[k * WAVE + lane];
     const T nan = Mth<T>::qnan();
-    for (int k = p.L; k < M; ++k) cache[((int64_t)k * N + n) * B + b] = nan;
+    for (int k = p.L; k < M; ++k) cache[(((int64_t)k * N + n) * B + b) * K + j] = nan;
 }
 
 // ---------------------------------------------------------------------------
 // Full-cache backward.  Block = 4 waves = 4 consecutive expressions, looping
-// over all point tiles together.  Children values come from the saved cache,
-// adjoints live in LDS.  grad_C is reduced per wave in FP64 over all points;
-// grad_X is summed over the block's 4 expressions in a fixed order per tile
-// and written as a per-block partial, which k_reduce_partials sums in FP64.
+// over all point tiles together.  Children values come from the saved cache;
+// the adjoints are accumulated in place in grad_cache, the caller's copy of
+// the upstream gradient (the reference's grad_cache, symbolic_evaluation.cpp:
+// 114-115): every element is owned by one lane, so no atomics are needed.
+// grad_C is reduced per wave in FP64 over all points; grad_X is summed over
+// the block's 4 expressions in a fixed order per tile and written as a
+// per-block partial, which k_reduce_partials sums in FP64.
 // ---------------------------------------------------------------------------
 constexpr int BWD_WAVES = 4;
 
 template <typename T>
 __global__ __launch_bounds__(BWD_WAVES * 64) void k_full_backward(
-    const T *__restrict__ grad_out, const T *__restrict__ cache, const int64_t *__restrict__ ops,
-    const int64_t *__restrict__ ch, int M, int64_t B, int64_t N, int n_x, T *__restrict__ grad_C,
+    T *__restrict__ grad_cache, const T *__restrict__ cache, const int64_t *__restrict__ ops,
+    const int64_t *__restrict__ ch, int M, int64_t B, int64_t K, int64_t N, int n_x, T *__restrict__ grad_C,
     T *__restrict__ gx_partial, T *__restrict__ grad_X_direct, int32_t *err_flag) {
+    // Wave w of block i handles expression e = 4*i + w, i.e. constant set
+    // j = e % K of expression b = e / K (K = 1 for the single-set layout).
     extern __shared__ __align__(16) unsigned char smem[];
     const int wave = threadIdx.x / WAVE;
     const int lane = lane_id();
-    T *adj = reinterpret_cast<T *>(smem) + (size_t)wave * M * WAVE;
-    double *cextra = reinterpret_cast<double *>(reinterpret_cast<T *>(smem) + (size_t)BWD_WAVES * M * WAVE) +
-                     (size_t)wave * M;
-    T *gxs = reinterpret_cast<T *>(reinterpret_cast<double *>(reinterpret_cast<T *>(smem) +
-                                                              (size_t)BWD_WAVES * M * WAVE) +
-                                   (size_t)BWD_WAVES * M);  // [BWD_WAVES][64][NXREG]
-    const int64_t b = (int64_t)blockIdx.x * BWD_WAVES + wave;
-    const bool bvalid = b < B;
+    double *cextra = reinterpret_cast<double *>(smem) + (size_t)wave * M;
+    T *gxs = reinterpret_cast<T *>(reinterpret_cast<double *>(smem) + (size_t)BWD_WAVES * M);  // [4][64][NXREG]
+    const int64_t e_id = (int64_t)blockIdx.x * BWD_WAVES + wave;
+    const bool bvalid = e_id < B * K;
+    const int64_t b = bvalid ? e_id / K : 0, j = bvalid ? e_id % K : 0;
     const bool want_gx = gx_partial != nullptr || grad_X_direct != nullptr;
     Program<T> p;
     if (bvalid) p = load_program<T>(ops, ch, (const T *)nullptr, b, 1, B, 1, B, M, n_x);
@@ -124,16 +131,17 @@ __global__ __launch_bounds__(BWD_WAVES * 64) void k_full_backward(
     for (int64_t t = 0; t < n_tiles; ++t) {
         const int64_t n = t * TILE + lane;
         const bool active = bvalid && n < N;
+        const int64_t nn = active ? n : 0;
 #pragma unroll
         for (int q = 0; q < NXREG; ++q) acc.gxl[q] = T(0);
-        for (int k = 0; k < M; ++k) {
-            T g = active ? grad_out[((int64_t)k * N + n) * B + b] : T(0);
-            if (k < p.L) adj[k * WAVE + lane] = g;
-            else if (g != T(0)) err = err > ERR_GRAD_NO_OP ? err : ERR_GRAD_NO_OP;
+        for (int k = p.L; k < M; ++k) {
+            T g = active ? grad_cache[(((int64_t)k * N + n) * B + b) * K + j] : T(0);
+            if (g != T(0)) err = err > ERR_GRAD_NO_OP ? err : ERR_GRAD_NO_OP;
         }
-        auto val = [&](int k) -> T { return active ? cache[((int64_t)k * N + n) * B + b] : T(0); };
+        auto val = [&](int k) -> T { return cache[(((int64_t)k * N + nn) * B + b) * K + j]; };
+        auto adj = [&](int k) -> T & { return grad_cache[(((int64_t)k * N + nn) * B + b) * K + j]; };
         int32_t e = backward_tile<T, true>(p, val, adj, p.L, active, acc, cextra, n_x,
-                                           grad_X_direct ? grad_X_direct + (active ? n : 0) * n_x : nullptr);
+                                           grad_X_direct ? grad_X_direct + nn * n_x : nullptr);
         err = e > err ? e : err;
         if (want_gx && n_x <= NXREG) {
 #pragma unroll
@@ -152,8 +160,8 @@ __global__ __launch_bounds__(BWD_WAVES * 64) void k_full_backward(
     if (!bvalid) return;
     double lo, hi;
     finish_constants<T>(p, acc, cextra, lo, hi);
-    if (lane < M) grad_C[(int64_t)lane * B + b] = (T)lo;
-    if (lane + WAVE < M) grad_C[(int64_t)(lane + WAVE) * B + b] = (T)hi;
+    if (lane < M) grad_C[((int64_t)lane * B + b) * K + j] = (T)lo;
+    if (lane + WAVE < M) grad_C[((int64_t)(lane + WAVE) * B + b) * K + j] = (T)hi;
     err = wave_max_i32(err);
     if (lane == 0 && err) atomicMax(err_flag, err);
 }
@@ -212,20 +220,22 @@ struct FusedArgs {
     int32_t *queue_count;
     int32_t *work_counter;
     int64_t per_blk;
+    void *gtape;          // global tape/adjoint slots when they do not fit LDS
 };
 
-template <typename T, int LMAX, int W, int MODE, bool FWD_ONLY, bool WANT_GX>
+template <typename T, int LMAX, int W, int MODE, bool FWD_ONLY, bool WANT_GX, bool GT>
 __global__ __launch_bounds__(W * 64) void k_fused(FusedArgs a) {
+    // GT: the tape and adjoints of each wave live in a global workspace slot
+    // (used when LMAX is too long for LDS); otherwise in LDS.
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int blk_next;
-    const int wave = threadIdx.x / WAVE;
+    const int wave = uniform_i32((int)(threadIdx.x / WAVE));
     const int lane = lane_id();
-    T *tape = reinterpret_cast<T *>(smem) + (size_t)wave * 2 * LMAX * WAVE;
-    T *adj = tape + LMAX * WAVE;
-    double *cextra = reinterpret_cast<double *>(reinterpret_cast<T *>(smem) + (size_t)W * 2 * LMAX * WAVE) +
-                     (size_t)wave * LMAX;
-    T *gx_blk = reinterpret_cast<T *>(reinterpret_cast<double *>(reinterpret_cast<T *>(smem) +
-                                                                 (size_t)W * 2 * LMAX * WAVE) +
+    constexpr size_t TAPE_LDS = GT ? 0 : (size_t)W * 2 * LMAX * WAVE;
+    T *tape = GT ? (T *)a.gtape + ((size_t)blockIdx.x * W + wave) * 2 * LMAX * WAVE
+                 : reinterpret_cast<T *>(smem) + (size_t)wave * 2 * LMAX * WAVE;
+    T *adjp = tape + LMAX * WAVE;
+    double *cextra = reinterpret_cast<double *>(reinterpret_cast<T *>(smem) + TAPE_LDS) + (size_t)wave * LMAX;
+    T *gx_blk = reinterpret_cast<T *>(reinterpret_cast<double *>(reinterpret_cast<T *>(smem) + TAPE_LDS) +
                                       (size_t)W * LMAX);
     const T *X = (const T *)a.X;
     const T *C = (const T *)a.C;
@@ -234,13 +244,15 @@ __global__ __launch_bounds__(W * 64) void k_fused(FusedArgs a) {
     T *grad_C = (T *)a.grad_C;
     const bool gx_lds = WANT_GX && a.gx_partial != nullptr;
     const int64_t gx_len = a.N * a.n_x;
-    if (threadIdx.x == 0) blk_next = 0;
     if (gx_lds)
         for (int64_t i = threadIdx.x; i < gx_len; i += W * WAVE) gx_blk[i] = T(0);
     __syncthreads();
 
     const int64_t n_tiles = (a.N + TILE - 1) / TILE;
     const T inv_n = T(1.0) / T(a.N);
+    // Phase 1: block i owns [i*per_blk, ...) and wave w takes its slots
+    // w, w+W, ...  Phase 2: waves take queue slots from a global counter.
+    // Either way the slot index is a wave-uniform SGPR value.
     int64_t base = 0, count = 0;
     if (a.list == nullptr) {
         base = (int64_t)blockIdx.x * a.per_blk;
@@ -249,10 +261,17 @@ __global__ __launch_bounds__(W * 64) void k_fused(FusedArgs a) {
     } else {
         count = *a.list_count;
     }
+    int64_t slot = wave;
     for (;;) {
-        int i = 0;
-        if (lane == 0) i = (a.list == nullptr) ? atomicAdd(&blk_next, 1) : atomicAdd(a.work_counter, 1);
-        i = __shfl(i, 0, WAVE);
+        int64_t i;
+        if (a.list == nullptr) {
+            i = slot;
+            slot += W;
+        } else {
+            int got = 0;
+            if (lane == 0) got = atomicAdd(a.work_counter, 1);
+            i = uniform_i32(got);
+        }
         if (i >= count) break;
         const int64_t b = (a.list == nullptr) ? base + i : (int64_t)a.list[i];
         Program<T> p = load_program<T>(a.ops, a.ch, C, b, a.M, 1, a.M, 1, a.M, a.n_x);
@@ -299,11 +318,12 @@ __global__ __launch_bounds__(W * 64) void k_fused(FusedArgs a) {
                 if (active && g0 != T(0)) err = ERR_GRAD_NO_OP;
                 continue;
             }
-            for (int k = 0; k < L; ++k) adj[k * WAVE + lane] = T(0);
-            adj[p.root * WAVE + lane] = active ? g0 : T(0);
+            for (int k = 0; k < L; ++k) adjp[k * WAVE + lane] = T(0);
+            adjp[p.root * WAVE + lane] = active ? g0 : T(0);
 #pragma unroll
             for (int q = 0; q < NXREG; ++q) acc.gxl[q] = T(0);
             auto val = [&](int k) -> T { return tape[k * WAVE + lane]; };
+            auto adj = [&](int k) -> T & { return adjp[k * WAVE + lane]; };
             int32_t e = backward_tile<T, WANT_GX>(p, val, adj, L, active, acc, cextra, a.n_x,
                                                   (T *)a.grad_X_direct + (active ? n : 0) * a.n_x);
             err = e > err ? e : err;
@@ -342,48 +362,46 @@ __global__ __launch_bounds__(W * 64) void k_fused(FusedArgs a) {
     }
 }
 
-int g_num_cus = 0;
-int num_cus() {
-    if (g_num_cus == 0) {
-        int dev = 0;
-        (void)hipGetDevice(&dev);
-        int v = 0;
-        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
-        g_num_cus = v;
-    }
-    return g_num_cus;
-}
-
 inline int ret_last() { return (int)hipGetLastError(); }
 
+constexpr int P1_LMAX = 16;
+constexpr size_t GX_LDS_MAX = 24 * 1024;
+
+template <typename T> constexpr int p1_waves() { return sizeof(T) == 4 ? 4 : 2; }
+
 template <typename T>
-size_t fused_lds_bytes(int LMAX, int W, bool gx_lds, int64_t gx_len) {
-    return (size_t)W * 2 * LMAX * WAVE * sizeof(T) + (size_t)W * LMAX * sizeof(double) +
+size_t fused_lds_bytes(int LMAX, int W, bool gt, bool gx_lds, int64_t gx_len) {
+    return (gt ? 0 : (size_t)W * 2 * LMAX * WAVE * sizeof(T)) + (size_t)W * LMAX * sizeof(double) +
            (gx_lds ? (size_t)gx_len * sizeof(T) : 0);
 }
-
-constexpr int P1_LMAX = 16, P1_W = 4;
-constexpr int P2_W = 2;
-constexpr size_t GX_LDS_MAX = 32 * 1024;
+template <typename T>
+bool needs_global_tape(int LMAX, int W, bool gx_lds, int64_t gx_len) {
+    return fused_lds_bytes<T>(LMAX, W, false, gx_lds, gx_len) > lds_limit();
+}
 
 template <typename T, int LMAX, int W, int MODE, bool FWD, bool GX>
 int launch_fused_one(FusedArgs a, int64_t grid, hipStream_t s) {
     bool gx_lds = GX && a.gx_partial != nullptr;
-    size_t lds = fused_lds_bytes<T>(LMAX, W, gx_lds, a.N * a.n_x);
-    auto kern = k_fused<T, LMAX, W, MODE, FWD, GX>;
-    if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return (int)e;
+    bool gt = needs_global_tape<T>(LMAX, W, gx_lds, a.N * a.n_x);
+    size_t lds = fused_lds_bytes<T>(LMAX, W, gt, gx_lds, a.N * a.n_x);
+    if (lds > lds_limit()) return -3;
+    if (gt) {
+        if (a.gtape == nullptr) return -2;
+        hipLaunchKernelGGL((k_fused<T, LMAX, W, MODE, FWD, GX, true>), dim3((unsigned)grid), dim3(W * 64), lds, s, a);
+    } else {
+        hipLaunchKernelGGL((k_fused<T, LMAX, W, MODE, FWD, GX, false>), dim3((unsigned)grid), dim3(W * 64), lds, s,
+                           a);
     }
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(W * 64), lds, s, a);
     return ret_last();
 }
 
+inline int p2_lmax(int64_t M) { return M <= 32 ? 32 : (M <= 64 ? 64 : 128); }
+
 template <typename T, int MODE, bool FWD, bool GX>
 int launch_fused_phase(FusedArgs a, bool phase2, int lmax2, int64_t grid, hipStream_t s) {
-    if (!phase2) return launch_fused_one<T, P1_LMAX, P1_W, MODE, FWD, GX>(a, grid, s);
-    if (lmax2 <= 32) return launch_fused_one<T, 32, P2_W, MODE, FWD, GX>(a, grid, s);
-    if (lmax2 <= 64) return launch_fused_one<T, 64, P2_W, MODE, FWD, GX>(a, grid, s);
+    if (!phase2) return launch_fused_one<T, P1_LMAX, p1_waves<T>(), MODE, FWD, GX>(a, grid, s);
+    if (lmax2 <= 32) return launch_fused_one<T, 32, 1, MODE, FWD, GX>(a, grid, s);
+    if (lmax2 <= 64) return launch_fused_one<T, 64, 1, MODE, FWD, GX>(a, grid, s);
     return launch_fused_one<T, 128, 1, MODE, FWD, GX>(a, grid, s);
 }
 
@@ -404,13 +422,39 @@ int launch_fused_disp(FusedArgs a, int mode, bool fwd_only, bool want_gx, bool p
     return launch_fused_gx<T, 0, false>(a, want_gx, phase2, lmax2, grid, s);
 }
 
-// Grid sizes of the two fused phases (persistent blocks).
+// Grid sizes of the two fused phases (persistent blocks; phase-2 blocks hold
+// one wave each).
+template <typename T>
 void fused_grids(int64_t B, int64_t &g1, int64_t &g2) {
     int64_t cus = num_cus();
-    g1 = cus * 4;
-    int64_t need = (B + P1_W - 1) / P1_W;
+    const int w1 = p1_waves<T>();
+    g1 = cus * (sizeof(T) == 4 ? 4 : 8);
+    int64_t need = (B + w1 - 1) / w1;
     if (g1 > need) g1 = need > 0 ? need : 1;
-    g2 = cus * 2;
+    g2 = cus * 8;
+}
+
+// Workspace layout: [queue: B int32][4 counters] | gx partial rows | global tape.
+struct FusedWs {
+    size_t gx_off, gt_off, total;
+    bool gx_lds, gt;
+};
+template <typename T>
+FusedWs fused_ws_layout(int64_t N, int64_t n_x, int64_t B, int64_t M, bool want_gx) {
+    int64_t g1, g2;
+    fused_grids<T>(B, g1, g2);
+    FusedWs w;
+    const int64_t gx_len = N * n_x;
+    w.gx_lds = want_gx && n_x <= NXREG && (size_t)gx_len * sizeof(T) <= GX_LDS_MAX;
+    w.gt = M > P1_LMAX && needs_global_tape<T>(p2_lmax(M), 1, w.gx_lds, gx_len);
+    size_t off = (((size_t)B * 4 + 16) + 255) / 256 * 256;
+    w.gx_off = off;
+    if (w.gx_lds) off += (size_t)(g1 + g2) * gx_len * sizeof(T);
+    off = (off + 255) / 256 * 256;
+    w.gt_off = off;
+    if (w.gt) off += (size_t)g2 * 2 * p2_lmax(M) * WAVE * sizeof(T);
+    w.total = off + 256;
+    return w;
 }
 
 template <typename T>
@@ -420,22 +464,19 @@ int fused_root(const T *X, int64_t N, int64_t n_x, const int64_t *ops, const int
     if (M <= 0 || M > 128 || B < 0 || N <= 0 || n_x < 0) return -1;
     if (B == 0) return 0;
     int64_t g1, g2;
-    fused_grids(B, g1, g2);
+    fused_grids<T>(B, g1, g2);
     const int64_t gx_len = N * n_x;
     const bool want_gx = grad_X != nullptr && !fwd_only;
-    const bool gx_lds = want_gx && n_x <= NXREG && (size_t)gx_len * sizeof(T) <= GX_LDS_MAX;
-    // workspace: queue (B int32), 2 counters, then gx partials (g1+g2 rows)
-    size_t need = (size_t)B * 4 + 16 + (gx_lds ? (size_t)(g1 + g2) * gx_len * sizeof(T) : 0);
-    if (workspace_bytes < (int64_t)need) return -2;
+    FusedWs L = fused_ws_layout<T>(N, n_x, B, M, want_gx);
+    if (workspace_bytes < (int64_t)L.total) return -2;
     char *ws = (char *)workspace;
     int32_t *queue = (int32_t *)ws;
     int32_t *counters = (int32_t *)(ws + (size_t)B * 4);
-    T *gx_part = gx_lds ? (T *)(ws + (((size_t)B * 4 + 16 + 255) / 256) * 256) : nullptr;
-    if (gx_lds && (char *)(gx_part + (g1 + g2) * gx_len) > ws + workspace_bytes) return -2;
+    T *gx_part = L.gx_lds ? (T *)(ws + L.gx_off) : nullptr;
 
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, counters, 0, (int64_t)4);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, flags, 0, (int64_t)2);
-    if (want_gx && !gx_lds) {
+    if (want_gx && !L.gx_lds) {
         // direct atomics into grad_X
         int64_t nb = (gx_len + 255) / 256;
         if (sizeof(T) == 4)
@@ -447,14 +488,15 @@ int fused_root(const T *X, int64_t N, int64_t n_x, const int64_t *ops, const int
     memset(&a, 0, sizeof(a));
     a.X = X; a.N = N; a.n_x = (int)n_x; a.ops = ops; a.ch = ch; a.C = C; a.M = (int)M; a.B = B; a.dy = dy;
     a.y_out = y_out; a.loss_out = loss_out; a.grad_C = grad_C;
-    a.gx_partial = gx_lds ? gx_part : nullptr;
+    a.gx_partial = gx_part;
     a.gx_row0 = 0;
-    a.grad_X_direct = (want_gx && !gx_lds) ? grad_X : nullptr;
+    a.grad_X_direct = (want_gx && !L.gx_lds) ? grad_X : nullptr;
     a.err_b = err_b; a.flags = flags;
     a.list = nullptr; a.list_count = nullptr;
     a.queue = queue; a.queue_count = counters;
     a.work_counter = counters + 1;
     a.per_blk = (B + g1 - 1) / g1;
+    a.gtape = L.gt ? (void *)(ws + L.gt_off) : nullptr;
     int r = launch_fused_disp<T>(a, mode, fwd_only != 0, want_gx, false, 0, g1, s);
     if (r) return r;
     if (M > P1_LMAX) {
@@ -462,9 +504,9 @@ int fused_root(const T *X, int64_t N, int64_t n_x, const int64_t *ops, const int
         a2.list = queue;
         a2.list_count = counters;
         a2.gx_row0 = g1;
-        r = launch_fused_disp<T>(a2, mode, fwd_only != 0, want_gx, true, (int)M, g2, s);
+        r = launch_fused_disp<T>(a2, mode, fwd_only != 0, want_gx, true, p2_lmax(M), g2, s);
         if (r) return r;
-    } else if (gx_lds) {
+    } else if (L.gx_lds) {
         // phase-2 rows stay unused: zero them so the reduction can read all rows
         int64_t cnt = g2 * gx_len;
         if (sizeof(T) == 4)
@@ -474,7 +516,7 @@ int fused_root(const T *X, int64_t N, int64_t n_x, const int64_t *ops, const int
             hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
                                (uint64_t *)(gx_part + g1 * gx_len), 0ull, cnt);
     }
-    if (gx_lds) {
+    if (L.gx_lds) {
         hipLaunchKernelGGL(k_reduce_partials<T>, dim3((unsigned)((gx_len + 255) / 256)), dim3(256), 0, s,
                            (const T *)gx_part, g1 + g2, gx_len, grad_X, 0);
     }
@@ -482,46 +524,38 @@ int fused_root(const T *X, int64_t N, int64_t n_x, const int64_t *ops, const int
 }
 
 template <typename T>
-int64_t fused_ws_bytes(int64_t N, int64_t n_x, int64_t B, int want_gx) {
-    int64_t g1, g2;
-    fused_grids(B, g1, g2);
-    int64_t gx_len = N * n_x;
-    bool gx_lds = want_gx && n_x <= NXREG && (size_t)gx_len * sizeof(T) <= GX_LDS_MAX;
-    return (((int64_t)B * 4 + 16 + 255) / 256) * 256 + (gx_lds ? (g1 + g2) * gx_len * (int64_t)sizeof(T) : 0) + 256;
+int64_t fused_ws_bytes(int64_t N, int64_t n_x, int64_t B, int64_t M, int want_gx) {
+    return (int64_t)fused_ws_layout<T>(N, n_x, B, M, want_gx != 0).total;
 }
 
 template <typename T>
 int forward_mb(const T *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch, const T *C, int64_t M,
-               int64_t B, T *cache, hipStream_t s) {
-    if (M <= 0 || M > 128 || N < 0 || B < 0 || n_x < 0) return -1;
+               int64_t B, int64_t K, T *cache, hipStream_t s) {
+    if (M <= 0 || M > 128 || N < 0 || B < 0 || n_x < 0 || K <= 0) return -1;
     if (N == 0 || B == 0) return 0;
     int64_t n_tiles = (N + TILE - 1) / TILE;
-    int64_t nblk = n_tiles * B;
+    int64_t nblk = n_tiles * B * K;
     if (nblk > 0x7fffffffLL) return -1;
     size_t lds = (size_t)M * WAVE * sizeof(T);
-    if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_full_forward<T>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return (int)e;
-    }
+    if (lds > lds_limit()) return -3;
     hipLaunchKernelGGL(k_full_forward<T>, dim3((unsigned)nblk), dim3(64), lds, s, X, N, (int)n_x, ops, ch, C,
-                       (int)M, B, n_tiles, cache);
+                       (int)M, B, K, n_tiles, cache);
     return ret_last();
 }
 
 template <typename T>
-int64_t backward_mb_ws_bytes(int64_t B, int64_t N, int64_t n_x) {
-    int64_t nblk = (B + BWD_WAVES - 1) / BWD_WAVES;
+int64_t backward_mb_ws_bytes(int64_t B, int64_t N, int64_t n_x, int64_t K = 1) {
+    int64_t nblk = (B * K + BWD_WAVES - 1) / BWD_WAVES;
     return nblk * N * n_x * (int64_t)sizeof(T) + 256;
 }
 
 template <typename T>
-int backward_mb(const T *grad_out, const T *cache, const int64_t *ops, const int64_t *ch, int64_t M, int64_t B,
-                int64_t N, int64_t n_x, T *grad_C, T *grad_X, void *workspace, int64_t ws_bytes, int32_t *d_err,
-                hipStream_t s) {
-    if (M <= 0 || M > 128 || N < 0 || B < 0 || n_x < 0) return -1;
+int backward_mb(T *grad_out, const T *cache, const int64_t *ops, const int64_t *ch, int64_t M, int64_t B,
+                int64_t K, int64_t N, int64_t n_x, T *grad_C, T *grad_X, void *workspace, int64_t ws_bytes,
+                int32_t *d_err, hipStream_t s) {
+    if (M <= 0 || M > 128 || N < 0 || B < 0 || n_x < 0 || K <= 0) return -1;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, d_err, 0, (int64_t)1);
-    int64_t nblk = (B + BWD_WAVES - 1) / BWD_WAVES;
+    int64_t nblk = (B * K + BWD_WAVES - 1) / BWD_WAVES;
     const bool gx_regs = n_x <= NXREG;
     if (grad_X) {
         int64_t len = N * n_x;
@@ -535,7 +569,7 @@ int backward_mb(const T *grad_out, const T *cache, const int64_t *ops, const int
         }
     }
     if (B == 0 || N == 0) {
-        int64_t len = M * B;
+        int64_t len = M * B * K;
         if (len > 0) {
             if (sizeof(T) == 4)
                 hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, (int32_t *)grad_C,
@@ -548,18 +582,13 @@ int backward_mb(const T *grad_out, const T *cache, const int64_t *ops, const int
     }
     T *gx_partial = nullptr;
     if (grad_X && gx_regs) {
-        if (ws_bytes < backward_mb_ws_bytes<T>(B, N, n_x)) return -2;
+        if (ws_bytes < backward_mb_ws_bytes<T>(B, N, n_x, K)) return -2;
         gx_partial = (T *)workspace;
     }
-    size_t lds = (size_t)BWD_WAVES * M * WAVE * sizeof(T) + (size_t)BWD_WAVES * M * sizeof(double) +
-                 (size_t)BWD_WAVES * WAVE * NXREG * sizeof(T);
-    if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_full_backward<T>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return (int)e;
-    }
+    size_t lds = (size_t)BWD_WAVES * M * sizeof(double) + (size_t)BWD_WAVES * WAVE * NXREG * sizeof(T);
+    if (lds > lds_limit()) return -3;
     hipLaunchKernelGGL(k_full_backward<T>, dim3((unsigned)nblk), dim3(BWD_WAVES * 64), lds, s, grad_out, cache, ops,
-                       ch, (int)M, B, N, (int)n_x, grad_C, gx_partial, (grad_X && !gx_regs) ? grad_X : nullptr,
+                       ch, (int)M, B, K, N, (int)n_x, grad_C, gx_partial, (grad_X && !gx_regs) ? grad_X : nullptr,
                        d_err);
     if (gx_partial) {
         int64_t len = N * n_x;
@@ -585,32 +614,56 @@ int sr_validate_mb(const int64_t *ops, const int64_t *ch, int64_t M, int64_t B, 
 
 int sr_forward_mb_f32(const float *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch, const float *C,
                       int64_t M, int64_t B, float *cache, void *stream) {
-    return forward_mb<float>(X, N, n_x, ops, ch, C, M, B, cache, (hipStream_t)stream);
+    return forward_mb<float>(X, N, n_x, ops, ch, C, M, B, 1, cache, (hipStream_t)stream);
 }
 int sr_forward_mb_f64(const double *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
                       const double *C, int64_t M, int64_t B, double *cache, void *stream) {
-    return forward_mb<double>(X, N, n_x, ops, ch, C, M, B, cache, (hipStream_t)stream);
+    return forward_mb<double>(X, N, n_x, ops, ch, C, M, B, 1, cache, (hipStream_t)stream);
 }
 
 int64_t sr_backward_mb_workspace_bytes(int64_t B, int64_t N, int64_t n_x, int is_f64) {
     return is_f64 ? backward_mb_ws_bytes<double>(B, N, n_x) : backward_mb_ws_bytes<float>(B, N, n_x);
 }
 
-int sr_backward_mb_f32(const float *grad_out, const float *cache, const int64_t *ops, const int64_t *ch, int64_t M,
+int sr_backward_mb_f32(float *grad_out, const float *cache, const int64_t *ops, const int64_t *ch, int64_t M,
                        int64_t B, int64_t N, int64_t n_x, float *grad_C, float *grad_X, void *workspace,
                        int64_t ws_bytes, int32_t *d_err, void *stream) {
-    return backward_mb<float>(grad_out, cache, ops, ch, M, B, N, n_x, grad_C, grad_X, workspace, ws_bytes, d_err,
+    return backward_mb<float>(grad_out, cache, ops, ch, M, B, 1, N, n_x, grad_C, grad_X, workspace, ws_bytes, d_err,
                               (hipStream_t)stream);
 }
-int sr_backward_mb_f64(const double *grad_out, const double *cache, const int64_t *ops, const int64_t *ch, int64_t M,
+int sr_backward_mb_f64(double *grad_out, const double *cache, const int64_t *ops, const int64_t *ch, int64_t M,
                        int64_t B, int64_t N, int64_t n_x, double *grad_C, double *grad_X, void *workspace,
                        int64_t ws_bytes, int32_t *d_err, void *stream) {
-    return backward_mb<double>(grad_out, cache, ops, ch, M, B, N, n_x, grad_C, grad_X, workspace, ws_bytes, d_err,
+    return backward_mb<double>(grad_out, cache, ops, ch, M, B, 1, N, n_x, grad_C, grad_X, workspace, ws_bytes, d_err,
                                (hipStream_t)stream);
 }
 
-int64_t sr_fused_workspace_bytes(int64_t N, int64_t n_x, int64_t B, int want_gx, int is_f64) {
-    return is_f64 ? fused_ws_bytes<double>(N, n_x, B, want_gx) : fused_ws_bytes<float>(N, n_x, B, want_gx);
+int sr_multi_forward_mb_f32(const float *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                            const float *C, int64_t M, int64_t B, int64_t K, float *cache, void *stream) {
+    return forward_mb<float>(X, N, n_x, ops, ch, C, M, B, K, cache, (hipStream_t)stream);
+}
+int sr_multi_forward_mb_f64(const double *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                            const double *C, int64_t M, int64_t B, int64_t K, double *cache, void *stream) {
+    return forward_mb<double>(X, N, n_x, ops, ch, C, M, B, K, cache, (hipStream_t)stream);
+}
+int64_t sr_multi_backward_mb_workspace_bytes(int64_t B, int64_t N, int64_t n_x, int64_t K, int is_f64) {
+    return is_f64 ? backward_mb_ws_bytes<double>(B, N, n_x, K) : backward_mb_ws_bytes<float>(B, N, n_x, K);
+}
+int sr_multi_backward_mb_f32(float *grad_out, const float *cache, const int64_t *ops, const int64_t *ch,
+                             int64_t M, int64_t B, int64_t N, int64_t n_x, int64_t K, float *grad_C, float *grad_X,
+                             void *workspace, int64_t ws_bytes, int32_t *d_err, void *stream) {
+    return backward_mb<float>(grad_out, cache, ops, ch, M, B, K, N, n_x, grad_C, grad_X, workspace, ws_bytes, d_err,
+                              (hipStream_t)stream);
+}
+int sr_multi_backward_mb_f64(double *grad_out, const double *cache, const int64_t *ops, const int64_t *ch,
+                             int64_t M, int64_t B, int64_t N, int64_t n_x, int64_t K, double *grad_C, double *grad_X,
+                             void *workspace, int64_t ws_bytes, int32_t *d_err, void *stream) {
+    return backward_mb<double>(grad_out, cache, ops, ch, M, B, K, N, n_x, grad_C, grad_X, workspace, ws_bytes, d_err,
+                               (hipStream_t)stream);
+}
+
+int64_t sr_fused_workspace_bytes(int64_t N, int64_t n_x, int64_t B, int64_t M, int want_gx, int is_f64) {
+    return is_f64 ? fused_ws_bytes<double>(N, n_x, B, M, want_gx) : fused_ws_bytes<float>(N, n_x, B, M, want_gx);
 }
 
 int sr_fused_root_f32(const float *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch, const float *C,

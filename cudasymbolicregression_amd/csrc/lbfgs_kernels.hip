@@ -29,6 +29,7 @@
 // when every slot of x is NaN with a non-empty history (the reference's
 // iterates stay NaN from there on).
 #include "sr_wave.h"
+#include "sr_device_info.h"
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -69,6 +70,7 @@ struct LbfgsArgs {
     int32_t *work_counter;
     const int32_t *list;
     const int32_t *list_count;
+    float *gtape;            // global tape/adjoint slots for programs too long for LDS
 };
 
 template <int SL>
@@ -140,7 +142,8 @@ __device__ inline double closure(Program<float> &p, const float *X, const float 
         for (int k = 0; k < L; ++k) adj[k * WAVE + lane] = 0.0f;
         adj[p.root * WAVE + lane] = active ? g0 : 0.0f;
         auto val = [&](int k) -> float { return tape[k * WAVE + lane]; };
-        int32_t ee = backward_tile<float, false>(p, val, adj, L, active, acc, cextra, n_x, (float *)nullptr);
+        auto adjr = [&](int k) -> float & { return adj[k * WAVE + lane]; };
+        int32_t ee = backward_tile<float, false>(p, val, adjr, L, active, acc, cextra, n_x, (float *)nullptr);
         e = ee > e ? ee : e;
     }
     double lo, hi;
@@ -158,21 +161,23 @@ __device__ inline void set_point(Program<float> &p, const Vec<SL> &x, const Vec<
     if (SL > 1) p.c1 = x.v[SL - 1] + a * d.v[SL - 1];
 }
 
-template <int LMAX, int SL, int W>
+template <int LMAX, int SL, int W, bool GT>
 __global__ __launch_bounds__(W * 64) void k_lbfgs(LbfgsArgs a) {
+    // GT: tape and adjoints in a global workspace slot instead of LDS.
     extern __shared__ __align__(16) unsigned char smem[];
-    const int wave = threadIdx.x / WAVE;
+    const int wave = uniform_i32((int)(threadIdx.x / WAVE));
     const int lane = lane_id();
-    float *tape = reinterpret_cast<float *>(smem) + (size_t)wave * 2 * LMAX * WAVE;
+    constexpr size_t TAPE_LDS = GT ? 0 : (size_t)W * 2 * LMAX * WAVE;
+    float *tape = GT ? a.gtape + ((size_t)blockIdx.x * W + wave) * 2 * LMAX * WAVE
+                     : reinterpret_cast<float *>(smem) + (size_t)wave * 2 * LMAX * WAVE;
     float *adj = tape + LMAX * WAVE;
-    double *cextra = reinterpret_cast<double *>(reinterpret_cast<float *>(smem) + (size_t)W * 2 * LMAX * WAVE) +
-                     (size_t)wave * LMAX;
+    double *cextra = reinterpret_cast<double *>(reinterpret_cast<float *>(smem) + TAPE_LDS) + (size_t)wave * LMAX;
     const int64_t count = a.list ? (int64_t)*a.list_count : a.B;
     const double c1 = 1e-4, c2 = 0.9;
     for (;;) {
-        int i = 0;
-        if (lane == 0) i = atomicAdd(a.work_counter, 1);
-        i = __shfl(i, 0, WAVE);
+        int got = 0;
+        if (lane == 0) got = atomicAdd(a.work_counter, 1);
+        const int i = uniform_i32(got);  // wave-uniform slot index (SGPR)
         if (i >= count) break;
         const int64_t b = a.list ? (int64_t)a.list[i] : (int64_t)i;
         Program<float> p = load_program<float>(a.ops, a.ch, a.C_init, b, a.M, 1, a.M, 1, a.M, a.n_x);
@@ -378,38 +383,43 @@ __global__ void k_fill_i32_l(int32_t *p, int32_t v, int64_t n) {
     if (i < n) p[i] = v;
 }
 
-int g_cus = 0;
-int cu_count() {
-    if (!g_cus) {
-        int dev = 0, v = 0;
-        (void)hipGetDevice(&dev);
-        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
-        g_cus = v;
-    }
-    return g_cus;
+inline size_t lbfgs_lds(int LMAX, int W, bool gt) {
+    return (gt ? 0 : (size_t)W * 2 * LMAX * WAVE * sizeof(float)) + (size_t)W * LMAX * sizeof(double);
 }
 
 template <int LMAX, int SL, int W>
-int launch_lbfgs(const LbfgsArgs &a, int blocks_per_cu, hipStream_t s) {
-    size_t lds = (size_t)W * 2 * LMAX * WAVE * sizeof(float) + (size_t)W * LMAX * sizeof(double);
-    auto kern = k_lbfgs<LMAX, SL, W>;
-    if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return (int)e;
+int launch_lbfgs(const LbfgsArgs &a, int64_t grid, hipStream_t s) {
+    const bool gt = lbfgs_lds(LMAX, W, false) > lds_limit();
+    const size_t lds = lbfgs_lds(LMAX, W, gt);
+    if (lds > lds_limit()) return -3;
+    if (gt) {
+        if (a.gtape == nullptr) return -2;
+        hipLaunchKernelGGL((k_lbfgs<LMAX, SL, W, true>), dim3((unsigned)grid), dim3(W * 64), lds, s, a);
+    } else {
+        hipLaunchKernelGGL((k_lbfgs<LMAX, SL, W, false>), dim3((unsigned)grid), dim3(W * 64), lds, s, a);
     }
-    int64_t grid = (int64_t)cu_count() * blocks_per_cu;
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(W * 64), lds, s, a);
     return (int)hipGetLastError();
+}
+
+constexpr int P2_BLOCKS_PER_CU = 8;
+
+inline int p2_lmax(int64_t M) { return M <= 64 ? 64 : 128; }
+
+// Workspace: [queue: B int32][4 counters] | global tape slots (phase 2, M > 64 only).
+inline size_t lbfgs_gt_off(int64_t B) { return (((size_t)B * 4 + 16) + 255) / 256 * 256; }
+inline bool lbfgs_needs_gt(int64_t M) { return M > 16 && lbfgs_lds(p2_lmax(M), 1, false) > lds_limit(); }
+inline int64_t lbfgs_ws(int64_t B, int64_t M) {
+    size_t off = lbfgs_gt_off(B);
+    if (lbfgs_needs_gt(M))
+        off += (size_t)num_cus() * P2_BLOCKS_PER_CU * 2 * p2_lmax(M) * WAVE * sizeof(float);
+    return (int64_t)off + 256;
 }
 
 }  // namespace
 
 extern "C" {
 
-int64_t sr_lbfgs_workspace_bytes(int64_t B, int64_t M) {
-    (void)M;
-    return ((int64_t)B * 4 + 64 + 255) / 256 * 256;
-}
+int64_t sr_lbfgs_workspace_bytes(int64_t B, int64_t M) { return lbfgs_ws(B, M); }
 
 // Fits the LEARNABLE_CONSTANT slots of B batch-first expressions to their
 // targets.  See include/sr_eval.h for the argument contract.
@@ -439,15 +449,18 @@ int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const i
     a.ls_max_iter = ls_max_iter; a.eps = eps;
     a.queue = queue; a.queue_count = counters; a.work_counter = counters + 1;
     a.list = nullptr; a.list_count = nullptr;
-    int r = (M <= 64) ? launch_lbfgs<16, 1, 4>(a, 4, s) : launch_lbfgs<16, 2, 4>(a, 4, s);
+    a.gtape = lbfgs_needs_gt(M) ? (float *)(ws + lbfgs_gt_off(B)) : nullptr;
+    const int64_t g1 = (int64_t)num_cus() * 4;
+    int r = (M <= 64) ? launch_lbfgs<16, 1, 4>(a, g1, s) : launch_lbfgs<16, 2, 4>(a, g1, s);
     if (r) return r;
     if (M > 16) {
         LbfgsArgs a2 = a;
         a2.list = queue;
         a2.list_count = counters;
         a2.work_counter = counters + 2;
-        if (M <= 64) r = launch_lbfgs<64, 1, 2>(a2, 2, s);
-        else r = launch_lbfgs<128, 2, 1>(a2, 2, s);
+        const int64_t g2 = (int64_t)num_cus() * P2_BLOCKS_PER_CU;
+        if (M <= 64) r = launch_lbfgs<64, 1, 1>(a2, g2, s);
+        else r = launch_lbfgs<128, 2, 1>(a2, g2, s);
         if (r) return r;
     }
     return (int)hipGetLastError();

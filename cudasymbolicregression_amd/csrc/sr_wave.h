@@ -33,10 +33,23 @@ __device__ inline double rl(double v, int k) {
     return __longlong_as_double(((int64_t)(uint32_t)lo) | ((int64_t)hi << 32));
 }
 
+// Values every lane of the wave agrees on, moved to SGPRs so that the
+// compiler sees them as uniform: branches and loops that depend on them then
+// compile to scalar control flow instead of exec-masked divergent loops.
+__device__ inline int32_t uniform_i32(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline double uniform_f64(double v) {
+    int64_t b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffff));
+    int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((int64_t)(uint32_t)lo) | ((int64_t)hi << 32));
+}
+
+// Butterfly sum: every lane computes the same additions in a commuted order,
+// so all lanes hold the bit-identical total; it is returned as a uniform.
 __device__ inline double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, WAVE);
-    return v;
+    return uniform_f64(v);
 }
 __device__ inline int32_t wave_max_i32(int32_t v) {
 #pragma unroll
@@ -44,7 +57,7 @@ __device__ inline int32_t wave_max_i32(int32_t v) {
         int32_t o = __shfl_xor(v, off, WAVE);
         v = o > v ? o : v;
     }
-    return v;
+    return uniform_i32(v);
 }
 
 // Program of one expression held in registers (M <= 128).
@@ -184,22 +197,24 @@ struct GradAcc {
     T gxl[NXREG];        // variable adjoints of the current tile
 };
 
-// Reverse sweep of one tile over nodes [0, L) with adjoints preloaded in adj.
-// `val(k)` returns this lane's forward value of node k (LDS tape or the saved
-// cache).  Constant adjoints go to acc.cacc (first NCREG constants) or, beyond
-// that, to a wave-uniform FP64 LDS slot cextra[k].  Variable adjoints go to
-// acc.gxl[v] (n_x <= NXREG) or straight to gx_global_row with atomics.
-// Inactive lanes (points past N) contribute nothing.  Returns this lane's
-// error code (0, 201 or 204; the larger one if both occur).
-template <typename T, bool WANT_GX, typename Val>
-__device__ inline int32_t backward_tile(const Program<T> &p, Val val, T *adj, int L, bool active,
+// Reverse sweep of one tile over nodes [0, L) with adjoints preloaded.
+// `val(k)` returns this lane's forward value of node k and `adj(k)` a
+// reference to its adjoint (LDS tape, a global tape, or the caller's
+// grad_cache).  Constant adjoints go to acc.cacc (first NCREG constants) or,
+// beyond that, to a wave-uniform FP64 LDS slot cextra[k].  Variable adjoints
+// go to acc.gxl[v] (n_x <= NXREG) or straight to gx_global_row with atomics.
+// Inactive lanes (points past N) read but never write and contribute
+// nothing.  Returns this lane's error code (0, 201 or 204; the larger one if
+// both occur).
+template <typename T, bool WANT_GX, typename Val, typename Adj>
+__device__ inline int32_t backward_tile(const Program<T> &p, Val val, Adj adj, int L, bool active,
                                         GradAcc<T> &acc, double *cextra, int n_x, T *gx_global_row) {
     const int lane = lane_id();
     int32_t err = 0;
     for (int k = L - 1; k >= 0; --k) {
         uint32_t w = p.word(k);
         int op = node_op(w);
-        T g = adj[k * WAVE + lane];
+        T g = adj(k);
         if (op == NO_OP) {
             if (active && g != T(0)) err = err > ERR_GRAD_NO_OP ? err : ERR_GRAD_NO_OP;
             continue;
@@ -239,8 +254,8 @@ __device__ inline int32_t backward_tile(const Program<T> &p, Val val, T *adj, in
         bool add0, add1;
         int32_t e = bwd_node<T>(op, g, a0, a1, go0, add0, go1, add1);
         if (active && e > err) err = e;
-        if (add0) adj[c0 * WAVE + lane] += go0;
-        if (add1) adj[c1 * WAVE + lane] += go1;
+        if (active && add0) adj(c0) += go0;
+        if (active && add1) adj(c1) += go1;
     }
     return err;
 }

@@ -79,7 +79,7 @@ def fused_root(
     flags = torch.empty(2, dtype=torch.int32, device=dev)
     lib = _lib.hip()
     is64 = dt == torch.float64
-    ws_bytes = lib.sr_fused_workspace_bytes(N, n_x, B, int(gX is not None), int(is64))
+    ws_bytes = lib.sr_fused_workspace_bytes(N, n_x, B, M, int(gX is not None), int(is64))
     ws = torch.empty(max(ws_bytes, 256), dtype=torch.uint8, device=dev)
     fn = lib.sr_fused_root_f64 if is64 else lib.sr_fused_root_f32
     _lib.check_status(

@@ -110,3 +110,40 @@ def allclose_nan(a, b, rtol=1e-5, atol=1e-6):
         idx = np.argmax(d - tol)
         return False, f"max excess {float((d - tol)[idx])}: {a[fin][idx]} vs {b[fin][idx]}"
     return True, ""
+
+
+def _cls(a):
+    return np.where(np.isnan(a), 1, np.where(np.isposinf(a), 2, np.where(np.isneginf(a), 3, 0)))
+
+
+def check_vs_f64(gpu, ref32, ref64, rtol=1e-5, atol=1e-6, factor=2.0):
+    """Accuracy of a GPU result judged against the FP64 oracle.
+
+    Where the reference's FP32 result (ref32) and the FP64 oracle (ref64) agree
+    on NaN / +inf / -inf, the GPU must agree too; elsewhere (an ill-conditioned
+    point where FP32 rounding decides the class) the point is skipped.  For
+    finite points the GPU error against ref64 must not exceed
+    factor * |ref32 - ref64| + rtol * |ref64| + atol: at least as accurate as
+    the reference's FP32 path, within rtol.  Returns (ok, message).
+    """
+    g = np.asarray(gpu, np.float64)
+    r32 = np.asarray(ref32, np.float64)
+    r64 = np.asarray(ref64, np.float64)
+    if g.shape != r64.shape or r32.shape != r64.shape:
+        return False, f"shape {g.shape} {r32.shape} {r64.shape}"
+    cg, c32, c64 = _cls(g), _cls(r32), _cls(r64)
+    amb = c32 != c64
+    badcls = (cg != c32) & ~amb
+    if badcls.any():
+        idx = np.argwhere(badcls)[:5].tolist()
+        return False, f"class mismatch at {idx}: gpu {g[badcls][:5]} ref32 {r32[badcls][:5]}"
+    fin = (cg == 0) & (c32 == 0) & ~amb
+    eg = np.abs(g[fin] - r64[fin])
+    e32 = np.abs(r32[fin] - r64[fin])
+    bound = factor * e32 + rtol * np.abs(r64[fin]) + atol
+    bad = eg > bound
+    if bad.any():
+        i = int(np.argmax(eg - bound))
+        return False, (f"{int(bad.sum())} points exceed: worst gpu {g[fin][i]!r} ref32 {r32[fin][i]!r} "
+                       f"ref64 {r64[fin][i]!r}")
+    return True, f"{int(amb.sum())} ill-conditioned points skipped"

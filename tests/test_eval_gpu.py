@@ -1,6 +1,13 @@
-"""GPU parity of the evaluation kernels against the C oracle (FP32 within
-|a-b| <= 1e-5*max(|a|,|b|) + 1e-6 with identical NaN/inf masks; FP64 within
-1e-12 relative), through the public API and the C-ABI."""
+"""GPU parity of the evaluation kernels against the C oracle.
+
+* Every operator alone (forward and backward): FP32 within
+  |a-b| <= 1e-5*max(|a|,|b|) + 1e-6 of the reference's FP32 CPU algorithm,
+  identical NaN / inf masks and error codes; FP64 within 1e-12.
+* Composed random programs: judged against the FP64 oracle with
+  helpers.check_vs_f64 (at least as accurate as the reference's FP32 path,
+  within 1e-5 relative), because a composition such as exp(exp(x))
+  amplifies last-ulp differences between math libraries.
+"""
 import numpy as np
 import pytest
 import torch
@@ -11,10 +18,15 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 RTOL32, ATOL32 = 1e-5, 1e-6
+ALL_OPS = helpers.UNARY + helpers.BINARY
 
 
 def _dev():
     return torch.device("cuda:0")
+
+
+def _t(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(_dev())
 
 
 def _case(seed, B=16, M=24, n_x=3, N=100, dtype=np.float32, **kw):
@@ -25,50 +37,99 @@ def _case(seed, B=16, M=24, n_x=3, N=100, dtype=np.float32, **kw):
     return X, ops_bm, ch_bm, C_bm
 
 
-def _t(a):
-    return torch.from_numpy(np.ascontiguousarray(a)).to(_dev())
+def test_device_limits_reported():
+    p = torch.cuda.get_device_properties(0)
+    print(f"device {p.name}: {p.multi_processor_count} CUs, shared/block {p.shared_memory_per_block}, "
+          f"shared/CU {getattr(p, 'shared_memory_per_multiprocessor', 'n/a')}", flush=True)
+
+
+def _single_op_program(op, with_const):
+    """[X0 (, X1)] op  or  [C (, X0)] op  as a node-first (M, 1) program."""
+    ar = 1 if op < 40 else 2
+    leaves = ([1] if with_const else [50]) + ([51 if not with_const else 50] if ar == 2 else [])
+    ops = np.array(leaves + [op], np.int64)[:, None]
+    ch = np.full((len(ops), 1, 2), -1, np.int64)
+    ch[-1, 0, :ar] = np.arange(ar)
+    return ops, ch
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("seed", range(4))
-def test_backend_forward_matches_oracle(seed, dtype):
+@pytest.mark.parametrize("op", ALL_OPS)
+def test_single_op_forward_backward(op, dtype):
     from cudasymbolicregression_amd.evaluation import evaluate_backend
 
-    print(f"case forward seed={seed} dtype={np.dtype(dtype).name}", flush=True)
-    X, ops_bm, ch_bm, C_bm = _case(seed, dtype=dtype, N=130)
+    rng = np.random.default_rng(op)
+    N = 257
+    X = (rng.standard_normal((N, 2)) * 1.5).astype(dtype)
+    X[:8, 0] = [0.0, -0.0, 1.0, -1.0, 0.5, -2.0, 1e-30, 3.0]
+    for with_const in (False, True):
+        ops, ch = _single_op_program(op, with_const)
+        C = np.zeros(ops.shape, dtype)
+        C[0, 0] = dtype(0.75) if with_const else 0
+        ref = oracle.forward(X, ops, ch, C)
+        Xt = _t(X).requires_grad_(True)
+        Ct = _t(C).requires_grad_(True)
+        out = evaluate_backend(Xt, _t(ops), _t(ch), Ct)
+        tol = (1e-12, 1e-13) if dtype == np.float64 else (RTOL32, ATOL32)
+        ok, msg = helpers.allclose_nan(out.detach().cpu().numpy(), ref, *tol)
+        assert ok, f"forward op {op}: {msg}"
+        g = np.zeros_like(ref)
+        g[-1] = rng.standard_normal((N, 1)).astype(dtype)
+        gc, gx, e = oracle.backward(g, ref, ops, ch, 2)
+        if e:
+            with pytest.raises(RuntimeError, match="backward pass"):
+                out.backward(_t(g))
+            continue
+        out.backward(_t(g))
+        ok, msg = helpers.allclose_nan(Xt.grad.cpu().numpy(), gx, *tol)
+        assert ok, f"grad_X op {op}: {msg}"
+        # grad_C sums N terms: compare against FP64
+        ref64 = oracle.backward(g.astype(np.float64), oracle.forward(X.astype(np.float64), ops, ch,
+                                                                      C.astype(np.float64)), ops, ch, 2)[0]
+        ok, msg = helpers.check_vs_f64(Ct.grad.cpu().numpy(), gc, ref64)
+        assert ok, f"grad_C op {op}: {msg}"
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_backend_forward_composed(seed):
+    from cudasymbolicregression_amd.evaluation import evaluate_backend
+
+    X, ops_bm, ch_bm, C_bm = _case(seed, N=130)
     ops, ch, C = helpers.to_mb(ops_bm, ch_bm, C_bm)
     got = evaluate_backend(_t(X), _t(ops), _t(ch), _t(C)).cpu().numpy()
-    ref = oracle.forward(X, ops, ch, C)
-    tol = (1e-12, 1e-13) if dtype == np.float64 else (RTOL32, ATOL32)
-    ok, msg = helpers.allclose_nan(got, ref, *tol)
+    r32 = oracle.forward(X, ops, ch, C)
+    r64 = oracle.forward(X.astype(np.float64), ops, ch, C.astype(np.float64))
+    ok, msg = helpers.check_vs_f64(got, r32, r64)
     assert ok, msg
+    got64 = evaluate_backend(_t(X.astype(np.float64)), _t(ops), _t(ch), _t(C.astype(np.float64))).cpu().numpy()
+    ok, msg = helpers.allclose_nan(got64, r64, 1e-12, 1e-13)
+    assert ok, "f64 " + msg
 
 
-@pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("seed", range(4))
-def test_backend_backward_matches_oracle(seed, dtype):
+def test_backend_backward_composed(seed):
     from cudasymbolicregression_amd.evaluation import evaluate_backend
 
-    print(f"case backward seed={seed} dtype={np.dtype(dtype).name}", flush=True)
-    X, ops_bm, ch_bm, C_bm = _case(seed, dtype=dtype, N=70, unary=helpers.G2_UNARY + [14, 27, 29])
+    X, ops_bm, ch_bm, C_bm = _case(seed, N=70, M=32, unary=helpers.G2_UNARY + [14, 27, 29])
     ops, ch, C = helpers.to_mb(ops_bm, ch_bm, C_bm)
     cache = oracle.forward(X, ops, ch, C)
+    cache64 = oracle.forward(X.astype(np.float64), ops, ch, C.astype(np.float64))
     rng = np.random.default_rng(seed + 50)
-    g = rng.standard_normal(cache.shape).astype(dtype)
-    g[np.isnan(cache)] = 0
-    gc_ref, gx_ref, e_ref = oracle.backward(g, cache, ops, ch, X.shape[1])
+    g = rng.standard_normal(cache.shape).astype(np.float32)
+    g[np.isnan(cache) | np.isnan(cache64)] = 0
+    gc32, gx32, e32 = oracle.backward(g, cache, ops, ch, X.shape[1])
+    gc64, gx64, e64 = oracle.backward(g.astype(np.float64), cache64, ops, ch, X.shape[1])
     Xt = _t(X).requires_grad_(True)
     Ct = _t(C).requires_grad_(True)
     out = evaluate_backend(Xt, _t(ops), _t(ch), Ct)
-    if e_ref:
+    if e32:
         with pytest.raises(RuntimeError, match="backward pass"):
             out.backward(_t(g))
         return
     out.backward(_t(g))
-    tol = (1e-10, 1e-12) if dtype == np.float64 else (1e-4, 1e-5)
-    ok, msg = helpers.allclose_nan(Ct.grad.cpu().numpy(), gc_ref, *tol)
+    ok, msg = helpers.check_vs_f64(Ct.grad.cpu().numpy(), gc32, gc64, rtol=1e-5, atol=1e-5)
     assert ok, "grad_C " + msg
-    ok, msg = helpers.allclose_nan(Xt.grad.cpu().numpy(), gx_ref, *tol)
+    ok, msg = helpers.check_vs_f64(Xt.grad.cpu().numpy(), gx32, gx64, rtol=1e-5, atol=1e-5)
     assert ok, "grad_X " + msg
 
 
@@ -87,6 +148,11 @@ def test_validation_errors_gpu():
     ch[1, 0, 1] = -1
     with pytest.raises(RuntimeError, match="Validation failed: Operator arity does not match the number of children."):
         evaluate_backend(X, ops, ch, C)
+    for op in helpers.UNARY + helpers.BINARY:
+        o = torch.tensor([[op]], dtype=torch.int64, device=_dev())
+        c = -torch.ones(1, 1, 2, dtype=torch.int64, device=_dev())
+        with pytest.raises(RuntimeError, match="Operator arity does not match the number of children."):
+            evaluate_backend(X, o, c, torch.zeros(1, 1, device=_dev()))
 
 
 def test_backward_error_codes_gpu():
@@ -108,48 +174,89 @@ def test_backward_error_codes_gpu():
     assert torch.isnan(y).all()
     with pytest.raises(RuntimeError, match="passed to a NO_OP node"):
         y[-1].backward(torch.ones(1, 1, device=_dev()))
+    # zero upstream on a NO_OP is fine and gives zero gradients
+    X2 = torch.randn(10, 2, device=_dev(), requires_grad=True)
+    C2 = torch.zeros(1, 1, device=_dev(), requires_grad=True)
+    y = evaluate_backend(X2, ops, ch, C2)
+    y[-1].backward(torch.zeros(10, 1, device=_dev()))
+    assert (X2.grad == 0).all() and (C2.grad == 0).all()
 
 
 @pytest.mark.parametrize("mode", [0, 1])
-@pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("M", [12, 64, 128])
-def test_fused_root_matches_oracle(M, dtype, mode):
+def test_fused_root_matches_oracle(M, mode):
     from cudasymbolicregression_amd import fused
 
-    print(f"case fused M={M} dtype={np.dtype(dtype).name} mode={mode}", flush=True)
+    print(f"case fused M={M} mode={mode}", flush=True)
     seed = M + mode
-    X, ops_bm, ch_bm, C_bm = _case(seed, B=300, M=M, N=200, dtype=dtype, unary=helpers.G2_UNARY,
-                                   binary=helpers.G2_BINARY, depth=7)
+    X, ops_bm, ch_bm, C_bm = _case(seed, B=300, M=M, N=200, unary=helpers.G2_UNARY, binary=helpers.G2_BINARY,
+                                   depth=7)
     rng = np.random.default_rng(seed)
-    dy = rng.standard_normal((300, 200)).astype(dtype)
-    ref = oracle.fused_root(X, ops_bm, ch_bm, C_bm, dy, mode=mode)
+    dy = rng.standard_normal((300, 200)).astype(np.float32)
+    r32 = oracle.fused_root(X, ops_bm, ch_bm, C_bm, dy, mode=mode)
+    r64 = oracle.fused_root(X.astype(np.float64), ops_bm, ch_bm, C_bm.astype(np.float64), dy.astype(np.float64),
+                            mode=mode)
     out = fused.fused_root(_t(X), _t(ops_bm), _t(ch_bm), _t(C_bm), _t(dy), mode=mode, want_grad_X=True,
                            raise_errors=False)
-    tol = (1e-11, 1e-12) if dtype == np.float64 else (1e-4, 1e-5)
-    ok, msg = helpers.allclose_nan(out["y"].cpu().numpy(), ref["y"], *(tol if dtype == np.float64 else (RTOL32,
-                                                                                                        ATOL32)))
+    ok, msg = helpers.check_vs_f64(out["y"].cpu().numpy(), r32["y"], r64["y"])
     assert ok, "y " + msg
     err = out["err"].cpu().numpy()
-    assert np.array_equal(err != 0, ref["err"] != 0)
-    good = ref["err"] == 0
-    ok, msg = helpers.allclose_nan(out["grad_C"].cpu().numpy()[good], ref["grad_C"][good], *tol)
+    assert np.array_equal(err, r32["err"])
+    good = (r32["err"] == 0) & (r64["err"] == 0)
+    ok, msg = helpers.check_vs_f64(out["grad_C"].cpu().numpy()[good], r32["grad_C"][good], r64["grad_C"][good],
+                                   atol=1e-5)
     assert ok, "grad_C " + msg
     if mode == 1:
-        ok, msg = helpers.allclose_nan(out["loss"].cpu().numpy(), ref["loss"], 1e-6, 1e-9)
+        ok, msg = helpers.check_vs_f64(out["loss"].cpu().numpy(), r32["loss"], r64["loss"])
         assert ok, "loss " + msg
     if good.all():
-        ok, msg = helpers.allclose_nan(out["grad_X"].cpu().numpy(), ref["grad_X"], 1e-4 if dtype == np.float32
-                                       else 1e-10, 1e-4 if dtype == np.float32 else 1e-10)
+        ok, msg = helpers.check_vs_f64(out["grad_X"].cpu().numpy(), r32["grad_X"], r64["grad_X"], atol=1e-4)
         assert ok, "grad_X " + msg
+    # FP64 path against the FP64 oracle
+    out64 = fused.fused_root(_t(X.astype(np.float64)), _t(ops_bm), _t(ch_bm), _t(C_bm.astype(np.float64)),
+                             _t(dy.astype(np.float64)), mode=mode, raise_errors=False)
+    ok, msg = helpers.allclose_nan(out64["y"].cpu().numpy(), r64["y"], 1e-12, 1e-13)
+    assert ok, "y64 " + msg
 
 
 def test_public_evaluate_shape_and_view():
     from cudasymbolicregression_amd.evaluation import evaluate
 
-    X, ops_bm, ch_bm, C_bm = _case(3, B=7, M=10, N=33)
+    X, ops_bm, ch_bm, C_bm = _case(3, B=7, M=10, N=33, unary=[10, 11, 14, 27], binary=[40, 41, 42])
     y = evaluate(_t(X), _t(ops_bm), _t(ch_bm), _t(C_bm))
     assert y.shape == (7, 10, 33)
     ops, ch, C = helpers.to_mb(ops_bm, ch_bm, C_bm)
     ref = oracle.forward(X, ops, ch, C).transpose(2, 0, 1)
     ok, msg = helpers.allclose_nan(y.cpu().numpy(), ref, RTOL32, ATOL32)
     assert ok, msg
+
+
+@pytest.mark.parametrize("K", [1, 3])
+def test_multiple_constants_match_oracle(K):
+    from cudasymbolicregression_amd.evaluation import evaluate_multiple
+
+    X, ops_bm, ch_bm, C_bm = _case(21, B=9, M=16, N=90, unary=[10, 11, 12, 14], binary=[40, 41, 42])
+    rng = np.random.default_rng(4)
+    Ck = np.repeat(C_bm[:, :, None], K, axis=2) + ((ops_bm == 1)[:, :, None] *
+                                                   rng.standard_normal((*C_bm.shape, K))).astype(np.float32)
+    Ck = Ck.astype(np.float32)
+    Xt = _t(X).requires_grad_(True)
+    Ct = _t(Ck).requires_grad_(True)
+    y = evaluate_multiple(Xt, _t(ops_bm), _t(ch_bm), Ct)
+    assert y.shape == (9, 16, 90, K)
+    ops, ch = helpers.to_mb(ops_bm, ch_bm)
+    Cmbk = np.ascontiguousarray(Ck.transpose(1, 0, 2))
+    r32 = oracle.multi_forward(X, ops, ch, Cmbk)
+    r64 = oracle.multi_forward(X.astype(np.float64), ops, ch, Cmbk.astype(np.float64))
+    ok, msg = helpers.check_vs_f64(y.detach().cpu().numpy(), r32.transpose(2, 0, 1, 3), r64.transpose(2, 0, 1, 3))
+    assert ok, msg
+    g = rng.standard_normal(r32.shape).astype(np.float32)
+    g[np.isnan(r32) | np.isnan(r64)] = 0
+    gc32, gx32, e32 = oracle.multi_backward(g, r32, ops, ch, 3)
+    gc64, gx64, _ = oracle.multi_backward(g.astype(np.float64), r64, ops, ch, 3)
+    y.backward(_t(np.ascontiguousarray(g.transpose(2, 0, 1, 3))))
+    assert e32 == 0
+    ok, msg = helpers.check_vs_f64(Ct.grad.cpu().numpy(), gc32.transpose(1, 0, 2), gc64.transpose(1, 0, 2), atol=1e-5)
+    assert ok, "grad_C " + msg
+    ok, msg = helpers.check_vs_f64(Xt.grad.cpu().numpy(), gx32, gx64, atol=1e-5)
+    assert ok, "grad_X " + msg
