@@ -70,7 +70,7 @@ def hip() -> ctypes.CDLL:
         _declare(lib, "sr_multi_backward_mb_workspace_bytes", I64, [I64, I64, I64, I64, I])
         _declare(lib, "sr_fused_workspace_bytes", I64, [I64, I64, I64, I64, I, I])
         _declare(lib, "sr_lbfgs_fit", I,
-                 [P, P, I64, I64, P, P, P, I64, I64, P, P, P, P, P, P, P, P, P, I, I, c_double, I, I, c_double,
+                 [P, P, I64, I64, P, P, P, I64, I64, P, P, P, P, P, P, P, P, P, P, I, I, c_double, I, I, c_double,
                   P, I64, P])
         _declare(lib, "sr_lbfgs_workspace_bytes", I64, [I64, I64])
         _hip = lib

@@ -57,6 +57,7 @@ struct LbfgsArgs {
     float *C_out;
     double *loss_out;
     int32_t *iters_out;
+    int32_t *nclos_out;      // closures evaluated per problem (nullable)
     int32_t *err_out;
     int32_t *flags;          // [0] validation (max), [1] evaluation (max)
     int max_iter;
@@ -209,6 +210,7 @@ __global__ __launch_bounds__(W * 64) void k_lbfgs(LbfgsArgs a) {
         double rho[MHMAX], alph[MHMAX];
         int nh = 0;
         int32_t err = 0;
+        int nclos = 1;
         double f = closure<LMAX, SL>(p, X, Yb, N, a.n_x, tape, adj, cextra, g, err);
         int it = 0;
         for (; it < a.max_iter && err == 0; ++it) {
@@ -268,6 +270,7 @@ __global__ __launch_bounds__(W * 64) void k_lbfgs(LbfgsArgs a) {
                     Vec<SL> ga;
                     set_point<SL>(p, x, d, ai);
                     const double phi = closure<LMAX, SL>(p, X, Yb, N, a.n_x, tape, adj, cextra, ga, e2);
+                    ++nclos;
                     if (e2) break;
                     const double dphi = dot<SL>(ga, d, a.M);
                     const bool bad = (phi > f + c1 * (double)ai * dphi0) || (ls > 0 && phi >= phi_prev);
@@ -284,6 +287,7 @@ __global__ __launch_bounds__(W * 64) void k_lbfgs(LbfgsArgs a) {
                         if (bad && ls == 0 && !d_finite) {
                             set_point<SL>(p, x, d, 0.0f);
                             phi_lo = closure<LMAX, SL>(p, X, Yb, N, a.n_x, tape, adj, cextra, g_lo, e2);
+                            ++nclos;
                             if (e2) break;
                         }
                         bool zdone = false;
@@ -292,6 +296,7 @@ __global__ __launch_bounds__(W * 64) void k_lbfgs(LbfgsArgs a) {
                             Vec<SL> gj;
                             set_point<SL>(p, x, d, aj);
                             const double phij = closure<LMAX, SL>(p, X, Yb, N, a.n_x, tape, adj, cextra, gj, e2);
+                            ++nclos;
                             if (e2) break;
                             const double dphij = dot<SL>(gj, d, a.M);
                             const bool zbad = (phij > f + c1 * (double)aj * dphi0) || (phij >= phi_lo);
@@ -325,6 +330,7 @@ __global__ __launch_bounds__(W * 64) void k_lbfgs(LbfgsArgs a) {
                     Vec<SL> ga;
                     set_point<SL>(p, x, d, ai);
                     const double ft = closure<LMAX, SL>(p, X, Yb, N, a.n_x, tape, adj, cextra, ga, e2);
+                    ++nclos;
                     if (e2) break;
                     if (ft <= f + 1e-4 * (double)ai * dphi0) {
                         alpha = ai; f_new = ft; g_new = ga; have = true;
@@ -346,6 +352,7 @@ __global__ __launch_bounds__(W * 64) void k_lbfgs(LbfgsArgs a) {
                 p.c0 = x.v[0];
                 if (SL > 1) p.c1 = x.v[SL - 1];
                 f_new = closure<LMAX, SL>(p, X, Yb, N, a.n_x, tape, adj, cextra, g_new, e2);
+                ++nclos;
                 if (e2) { err = e2; break; }
             }
             Vec<SL> yv;
@@ -372,6 +379,7 @@ __global__ __launch_bounds__(W * 64) void k_lbfgs(LbfgsArgs a) {
         if (lane == 0) {
             a.loss_out[b] = f;
             a.iters_out[b] = it;
+            if (a.nclos_out) a.nclos_out[b] = nclos;
             a.err_out[b] = err;
             if (err) atomicMax(&a.flags[1], err);
         }
@@ -426,7 +434,7 @@ int64_t sr_lbfgs_workspace_bytes(int64_t B, int64_t M) { return lbfgs_ws(B, M); 
 int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
                  const float *C_init, int64_t M, int64_t B, const int64_t *x_off, const int64_t *y_off,
                  const int32_t *n_pts, const double *stop_loss, float *C_out, double *loss_out, int32_t *iters_out,
-                 int32_t *err_out, int32_t *flags, int max_iter, int history, double tol_grad, int line_search,
+                 int32_t *nclos_out, int32_t *err_out, int32_t *flags, int max_iter, int history, double tol_grad, int line_search,
                  int ls_max_iter, double eps, void *workspace, int64_t ws_bytes, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     if (M <= 0 || M > 128 || B < 0 || n_x < 0 || history < 1 || history > MHMAX || max_iter < 0 ||
@@ -444,7 +452,8 @@ int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const i
     memset(&a, 0, sizeof(a));
     a.X = X; a.Y = Y; a.N = N; a.n_x = (int)n_x; a.ops = ops; a.ch = ch; a.C_init = C_init; a.M = (int)M; a.B = B;
     a.x_off = x_off; a.y_off = y_off; a.n_pts = n_pts; a.stop_loss = stop_loss;
-    a.C_out = C_out; a.loss_out = loss_out; a.iters_out = iters_out; a.err_out = err_out; a.flags = flags;
+    a.C_out = C_out; a.loss_out = loss_out; a.iters_out = iters_out; a.nclos_out = nclos_out; a.err_out = err_out;
+    a.flags = flags;
     a.max_iter = max_iter; a.history = history; a.tol_grad = tol_grad; a.line_search = line_search;
     a.ls_max_iter = ls_max_iter; a.eps = eps;
     a.queue = queue; a.queue_count = counters; a.work_counter = counters + 1;

@@ -33,6 +33,7 @@ def _fit(X, Y, Ops, Ch, C_init, max_iter, history_size, tolerance_grad, line_sea
     C_out = torch.empty((B, M), dtype=torch.float32, device=dev)
     loss = torch.empty(B, dtype=torch.float64, device=dev)
     iters = torch.empty(B, dtype=torch.int32, device=dev)
+    nclos = torch.empty(B, dtype=torch.int32, device=dev)
     err = torch.empty(B, dtype=torch.int32, device=dev)
     flags = torch.empty(2, dtype=torch.int32, device=dev)
     lib = _lib.hip()
@@ -43,13 +44,13 @@ def _fit(X, Y, Ops, Ch, C_init, max_iter, history_size, tolerance_grad, line_sea
         lib.sr_lbfgs_fit(
             _lib.ptr(X), _lib.ptr(Y), N, n_x, _lib.ptr(Ops), _lib.ptr(Ch), _lib.ptr(C_init), M, B,
             _lib.ptr(x_off), _lib.ptr(y_off), _lib.ptr(n_pts), _lib.ptr(stop_loss),
-            _lib.ptr(C_out), _lib.ptr(loss), _lib.ptr(iters), _lib.ptr(err), _lib.ptr(flags),
+            _lib.ptr(C_out), _lib.ptr(loss), _lib.ptr(iters), _lib.ptr(nclos), _lib.ptr(err), _lib.ptr(flags),
             int(max_iter), int(history_size), float(tolerance_grad), _LINE_SEARCH[line_search_fn],
             int(line_search_max_iter), float(eps), _lib.ptr(ws), ws.numel(), _lib.stream_of(dev),
         ),
         "sr_lbfgs_fit",
     )
-    return C_out, loss, iters, err, flags
+    return C_out, loss, iters, err, flags, nclos
 
 
 def optimize(
@@ -103,7 +104,7 @@ def optimize(
         var = Yc.double().var(dim=1, unbiased=False)
         var = torch.where(var > 0, var, (Yc.double() ** 2).mean(dim=1))
         stop = (float(stop_rel_loss) * var).contiguous()
-    C_out, loss, iters, err, flags = _fit(Xc, Yc, Opsc, Chc, Cc, max_iter, history_size, tolerance_grad,
+    C_out, loss, iters, err, flags, nclos = _fit(Xc, Yc, Opsc, Chc, Cc, max_iter, history_size, tolerance_grad,
                                           line_search_fn, stop_loss=stop)
     if raise_errors:
         f = flags.tolist()
@@ -113,5 +114,5 @@ def optimize(
             raise RuntimeError(get_error_message(f[1], "symbolic evaluation backward pass"))
     Y_final = fused_root(Xc, Opsc, Chc, C_out, None, mode=0, fwd_only=True, raise_errors=False)["y"]
     if return_info:
-        return C_out, Y_final, dict(loss=loss, iters=iters, err=err)
+        return C_out, Y_final, dict(loss=loss, iters=iters, err=err, closures=nclos)
     return C_out, Y_final

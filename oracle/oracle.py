@@ -107,8 +107,9 @@ def forward(X, ops_mb, ch_mb, C_mb, use_ref: bool = False) -> np.ndarray:
     return cache
 
 
-def backward(grad_out, cache, ops_mb, ch_mb, n_x: int, use_ref: bool = False):
-    """Returns (grad_C (M,B), grad_X (N,n_x), error code)."""
+def backward(grad_out, cache, ops_mb, ch_mb, n_x: int, use_ref: bool = False, return_adjoints: bool = False):
+    """Returns (grad_C (M,B), grad_X (N,n_x), error code) and, with
+    return_adjoints, the accumulated (M, N, B) adjoint tape as a 4th item."""
     dt = cache.dtype
     g = np.array(grad_out, dtype=dt, copy=True, order="C")
     cache = np.ascontiguousarray(cache)
@@ -119,6 +120,8 @@ def backward(grad_out, cache, ops_mb, ch_mb, n_x: int, use_ref: bool = False):
     gX = np.zeros((N, n_x), dt)
     fn = getattr(ref(), f"ref_backward_{_dt(dt)}") if use_ref else getattr(lib(), f"oracle_backward_{_dt(dt)}")
     e = fn(_p(g), _p(cache), _p(ops_mb), _p(ch_mb), M, B, N, n_x, _p(gC), _p(gX))
+    if return_adjoints:
+        return gC, gX, int(e), g
     return gC, gX, int(e)
 
 
