@@ -73,6 +73,11 @@ def hip() -> ctypes.CDLL:
                  [P, P, I64, I64, P, P, P, I64, I64, P, P, P, P, P, P, P, P, P, P, I, I, c_double, I, I, c_double,
                   P, I64, P])
         _declare(lib, "sr_lbfgs_workspace_bytes", I64, [I64, I64])
+        _declare(lib, "sr_fit_num_buckets", I, [])
+        _declare(lib, "sr_fit_num_variants", I, [I])
+        _declare(lib, "sr_fit_get_variant", I, [I])
+        _declare(lib, "sr_fit_set_variant", I, [I, I])
+        _declare(lib, "sr_fit_variant_info", I, [I, I, POINTER(c_int), POINTER(c_int)])
         _hip = lib
     return _hip
 

@@ -111,7 +111,7 @@ __global__ __launch_bounds__(BWD_WAVES * 64) void k_full_backward(
     // Wave w of block i handles expression e = 4*i + w, i.e. constant set
     // j = e % K of expression b = e / K (K = 1 for the single-set layout).
     extern __shared__ __align__(16) unsigned char smem[];
-    const int wave = threadIdx.x / WAVE;
+    const int wave = uniform_i32((int)(threadIdx.x / WAVE));
     const int lane = lane_id();
     double *cextra = reinterpret_cast<double *>(smem) + (size_t)wave * M;
     T *gxs = reinterpret_cast<T *>(reinterpret_cast<double *>(smem) + (size_t)BWD_WAVES * M);  // [4][64][NXREG]
@@ -268,9 +268,7 @@ __global__ __launch_bounds__(W * 64) void k_fused(FusedArgs a) {
             i = slot;
             slot += W;
         } else {
-            int got = 0;
-            if (lane == 0) got = atomicAdd(a.work_counter, 1);
-            i = uniform_i32(got);
+            i = wave_fetch_add(a.work_counter);
         }
         if (i >= count) break;
         const int64_t b = (a.list == nullptr) ? base + i : (int64_t)a.list[i];

@@ -1,0 +1,1046 @@
+// Device-resident batched L-BFGS for expression constants (the fit path).
+//
+// Algorithm and options follow the reference's LBFGS_batch.step with
+// optimize's closure (symbolic_torch/lbfgs_batch.py:65-326,
+// symbolic_torch/optimization.py:63-101), per problem:
+//   * p = -H g by the two-loop recursion (rho = 1/(y.s + eps),
+//     gamma = s.y/(y.y + eps) from the newest pair, no scaling before the
+//     first pair);
+//   * strong Wolfe bracketing (c1 = 1e-4, c2 = 0.9, alpha0 = 1,
+//     alpha <- alpha + 0.8*(20 - alpha); a point meeting the curvature test is
+//     accepted before the bracket tests) with the bisection zoom, or Armijo
+//     backtracking (halving);
+//   * x <- x + alpha p, history push (drop oldest beyond m);
+//   * stop when ||g||_2 < tol_grad (NaN never stops).
+// Closures the reference recomputes at points it already evaluated (phi_0 at
+// x, phi_lo, and f at the accepted step) are reused: evaluation is
+// deterministic, so the values are identical.  Inner products, the norm, the
+// losses and the Wolfe tests run in FP64; vectors and step lengths stay FP32
+// like the reference's tensors.  A problem stops early when it has converged,
+// when its loss reaches the optional per-problem stop_loss, or when every
+// slot of x is NaN with a non-empty history (the reference's iterates stay
+// NaN from there on).
+//
+// Execution model (gfx950, wave64):
+//   * A problem's constant slots are COMPACTED: lane j of a slot vector holds
+//     the j-th LEARNABLE_CONSTANT node (and 64 + j when SL == 2).  Every
+//     other node has gradient 0, so all of them follow one common trajectory
+//     (their steps are +-0, or NaN once a non-finite scalar appears); lane
+//     nc ("the shared slot") carries it, which keeps the reference's
+//     NaN-propagation through the M-slot inner products and its all-NaN
+//     stop test exact.
+//   * The L-BFGS iteration is a state machine around ONE closure call site,
+//     so the (large) evaluation code is instantiated once per kernel.
+//   * Closure: one tile = 64 points, one per lane.  The tape (node values)
+//     and the adjoints are [node][lane] arrays in LDS (no bank conflicts);
+//     programs longer than 64 nodes keep them in a global workspace slot.
+//     Node words are read with v_readlane, so the opcode dispatch is scalar.
+//   * W > 1 waves may share one problem: points are split between them, the
+//     per-wave FP64 partial sums meet in LDS in a fixed order, and every wave
+//     runs the (deterministic) L-BFGS logic on identical values.
+//   * Problems are bucketed by program length so every kernel is sized for
+//     its bucket; the buckets run concurrently on forked streams.
+#include "sr_wave.h"
+#include "sr_device_info.h"
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+using namespace sr;
+
+namespace {
+
+constexpr int MHMAX = 16;      // largest supported history_size
+constexpr int FIT_NCREG = 8;   // constants accumulated in per-lane FP64 registers
+
+struct FitArgs {
+    const float *X;
+    const float *Y;
+    int64_t N;
+    int n_x;
+    const int64_t *ops;
+    const int64_t *ch;
+    const float *C_init;
+    int M;
+    int64_t B;
+    const int64_t *x_off;    // ragged: element offset of problem b's X rows (nullable)
+    const int64_t *y_off;    // ragged: element offset of problem b's targets (nullable)
+    const int32_t *n_pts;    // ragged: number of points of problem b (nullable)
+    const double *stop_loss; // per-problem early stop threshold on the loss (nullable)
+    float *C_out;
+    double *loss_out;
+    int32_t *iters_out;
+    int32_t *nclos_out;      // closures evaluated per problem (nullable)
+    int32_t *err_out;
+    int32_t *flags;          // [0] validation (max), [1] evaluation (max)
+    int max_iter;
+    int history;
+    double tol_grad;
+    int line_search;         // 0 strong Wolfe, 1 backtracking
+    int ls_max_iter;
+    double eps;
+    const int32_t *list;     // problems of this bucket
+    const int32_t *list_count;
+    int32_t *work_counter;
+    float *gtape;            // global tape slots (programs longer than 64 nodes)
+};
+
+// ------------------------------------------------------------------ slots
+template <int SL>
+struct SVec {
+    float v[SL];
+};
+
+template <int SL>
+__device__ inline float slot_get(const SVec<SL> &a, int j) {
+    return (SL == 1 || j < WAVE) ? rl(a.v[0], j & (WAVE - 1)) : rl(a.v[SL - 1], j - WAVE);
+}
+
+// FP64 inner product over the K live slots; lanes past K never contribute.
+template <int SL>
+__device__ inline double sdot(const SVec<SL> &a, const SVec<SL> &b, int K) {
+    const int lane = lane_id();
+    double s = (lane < K) ? (double)a.v[0] * (double)b.v[0] : 0.0;
+    if (SL > 1 && lane + WAVE < K) s += (double)a.v[SL - 1] * (double)b.v[SL - 1];
+    return wave_sum_rows(s, K <= 16 ? 1 : 4);
+}
+
+// ---------------------------------------------------------------- program
+// One problem's program, node-lane words; LEARNABLE_CONSTANT nodes carry
+// their compact ordinal in the first-child field.
+struct FitProg {
+    uint32_t w0, w1;
+    float cinit0, cinit1;   // C_init of node lane k (for the non-constant output slots)
+    uint64_t cm0, cm1;      // LEARNABLE_CONSTANT nodes
+    int L, root, nc, K;     // K = nc + (M > nc): live slots incl. the shared one
+    int32_t verr;
+    bool shared_nan_init;   // every non-constant slot starts as NaN
+    __device__ inline uint32_t word(int k) const { return k < WAVE ? rl_u32(w0, k) : rl_u32(w1, k - WAVE); }
+};
+
+__device__ inline FitProg load_fit_program(const FitArgs &a, int64_t b) {
+    Program<float> p = load_program<float>(a.ops, a.ch, a.C_init, b, a.M, 1, a.M, 1, a.M, a.n_x);
+    FitProg f;
+    const int lane = lane_id();
+    f.cm0 = p.cmask0;
+    f.cm1 = p.cmask1;
+    f.L = p.L;
+    f.root = p.root;
+    f.verr = p.verr;
+    f.cinit0 = p.c0;
+    f.cinit1 = p.c1;
+    const int n0 = __popcll(p.cmask0);
+    f.nc = n0 + __popcll(p.cmask1);
+    f.K = f.nc + (a.M > f.nc ? 1 : 0);
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
+    const int o0 = __popcll(p.cmask0 & below);
+    const int o1 = n0 + __popcll(p.cmask1 & below);
+    const uint32_t cw = (uint32_t)LEARNABLE_CONSTANT | (NULL_SLOT << 24);
+    f.w0 = ((p.cmask0 >> lane) & 1ull) ? (cw | ((uint32_t)o0 << 16)) : p.w0;
+    f.w1 = ((p.cmask1 >> lane) & 1ull) ? (cw | ((uint32_t)o1 << 16)) : p.w1;
+    const bool nonconst_finite0 = lane < a.M && !((p.cmask0 >> lane) & 1ull) && !(p.c0 != p.c0);
+    const bool nonconst_finite1 = lane + WAVE < a.M && !((p.cmask1 >> lane) & 1ull) && !(p.c1 != p.c1);
+    f.shared_nan_init = (__ballot(nonconst_finite0) | __ballot(nonconst_finite1)) == 0ull;
+    return f;
+}
+
+// Compact start point: lane j <- C_init of the j-th constant; shared slot 0.
+template <int SL>
+__device__ inline SVec<SL> compact_start(const FitProg &p) {
+    const int lane = lane_id();
+    SVec<SL> x;
+#pragma unroll
+    for (int h = 0; h < SL; ++h) x.v[h] = 0.0f;
+    int j = 0;
+    for (uint64_t m = p.cm0; m; m &= m - 1, ++j) {
+        const int bit = __ffsll((long long)m) - 1;
+        const float v = rl(p.cinit0, bit);
+        if (j < WAVE) { if (lane == j) x.v[0] = v; }
+        else if (lane == j - WAVE) x.v[SL - 1] = v;
+    }
+    for (uint64_t m = p.cm1; m; m &= m - 1, ++j) {
+        const int bit = __ffsll((long long)m) - 1;
+        const float v = rl(p.cinit1, bit);
+        if (j < WAVE) { if (lane == j) x.v[0] = v; }
+        else if (lane == j - WAVE) x.v[SL - 1] = v;
+    }
+    return x;
+}
+
+// ------------------------------------------------------------------ tapes
+// ------------------------------------------------------------ instructions
+// A problem's program is compiled once into an instruction list: one
+// instruction per operator node up to the root (leaves are folded into the
+// operand descriptors of their parents), plus a copy instruction when the
+// root itself is a leaf.  Instruction i writes tape slot i, so the root is
+// always the last slot.  Word: bits 0-5 opcode (the reference's operator id,
+// or OP_COPY), bits 6-18 and 19-31 the two operand descriptors
+// (2-bit kind, 11-bit index).
+constexpr int K_TAPE = 0, K_VAR = 1, K_CONST = 2, K_IMM = 3;
+constexpr int IMM_NOOP = 8;  // a NO_OP node used as an operand: NaN, and 201 if it receives a gradient
+constexpr int OP_COPY = 63;
+constexpr int MAX_NX_CODE = 2047;
+
+__host__ __device__ inline int instr_op(uint32_t w) { return (int)(w & 63u); }
+__host__ __device__ inline uint32_t instr_d0(uint32_t w) { return (w >> 6) & 0x1FFFu; }
+__host__ __device__ inline uint32_t instr_d1(uint32_t w) { return w >> 19; }
+__host__ __device__ inline int desc_kind(uint32_t d) { return (int)(d & 3u); }
+__host__ __device__ inline int desc_idx(uint32_t d) { return (int)(d >> 2); }
+
+// Descriptor of node word w as an operand of its parent.
+__device__ inline uint32_t operand_desc(uint32_t w, uint32_t instr_index, bool is_instr) {
+    const int op = node_op(w);
+    if (is_instr) return (uint32_t)K_TAPE | (instr_index << 2);
+    if (op == LEARNABLE_CONSTANT) return (uint32_t)K_CONST | ((uint32_t)node_c0(w) << 2);
+    if (op >= VAR_START_ID && op != UNKNOWN_LEAF) return (uint32_t)K_VAR | ((uint32_t)(op - VAR_START_ID) << 2);
+    if (op >= CONST_1 && op <= E) return (uint32_t)K_IMM | ((uint32_t)(op - 1) << 2);  // 1..5, 6 = pi, 7 = e
+    if (op == NO_OP) return (uint32_t)K_IMM | ((uint32_t)IMM_NOOP << 2);
+    return (uint32_t)K_IMM;  // unknown leaf: 0
+}
+
+// Instruction count of a batch-first program row under the same rule as
+// compile_code (used to bucket problems before they are compiled).
+__device__ inline int count_instructions(const int64_t *row, int M, int n_x, int &nc) {
+    int nn = 0;
+    nc = 0;
+    for (int k = 0; k < M; ++k) {
+        if (row[k] != NO_OP) ++nn;
+        if (row[k] == LEARNABLE_CONSTANT) ++nc;
+    }
+    const int root = nn > 0 ? nn - 1 : 0;
+    int ni = 0;
+    for (int k = 0; k <= root && k < M; ++k)
+        if (arity_of(node_op(pack_node(row[k], -1, -1, n_x))) > 0) ++ni;
+    const bool root_is_op = arity_of(node_op(pack_node(row[root], -1, -1, n_x))) > 0;
+    return ni + (root_is_op ? 0 : 1);
+}
+
+struct Code {
+    uint32_t iw0, iw1;  // lane i: instruction i (and 64 + i)
+    int NI;
+    __device__ inline uint32_t word(int i) const { return i < WAVE ? rl_u32(iw0, i) : rl_u32(iw1, i - WAVE); }
+};
+
+__device__ inline void place_word(Code &c, int i, uint32_t v) {
+    const int lane = lane_id();
+    if (i < WAVE) { if (lane == i) c.iw0 = v; }
+    else if (lane == i - WAVE) c.iw1 = v;
+}
+
+__device__ inline Code compile_code(const FitProg &p, int M) {
+    const int lane = lane_id();
+    const int root = p.root;
+    const uint32_t w0 = p.w0, w1 = p.w1;
+    const bool op0 = lane < M && lane <= root && arity_of(node_op(w0)) > 0;
+    const bool op1 = lane + WAVE < M && lane + WAVE <= root && arity_of(node_op(w1)) > 0;
+    const uint64_t om0 = __ballot(op0), om1 = __ballot(op1);
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
+    const uint32_t ix0 = (uint32_t)__popcll(om0 & below);
+    const uint32_t ix1 = (uint32_t)(__popcll(om0) + __popcll(om1 & below));
+    const uint32_t d0 = operand_desc(w0, ix0, op0), d1 = operand_desc(w1, ix1, op1);
+    auto child_desc = [&](int c) -> uint32_t {  // every lane takes part in both shuffles
+        const int cc = c & 127;
+        const uint32_t lo = (uint32_t)__shfl((int)d0, cc & (WAVE - 1), WAVE);
+        const uint32_t hi = (uint32_t)__shfl((int)d1, cc & (WAVE - 1), WAVE);
+        return cc < WAVE ? lo : hi;
+    };
+    const uint32_t c00 = child_desc(node_c0(w0)), c01 = child_desc(node_c1(w0));
+    const uint32_t c10 = child_desc(node_c0(w1)), c11 = child_desc(node_c1(w1));
+    const uint32_t i0 = (uint32_t)node_op(w0) | (c00 << 6) | ((arity_of(node_op(w0)) == 2 ? c01 : 0u) << 19);
+    const uint32_t i1 = (uint32_t)node_op(w1) | (c10 << 6) | ((arity_of(node_op(w1)) == 2 ? c11 : 0u) << 19);
+    Code c;
+    c.iw0 = 0u;
+    c.iw1 = 0u;
+    int i = 0;
+    for (uint64_t m = om0; m; m &= m - 1, ++i) place_word(c, i, rl_u32(i0, __ffsll((long long)m) - 1));
+    for (uint64_t m = om1; m; m &= m - 1, ++i) place_word(c, i, rl_u32(i1, __ffsll((long long)m) - 1));
+    const uint32_t rw = root < WAVE ? rl_u32(w0, root) : rl_u32(w1, root - WAVE);
+    if (arity_of(node_op(rw)) == 0) {
+        const uint32_t rd = root < WAVE ? rl_u32(d0, root) : rl_u32(d1, root - WAVE);
+        place_word(c, i, (uint32_t)OP_COPY | (rd << 6));
+        ++i;
+    }
+    c.NI = i;
+    return c;
+}
+
+// Tape of one wave: the value of instruction i at point p of a tile group is
+// v[(i*PPT + p)*64 + lane], its adjoint a[...] likewise; a holds one extra
+// row group (index LT) that absorbs the adjoint writes of non-tape operands.
+// The arrays are in LDS (bank-conflict free: 64 lanes touch 64 consecutive
+// words) or, for the longest programs, in a global workspace slot.  Each lane
+// only touches its own column, so a write needs no synchronisation before
+// the same lane reads it back.
+template <int PPT>
+struct Tape {
+    float *v;
+    float *a;
+    __device__ inline float val(int i, int p) const { return v[(i * PPT + p) * WAVE + lane_id()]; }
+    __device__ inline void set(int i, int p, float x) const { v[(i * PPT + p) * WAVE + lane_id()] = x; }
+    __device__ inline float adj(int i, int p) const { return a[(i * PPT + p) * WAVE + lane_id()]; }
+    __device__ inline void set_adj(int i, int p, float x) const { a[(i * PPT + p) * WAVE + lane_id()] = x; }
+};
+
+__device__ inline float imm_value(int idx) {
+    return idx == 0 ? 0.0f
+                    : (idx <= 5 ? (float)idx
+                                : (idx == 6 ? (float)3.14159265358979323846
+                                            : (idx == 7 ? (float)2.71828182845904523536 : __builtin_nanf(""))));
+}
+
+// ---------------------------------------------------------------- closure
+// One wave's share of an objective evaluation at the compact constants xt:
+// tile groups of 64*PPT points starting at group `wslot`, every W-th group.
+// Returns the wave's FP64 loss sum (uniform), its FP64 dL/dC partials in
+// compact lanes (gc), and its error code (uniform).  Per instruction there is
+// one opcode dispatch for all PPT points; operand fetch and adjoint scatter
+// are selects on the (uniform) operand descriptors.
+template <int SL, int NX, int PPT, int LT>
+__device__ inline void closure_code(const Code &c, int nc, const SVec<SL> &xt, const float *X, const float *Yb,
+                                    int64_t N, int n_x, int wslot, int W, const Tape<PPT> &t, double &loss,
+                                    double (&gc)[SL], int32_t &err) {
+    const int lane = lane_id();
+    const int NI = c.NI;
+    double lacc = 0.0;
+    double cacc[FIT_NCREG];
+#pragma unroll
+    for (int q = 0; q < FIT_NCREG; ++q) cacc[q] = 0.0;
+#pragma unroll
+    for (int h = 0; h < SL; ++h) gc[h] = 0.0;
+    int32_t e = 0;
+    const float inv_n = 1.0f / (float)N;
+    const int64_t group = (int64_t)WAVE * PPT;
+    for (int64_t base = (int64_t)wslot * group; base < N; base += (int64_t)W * group) {
+        bool act[PPT];
+        int64_t n[PPT];
+        float xv[PPT][NX], target[PPT];
+#pragma unroll
+        for (int p = 0; p < PPT; ++p) {
+            n[p] = base + p * WAVE + lane;
+            act[p] = n[p] < N;
+            const float *row = X + (act[p] ? n[p] : 0) * (int64_t)n_x;
+#pragma unroll
+            for (int v = 0; v < NX; ++v) xv[p][v] = (act[p] && v < n_x) ? row[v] : 0.0f;
+            target[p] = act[p] ? Yb[n[p]] : 0.0f;
+        }
+        auto fetch = [&](uint32_t d, float (&a)[PPT]) {
+            const int kind = desc_kind(d), idx = desc_idx(d);
+            const int tidx = kind == K_TAPE ? idx : 0;
+            const int vidx = kind == K_VAR ? idx : 0;
+            const float cv = slot_get<SL>(xt, kind == K_CONST ? idx : 0);
+            const float iv = imm_value(kind == K_IMM ? idx : 0);
+#pragma unroll
+            for (int p = 0; p < PPT; ++p) {
+                const float tv = t.val(tidx, p);
+                float vv = 0.0f;
+                if (n_x <= NX) {
+#pragma unroll
+                    for (int v = 0; v < NX; ++v) vv = (v == vidx) ? xv[p][v] : vv;
+                } else if (kind == K_VAR) {
+                    vv = act[p] ? X[n[p] * (int64_t)n_x + vidx] : 0.0f;
+                }
+                a[p] = kind == K_TAPE ? tv : (kind == K_VAR ? vv : (kind == K_CONST ? cv : iv));
+            }
+        };
+        // ---- forward; adjoints are zeroed on the way
+        for (int i = 0; i < NI; ++i) {
+            const uint32_t w = c.word(i);
+            const int op = instr_op(w);
+            float a0[PPT], a1[PPT], r[PPT];
+            fetch(instr_d0(w), a0);
+            if (op >= ADD && op <= POW) {
+                fetch(instr_d1(w), a1);
+            } else {
+#pragma unroll
+                for (int p = 0; p < PPT; ++p) a1[p] = 0.0f;
+            }
+            if (op == OP_COPY) {
+#pragma unroll
+                for (int p = 0; p < PPT; ++p) r[p] = a0[p];
+            } else {
+                fwd_nodes<float, PPT>(op, a0, a1, r);
+            }
+#pragma unroll
+            for (int p = 0; p < PPT; ++p) {
+                t.set(i, p, r[p]);
+                t.set_adj(i, p, 0.0f);
+            }
+        }
+        // ---- loss and the MSE upstream gradient (2 * (y - Y)) * (1 / N)
+#pragma unroll
+        for (int p = 0; p < PPT; ++p) {
+            const float d = t.val(NI - 1, p) - target[p];
+            if (act[p]) lacc += (double)d * (double)d;
+            t.set_adj(NI - 1, p, act[p] ? (2.0f * d) * inv_n : 0.0f);
+        }
+        // ---- reverse sweep
+        for (int i = NI - 1; i >= 0; --i) {
+            const uint32_t w = c.word(i);
+            const int op = instr_op(w);
+            const bool binary = op >= ADD && op <= POW;
+            float g[PPT], a0[PPT], a1[PPT], go0[PPT], go1[PPT];
+            bool add0[PPT], add1[PPT];
+            int32_t ee[PPT];
+#pragma unroll
+            for (int p = 0; p < PPT; ++p) g[p] = t.adj(i, p);
+            fetch(instr_d0(w), a0);
+            if (binary) {
+                fetch(instr_d1(w), a1);
+            } else {
+#pragma unroll
+                for (int p = 0; p < PPT; ++p) a1[p] = 0.0f;
+            }
+            if (op == OP_COPY) {
+#pragma unroll
+                for (int p = 0; p < PPT; ++p) {
+                    go0[p] = g[p];
+                    add0[p] = true;
+                    go1[p] = 0.0f;
+                    add1[p] = false;
+                    ee[p] = 0;
+                }
+            } else {
+                bwd_nodes<float, PPT>(op, g, a0, a1, go0, add0, go1, add1, ee);
+            }
+#pragma unroll
+            for (int p = 0; p < PPT; ++p)
+                if (act[p] && ee[p] > e) e = ee[p];
+            // scatter to the operands
+            auto scatter = [&](uint32_t d, const float (&go)[PPT], const bool (&add)[PPT]) {
+                const int kind = desc_kind(d), idx = desc_idx(d);
+                const int slot = kind == K_TAPE ? idx : LT;  // row LT absorbs the other kinds
+#pragma unroll
+                for (int p = 0; p < PPT; ++p) {
+                    const float o = t.adj(slot, p);
+                    t.set_adj(slot, p, (act[p] && add[p]) ? o + go[p] : o);
+                    if (kind == K_IMM && idx == IMM_NOOP && act[p] && add[p] && go[p] != 0.0f)
+                        e = e > ERR_GRAD_NO_OP ? e : ERR_GRAD_NO_OP;
+                }
+                if (kind == K_CONST) {
+                    double gv = 0.0;
+#pragma unroll
+                    for (int p = 0; p < PPT; ++p) gv += (act[p] && add[p]) ? (double)go[p] : 0.0;
+                    if (idx < FIT_NCREG) {
+#pragma unroll
+                        for (int q = 0; q < FIT_NCREG; ++q)
+                            if (q == idx) cacc[q] += gv;
+                    } else {
+                        const double s = wave_sum_rows(gv, 4);
+                        if (idx < WAVE) { if (lane == idx) gc[0] += s; }
+                        else if (lane == idx - WAVE) gc[SL - 1] += s;
+                    }
+                }
+            };
+            scatter(instr_d0(w), go0, add0);
+            if (binary) scatter(instr_d1(w), go1, add1);
+        }
+    }
+    loss = wave_sum_rows(lacc, 4);
+    const int ncr = nc < FIT_NCREG ? nc : FIT_NCREG;
+#pragma unroll
+    for (int q = 0; q < FIT_NCREG; ++q) {
+        if (q < ncr) {
+            const double s = wave_sum_rows(cacc[q], 4);
+            if (lane == q) gc[0] = s;
+        }
+    }
+    err = wave_max_err(e);
+}
+
+// ----------------------------------------------------------------- kernel
+// W == 1: a block holds block_waves independent waves, one problem each
+// (4, or 2 with two slot registers per lane, whose history is twice as large).
+// W > 1:  a block of W waves works on one problem.
+__host__ __device__ constexpr int block_waves(int W, int SL) { return W == 1 ? (SL == 1 ? 4 : 2) : W; }
+
+// Per-group LDS: history S, Y [hist][SL*64] floats, rho [MHMAX] doubles.
+__host__ __device__ inline size_t hist_bytes(int hist, int SL) {
+    return (size_t)2 * hist * SL * WAVE * sizeof(float) + (size_t)MHMAX * sizeof(double);
+}
+__host__ __device__ inline size_t part_bytes(int W, int SL) {
+    return W > 1 ? (size_t)2 * W * (SL * WAVE + 2) * sizeof(double) : 0;
+}
+// Tape of one wave: LT value rows and LT + 1 adjoint rows per point of a
+// tile group, 64 floats each.
+__host__ __device__ inline size_t tape_floats(int LT, int PPT) { return (size_t)(2 * LT + 1) * PPT * WAVE; }
+__host__ __device__ inline size_t tape_bytes(int LT, int PPT, bool mem) {
+    return mem ? 0 : tape_floats(LT, PPT) * sizeof(float);
+}
+__host__ __device__ inline size_t fit_lds(int LT, int PPT, bool mem, int W, int SL, int hist) {
+    const int groups = W == 1 ? block_waves(W, SL) : 1;
+    return (size_t)block_waves(W, SL) * tape_bytes(LT, PPT, mem) + (size_t)groups * hist_bytes(hist, SL) +
+           part_bytes(W, SL) + 16;
+}
+
+// Internal error: a program reached a kernel whose tape it does not fit
+// (the bucketing rule and the compiler disagree).  Never expected.
+constexpr int32_t ERR_BUCKET = 900;
+
+enum FitState : int { ST_INIT = 0, ST_SW = 1, ST_SW_ZERO = 2, ST_ZOOM = 3, ST_BT = 4, ST_FINAL = 5 };
+
+// LT: most instructions of a program in the bucket; NX: variables held in
+// registers per point; SL: slot registers per lane; W: waves per problem;
+// MEM: tape in the global workspace instead of LDS; WPE: waves per SIMD the
+// register budget is sized for; PPT: points per lane per dispatch.
+template <int LT, int NX, int SL, int W, bool MEM, int WPE, int PPT>
+__global__ __launch_bounds__(block_waves(W, SL) * 64) __attribute__((amdgpu_waves_per_eu(WPE)))
+void k_fit(FitArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int wave = uniform_i32((int)(threadIdx.x / WAVE));
+    const int lane = lane_id();
+    const int group = W == 1 ? wave : 0;   // problem slot of this wave inside the block
+    const int wslot = W == 1 ? 0 : wave;   // this wave's share of the points
+    const int hist = a.history;
+    Tape<PPT> tape;
+    {
+        float *slot = MEM ? a.gtape + ((size_t)blockIdx.x * block_waves(W, SL) + wave) * tape_floats(LT, PPT)
+                          : reinterpret_cast<float *>(smem) + (size_t)wave * tape_floats(LT, PPT);
+        tape.v = slot;
+        tape.a = slot + (size_t)LT * PPT * WAVE;
+    }
+    unsigned char *hbase = smem + (size_t)block_waves(W, SL) * tape_bytes(LT, PPT, MEM);
+    const int groups = W == 1 ? block_waves(W, SL) : 1;
+    unsigned char *gbase = hbase + (size_t)group * hist_bytes(hist, SL);
+    float *Sh = reinterpret_cast<float *>(gbase);
+    float *Yh = Sh + (size_t)hist * SL * WAVE;
+    double *rho_l = reinterpret_cast<double *>(Yh + (size_t)hist * SL * WAVE);
+    double *part = reinterpret_cast<double *>(hbase + (size_t)groups * hist_bytes(hist, SL));
+    int *next_slot = reinterpret_cast<int *>(hbase + (size_t)groups * hist_bytes(hist, SL) + part_bytes(W, SL));
+    const int64_t count = (int64_t)*a.list_count;
+    const double c1 = 1e-4, c2 = 0.9;
+
+    int buf = 0;
+
+    for (;;) {
+        int i;
+        if (W == 1) {
+            i = wave_fetch_add(a.work_counter);
+        } else {
+            if (threadIdx.x == 0) *next_slot = atomicAdd(a.work_counter, 1);
+            __syncthreads();
+            i = uniform_i32(*next_slot);
+            __syncthreads();
+        }
+        if (i >= count) break;
+        const int64_t b = (int64_t)a.list[i];
+        const FitProg p = load_fit_program(a, b);
+        const bool writer = (wslot == 0);
+        if (p.verr) {
+            if (writer) {
+                if (lane < a.M) a.C_out[b * a.M + lane] = p.cinit0;
+                if (lane + WAVE < a.M) a.C_out[b * a.M + lane + WAVE] = p.cinit1;
+                if (lane == 0) {
+                    atomicMax(&a.flags[0], p.verr);
+                    a.err_out[b] = p.verr;
+                    a.iters_out[b] = 0;
+                    if (a.nclos_out) a.nclos_out[b] = 0;
+                    a.loss_out[b] = __builtin_nan("");
+                }
+            }
+            continue;
+        }
+        const Code code = compile_code(p, a.M);
+        if (code.NI > LT) {
+            if (writer && lane == 0) {
+                atomicMax(&a.flags[1], ERR_BUCKET);
+                a.err_out[b] = ERR_BUCKET;
+                a.iters_out[b] = 0;
+                if (a.nclos_out) a.nclos_out[b] = 0;
+                a.loss_out[b] = __builtin_nan("");
+            }
+            continue;
+        }
+        const float *X = a.X + (a.x_off ? a.x_off[b] : 0);
+        const float *Yb = a.Y + (a.y_off ? a.y_off[b] : b * a.N);
+        const int64_t N = a.n_pts ? (int64_t)a.n_pts[b] : a.N;
+        const double stop = a.stop_loss ? a.stop_loss[b] : -1.0;
+        const int K = p.K;
+
+        SVec<SL> x = compact_start<SL>(p), g, d, g_prev, g_lo, g_new, ga;
+#pragma unroll
+        for (int h = 0; h < SL; ++h) { g.v[h] = d.v[h] = g_prev.v[h] = g_lo.v[h] = g_new.v[h] = ga.v[h] = 0.0f; }
+        int st = ST_INIT;
+        float a_eval = 0.0f, ai = 1.0f, a_prev = 0.0f, lo = 0.0f, hi = 0.0f, alpha = 0.0f;
+        double f = 0.0, f_new = 0.0, dphi0 = 0.0, phi_prev = 0.0, phi_lo = 0.0;
+        int ls = 0, zi = 0, it = 0, nh = 0, head = 0, nclos = 0;
+        bool d_finite = true;
+        int32_t err = 0;
+
+        for (;;) {
+            // ---------------- one objective evaluation at x (+ a_eval d)
+            SVec<SL> xt;
+#pragma unroll
+            for (int h = 0; h < SL; ++h) xt.v[h] = (st == ST_INIT) ? x.v[h] : x.v[h] + a_eval * d.v[h];
+            double phi, gcd[SL];
+            int32_t e2;
+            closure_code<SL, NX, PPT, LT>(code, p.nc, xt, X, Yb, N, a.n_x, wslot, W, tape, phi, gcd, e2);
+            if (W > 1) {
+                // partial sums of the W waves meet in LDS; every wave adds them
+                // in the same order, so all waves hold identical results
+                double *pb = part + (size_t)buf * W * (SL * WAVE + 2);
+                double *mine = pb + (size_t)wslot * (SL * WAVE + 2);
+                if (lane == 0) { mine[0] = phi; mine[1] = (double)e2; }
+#pragma unroll
+                for (int h = 0; h < SL; ++h) mine[2 + h * WAVE + lane] = gcd[h];
+                __syncthreads();
+                phi = 0.0;
+                int32_t em = 0;
+#pragma unroll
+                for (int h = 0; h < SL; ++h) gcd[h] = 0.0;
+                for (int w = 0; w < W; ++w) {
+                    const double *o = pb + (size_t)w * (SL * WAVE + 2);
+                    phi += o[0];
+                    const int32_t ew = (int32_t)o[1];
+                    em = ew > em ? ew : em;
+#pragma unroll
+                    for (int h = 0; h < SL; ++h) gcd[h] += o[2 + h * WAVE + lane];
+                }
+                phi = uniform_f64(phi);
+                e2 = uniform_i32(em);
+                buf ^= 1;
+            }
+            phi = phi / (double)N;
+#pragma unroll
+            for (int h = 0; h < SL; ++h) ga.v[h] = (float)gcd[h];
+            ++nclos;
+            if (e2) { err = e2; break; }
+
+            // ---------------- advance the iteration's state
+            bool accept = false;  // (alpha, f_new, g_new) decided
+            bool eval_final = false;
+            bool zoom_next = false;
+            if (st == ST_INIT) {
+                f = phi;
+                g = ga;
+            } else if (st == ST_FINAL) {
+                f_new = phi;
+                g_new = ga;
+                accept = true;
+            } else if (st == ST_SW) {
+                const double dphi = sdot<SL>(ga, d, K);
+                const bool bad = (phi > f + c1 * (double)ai * dphi0) || (ls > 0 && phi >= phi_prev);
+                const bool good = fabs(dphi) <= c2 * fabs(dphi0);
+                if (good) {
+                    alpha = ai; f_new = phi; g_new = ga; accept = true;
+                } else if (bad || dphi >= 0.0) {
+                    lo = bad ? a_prev : ai;
+                    hi = bad ? ai : a_prev;
+                    phi_lo = bad ? phi_prev : phi;
+                    g_lo = bad ? g_prev : ga;
+                    zi = 0;
+                    if (bad && ls == 0 && !d_finite) {
+                        st = ST_SW_ZERO;  // x + 0*d is not x: evaluate it
+                        a_eval = 0.0f;
+                        continue;
+                    }
+                    zoom_next = true;
+                } else {
+                    a_prev = ai;
+                    phi_prev = phi;
+                    g_prev = ga;
+                    ai = ai + (20.0f - ai) * 0.8f;
+                    ++ls;
+                    if (ls < a.ls_max_iter) { a_eval = ai; continue; }
+                    alpha = ai;
+                    eval_final = true;
+                }
+            } else if (st == ST_SW_ZERO) {
+                phi_lo = phi;
+                g_lo = ga;
+                zoom_next = true;
+            } else if (st == ST_ZOOM) {
+                const float aj = a_eval;
+                const double dphij = sdot<SL>(ga, d, K);
+                const bool zbad = (phi > f + c1 * (double)aj * dphi0) || (phi >= phi_lo);
+                if (fabs(dphij) <= c2 * fabs(dphi0)) {
+                    alpha = aj; f_new = phi; g_new = ga; accept = true;
+                } else {
+                    const float new_hi = zbad ? aj : hi;
+                    const bool flip = dphij * (double)(new_hi - lo) >= 0.0;
+                    hi = (!zbad && flip) ? lo : new_hi;
+                    if (!zbad) { lo = aj; phi_lo = phi; g_lo = ga; }
+                    if (fabsf(hi - lo) < 1e-9f) {
+                        alpha = aj; f_new = phi; g_new = ga; accept = true;
+                    } else {
+                        ++zi;
+                        zoom_next = true;
+                    }
+                }
+            } else {  // ST_BT
+                if (phi <= f + 1e-4 * (double)ai * dphi0) {
+                    alpha = ai; f_new = phi; g_new = ga; accept = true;
+                } else {
+                    ai = ai * 0.5f;
+                    ++ls;
+                    if (ls < a.ls_max_iter) { a_eval = ai; continue; }
+                    alpha = ai;
+                    eval_final = true;
+                }
+            }
+            if (zoom_next) {
+                if (zi < 10) {
+                    a_eval = (lo + hi) / 2.0f;
+                    st = ST_ZOOM;
+                    continue;
+                }
+                alpha = lo; f_new = phi_lo; g_new = g_lo; accept = true;
+            }
+            if (eval_final) {
+                // no point was accepted: step by alpha and evaluate there
+                st = ST_FINAL;
+                a_eval = alpha;
+                continue;
+            }
+            if (accept) {
+                // ---------------- step and history push
+                SVec<SL> s, yv;
+#pragma unroll
+                for (int h = 0; h < SL; ++h) {
+                    s.v[h] = alpha * d.v[h];
+                    x.v[h] = x.v[h] + s.v[h];
+                    yv.v[h] = g_new.v[h] - g.v[h];
+                }
+                const double rho_new = 1.0 / (sdot<SL>(yv, s, K) + a.eps);
+                int slot;
+                if (nh == hist) {
+                    slot = head;
+                    head = (head + 1 == hist) ? 0 : head + 1;
+                } else {
+                    slot = head + nh < hist ? head + nh : head + nh - hist;
+                    ++nh;
+                }
+#pragma unroll
+                for (int h = 0; h < SL; ++h) {
+                    Sh[((size_t)slot * SL + h) * WAVE + lane] = s.v[h];
+                    Yh[((size_t)slot * SL + h) * WAVE + lane] = yv.v[h];
+                }
+                if (lane == 0) rho_l[slot] = rho_new;
+                g = g_new;
+                f = f_new;
+                ++it;
+            }
+            // ---------------- start of an iteration: tests, direction, line search
+            if (it >= a.max_iter) break;
+            const double gn = sqrt(sdot<SL>(g, g, K));
+            if (gn < a.tol_grad) break;
+            if (f <= stop) break;
+            if (nh > 0) {
+                bool notnan = false;
+#pragma unroll
+                for (int h = 0; h < SL; ++h) {
+                    const int j = lane + h * WAVE;
+                    if (j < K) {
+                        const bool isn = (x.v[h] != x.v[h]) || (j == p.nc && p.shared_nan_init);
+                        if (!isn) notnan = true;
+                    }
+                }
+                if (__ballot(notnan) == 0ull) break;
+            }
+            // two-loop recursion, newest pair first; alpha_j kept in lane j
+            SVec<SL> q = g;
+            double alph_lane = 0.0;
+            for (int jj = nh - 1; jj >= 0; --jj) {
+                const int sl = head + jj < hist ? head + jj : head + jj - hist;
+                SVec<SL> sj, yj;
+#pragma unroll
+                for (int h = 0; h < SL; ++h) {
+                    sj.v[h] = Sh[((size_t)sl * SL + h) * WAVE + lane];
+                    yj.v[h] = Yh[((size_t)sl * SL + h) * WAVE + lane];
+                }
+                const double aj = uniform_f64(rho_l[sl]) * sdot<SL>(sj, q, K);
+                if (lane == jj) alph_lane = aj;
+                const float af = (float)aj;
+#pragma unroll
+                for (int h = 0; h < SL; ++h) q.v[h] = q.v[h] - af * yj.v[h];
+            }
+            SVec<SL> z = q;
+            if (nh > 0) {
+                const int sl = head + nh - 1 < hist ? head + nh - 1 : head + nh - 1 - hist;
+                SVec<SL> sn, yn;
+#pragma unroll
+                for (int h = 0; h < SL; ++h) {
+                    sn.v[h] = Sh[((size_t)sl * SL + h) * WAVE + lane];
+                    yn.v[h] = Yh[((size_t)sl * SL + h) * WAVE + lane];
+                }
+                const float gamma = (float)(sdot<SL>(sn, yn, K) / (sdot<SL>(yn, yn, K) + a.eps));
+#pragma unroll
+                for (int h = 0; h < SL; ++h) z.v[h] = gamma * q.v[h];
+                for (int jj = 0; jj < nh; ++jj) {
+                    const int sj_ = head + jj < hist ? head + jj : head + jj - hist;
+                    SVec<SL> sj, yj;
+#pragma unroll
+                    for (int h = 0; h < SL; ++h) {
+                        sj.v[h] = Sh[((size_t)sj_ * SL + h) * WAVE + lane];
+                        yj.v[h] = Yh[((size_t)sj_ * SL + h) * WAVE + lane];
+                    }
+                    const double beta = uniform_f64(rho_l[sj_]) * sdot<SL>(yj, z, K);
+                    const float coef = (float)(uniform_f64(rl(alph_lane, jj)) - beta);
+#pragma unroll
+                    for (int h = 0; h < SL; ++h) z.v[h] = z.v[h] + sj.v[h] * coef;
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < SL; ++h) d.v[h] = -z.v[h];
+            dphi0 = sdot<SL>(g, d, K);
+            {
+                bool bad = false;
+#pragma unroll
+                for (int h = 0; h < SL; ++h)
+                    if (lane + h * WAVE < K && !isfinite(d.v[h])) bad = true;
+                d_finite = __ballot(bad) == 0ull;
+            }
+            ls = 0;
+            ai = 1.0f;
+            if (a.line_search == 0) {
+                a_prev = 0.0f;
+                phi_prev = f;
+                g_prev = g;
+                st = ST_SW;
+            } else {
+                st = ST_BT;
+            }
+            if (a.ls_max_iter <= 0) {
+                alpha = ai;
+                st = ST_FINAL;
+            }
+            a_eval = ai;
+        }
+
+        // ---------------- write back all M slots
+        if (writer) {
+            const float shared = slot_get<SL>(x, p.nc < 2 * WAVE ? p.nc : 0);
+            const bool shared_nan = (K > p.nc) && (shared != shared);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int k = lane + h * WAVE;
+                const uint64_t cm = h == 0 ? p.cm0 : p.cm1;
+                const uint64_t below = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
+                const int o = (h == 0 ? 0 : __popcll(p.cm0)) + __popcll(cm & below);
+                // every lane takes part in the shuffles
+                const float v0 = __shfl(x.v[0], o & (WAVE - 1), WAVE);
+                const float v1 = __shfl(x.v[SL - 1], o & (WAVE - 1), WAVE);
+                if (k < a.M) {
+                    float out;
+                    if ((cm >> lane) & 1ull) out = (SL == 1 || o < WAVE) ? v0 : v1;
+                    else out = shared_nan ? __builtin_nanf("") : (h == 0 ? p.cinit0 : p.cinit1);
+                    a.C_out[b * a.M + k] = out;
+                }
+            }
+            if (lane == 0) {
+                a.loss_out[b] = f;
+                a.iters_out[b] = it;
+                if (a.nclos_out) a.nclos_out[b] = nclos;
+                a.err_out[b] = err;
+                if (err) atomicMax(&a.flags[1], err);
+            }
+        }
+    }
+}
+
+// Buckets problems by instruction count NI (operator nodes up to the root,
+// see compile_code) and live slot count K: NI <= 8, <= 16, <= 32, <= 64
+// (with K <= 64), and the rest.
+__global__ void k_classify(const int64_t *__restrict__ ops, int M, int64_t B, int n_x, int32_t *lists,
+                           int32_t *counts) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    int nc = 0;
+    const int ni = count_instructions(ops + b * M, M, n_x, nc);
+    const int K = nc + (M > nc ? 1 : 0);
+    const int bucket = K > 64 ? 4 : (ni <= 8 ? 0 : (ni <= 16 ? 1 : (ni <= 32 ? 2 : (ni <= 64 ? 3 : 4))));
+    const int slot = atomicAdd(&counts[bucket], 1);
+    lists[(int64_t)bucket * B + slot] = (int32_t)b;
+}
+
+__global__ void k_fill_i32_f(int32_t *p, int32_t v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------ launch plan
+template <int LT, int NX, int SL, int W, bool MEM, int WPE, int PPT>
+int launch_fit(const FitArgs &a, hipStream_t s, int grid) {
+    const size_t lds = fit_lds(LT, PPT, MEM, W, SL, a.history);
+    if (lds > lds_limit_device()) return -3;
+    hipLaunchKernelGGL((k_fit<LT, NX, SL, W, MEM, WPE, PPT>), dim3((unsigned)grid), dim3(block_waves(W, SL) * 64),
+                       lds, s, a);
+    return (int)hipGetLastError();
+}
+
+// Persistent grid: as many blocks as the CUs can hold at once (cached per
+// history size, which sets the LDS request).
+template <int LT, int NX, int SL, int W, bool MEM, int WPE, int PPT>
+int grid_for(int hist) {
+    static int cache[MHMAX + 1] = {0};
+    if (cache[hist] == 0) {
+        int per_cu = 0;
+        const size_t lds = fit_lds(LT, PPT, MEM, W, SL, hist);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_fit<LT, NX, SL, W, MEM, WPE, PPT>,
+                                                         block_waves(W, SL) * 64, lds) != hipSuccess ||
+            per_cu <= 0)
+            per_cu = 1;
+        cache[hist] = num_cus() * per_cu;
+    }
+    return cache[hist];
+}
+
+// Bucket kernels <LT, NX, SL, W, MEM, WPE, PPT>.  Each bucket has a few
+// launch variants (waves per problem W x points per lane PPT); one is active
+// per bucket and process, chosen by sr_fit_set_variant (the autotuner) or the
+// built-in default.
+constexpr int NXR = 4;
+constexpr int NBUCKETS = 5;
+constexpr int B4_BLOCKS_PER_CU = 2;  // x 2 waves: 1024 long problems in flight
+
+inline int b4_grid() { return num_cus() * B4_BLOCKS_PER_CU; }
+
+template <int LT, int SL, int W, bool MEM, int WPE, int PPT>
+int launch_variant(const FitArgs &a, hipStream_t s) {
+    const int grid = MEM ? b4_grid() : grid_for<LT, NXR, SL, W, MEM, WPE, PPT>(a.history);
+    return launch_fit<LT, NXR, SL, W, MEM, WPE, PPT>(a, s, grid);
+}
+
+struct FitVariant {
+    int (*launch)(const FitArgs &, hipStream_t);
+    int waves;  // waves per problem
+    int ppt;    // points per lane per dispatch
+};
+
+const FitVariant kB0[] = {{launch_variant<8, 1, 1, false, 3, 2>, 1, 2},
+                          {launch_variant<8, 1, 2, false, 3, 2>, 2, 2},
+                          {launch_variant<8, 1, 4, false, 3, 2>, 4, 2},
+                          {launch_variant<8, 1, 1, false, 2, 4>, 1, 4}};
+const FitVariant kB1[] = {{launch_variant<16, 1, 1, false, 3, 2>, 1, 2},
+                          {launch_variant<16, 1, 2, false, 3, 2>, 2, 2},
+                          {launch_variant<16, 1, 4, false, 3, 2>, 4, 2}};
+const FitVariant kB2[] = {{launch_variant<32, 1, 2, false, 3, 1>, 2, 1},
+                          {launch_variant<32, 1, 4, false, 3, 1>, 4, 1},
+                          {launch_variant<32, 1, 4, false, 3, 2>, 4, 2}};
+const FitVariant kB3[] = {{launch_variant<64, 1, 4, false, 2, 1>, 4, 1},
+                          {launch_variant<64, 1, 2, false, 2, 1>, 2, 1}};
+const FitVariant kB4[] = {{launch_variant<128, 2, 1, true, 2, 1>, 1, 1}};
+
+const FitVariant *const kVariants[NBUCKETS] = {kB0, kB1, kB2, kB3, kB4};
+constexpr int kNumVariants[NBUCKETS] = {4, 3, 3, 2, 1};
+int g_variant[NBUCKETS] = {1, 2, 1, 1, 0};  // fastest on the bench workload (autotune.tune_fit)
+
+inline int launch_bucket(int k, const FitArgs &a, hipStream_t s) { return kVariants[k][g_variant[k]].launch(a, s); }
+
+// Workspace: lists [5][B] int32 | counters [5] counts + [5] work | global tape (bucket 4).
+inline size_t fit_gt_off(int64_t B) { return (((size_t)NBUCKETS * B * 4 + 64) + 255) / 256 * 256; }
+inline int64_t fit_ws(int64_t B, int64_t M) {
+    size_t off = fit_gt_off(B);
+    if (M > 64)  // more than 64 instructions or live slots needs more than 64 nodes
+        off += (size_t)b4_grid() * block_waves(1, 2) * tape_floats(128, 1) * sizeof(float);
+    return (int64_t)off + 256;
+}
+
+struct Fork {
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_join[3] = {nullptr, nullptr, nullptr};
+    bool ok = false;
+};
+
+// Side streams (default priority) and events of the current device, created
+// once per device and kept for the life of the process.
+Fork &fork_for_device() {
+    static Fork forks[64];
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    Fork &f = forks[dev & 63];
+    if (!f.ok) {
+        bool good = hipEventCreateWithFlags(&f.ev_fork, hipEventDisableTiming) == hipSuccess;
+        for (int i = 0; i < 3 && good; ++i) {
+            good = hipStreamCreateWithFlags(&f.side[i], hipStreamNonBlocking) == hipSuccess &&
+                   hipEventCreateWithFlags(&f.ev_join[i], hipEventDisableTiming) == hipSuccess;
+        }
+        f.ok = good;
+    }
+    return f;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t sr_lbfgs_workspace_bytes(int64_t B, int64_t M) { return fit_ws(B, M); }
+
+// Launch-variant control of the fit kernels (see include/sr_eval.h).
+int sr_fit_num_buckets(void) { return NBUCKETS; }
+int sr_fit_num_variants(int bucket) { return (bucket < 0 || bucket >= NBUCKETS) ? -1 : kNumVariants[bucket]; }
+int sr_fit_get_variant(int bucket) { return (bucket < 0 || bucket >= NBUCKETS) ? -1 : g_variant[bucket]; }
+int sr_fit_set_variant(int bucket, int variant) {
+    if (bucket < 0 || bucket >= NBUCKETS || variant < 0 || variant >= kNumVariants[bucket]) return -1;
+    g_variant[bucket] = variant;
+    return 0;
+}
+int sr_fit_variant_info(int bucket, int variant, int *waves, int *ppt) {
+    if (bucket < 0 || bucket >= NBUCKETS || variant < 0 || variant >= kNumVariants[bucket]) return -1;
+    if (waves) *waves = kVariants[bucket][variant].waves;
+    if (ppt) *ppt = kVariants[bucket][variant].ppt;
+    return 0;
+}
+
+// Fits the LEARNABLE_CONSTANT slots of B batch-first expressions to their
+// targets.  See include/sr_eval.h for the argument contract.
+int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                 const float *C_init, int64_t M, int64_t B, const int64_t *x_off, const int64_t *y_off,
+                 const int32_t *n_pts, const double *stop_loss, float *C_out, double *loss_out, int32_t *iters_out,
+                 int32_t *nclos_out, int32_t *err_out, int32_t *flags, int max_iter, int history, double tol_grad,
+                 int line_search, int ls_max_iter, double eps, void *workspace, int64_t ws_bytes, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (M <= 0 || M > 128 || B < 0 || n_x < 0 || n_x > MAX_NX_CODE + 1 || history < 1 || history > MHMAX ||
+        max_iter < 0 || ls_max_iter < 0 || (line_search != 0 && line_search != 1))
+        return -1;
+    if (n_pts == nullptr && N <= 0) return -1;
+    if (ws_bytes < sr_lbfgs_workspace_bytes(B, M)) return -2;
+    hipLaunchKernelGGL(k_fill_i32_f, dim3(1), dim3(64), 0, s, flags, 0, (int64_t)2);
+    if (B == 0) return (int)hipGetLastError();
+    char *ws = (char *)workspace;
+    int32_t *lists = (int32_t *)ws;
+    int32_t *counters = (int32_t *)(ws + (size_t)NBUCKETS * B * 4);  // [0..4] counts, [5..9] work
+    hipLaunchKernelGGL(k_fill_i32_f, dim3(1), dim3(64), 0, s, counters, 0, (int64_t)(2 * NBUCKETS));
+    hipLaunchKernelGGL(k_classify, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, ops, (int)M, B, (int)n_x,
+                       lists, counters);
+    FitArgs a;
+    memset(&a, 0, sizeof(a));
+    a.X = X; a.Y = Y; a.N = N; a.n_x = (int)n_x; a.ops = ops; a.ch = ch; a.C_init = C_init; a.M = (int)M; a.B = B;
+    a.x_off = x_off; a.y_off = y_off; a.n_pts = n_pts; a.stop_loss = stop_loss;
+    a.C_out = C_out; a.loss_out = loss_out; a.iters_out = iters_out; a.nclos_out = nclos_out; a.err_out = err_out;
+    a.flags = flags;
+    a.max_iter = max_iter; a.history = history; a.tol_grad = tol_grad; a.line_search = line_search;
+    a.ls_max_iter = ls_max_iter; a.eps = eps;
+    a.gtape = M > 64 ? (float *)(ws + fit_gt_off(B)) : nullptr;
+    auto bucket_args = [&](int k) {
+        FitArgs r = a;
+        r.list = lists + (size_t)k * B;
+        r.list_count = counters + k;
+        r.work_counter = counters + NBUCKETS + k;
+        return r;
+    };
+    int r = (int)hipGetLastError();
+    if (r) return r;
+    // A program has at most M instructions and M live slots, so short rows
+    // only need the first buckets.
+    if (M <= 8) return launch_bucket(0, bucket_args(0), s);
+    // Buckets 4+3, 2 and 1 on side streams, bucket 0 on the caller's stream,
+    // joined back before returning.
+    Fork &f = fork_for_device();
+    if (!f.ok) return (int)hipErrorOutOfMemory;
+    if ((r = (int)hipEventRecord(f.ev_fork, s))) return r;
+    for (int i = 0; i < 3; ++i)
+        if ((r = (int)hipStreamWaitEvent(f.side[i], f.ev_fork, 0))) return r;
+    if (M > 64 && (r = launch_bucket(4, bucket_args(4), f.side[0]))) return r;
+    if (M > 32 && (r = launch_bucket(3, bucket_args(3), f.side[0]))) return r;
+    if (M > 16 && (r = launch_bucket(2, bucket_args(2), f.side[1]))) return r;
+    if ((r = launch_bucket(1, bucket_args(1), f.side[2]))) return r;
+    if ((r = launch_bucket(0, bucket_args(0), s))) return r;
+    for (int i = 0; i < 3; ++i) {
+        if ((r = (int)hipEventRecord(f.ev_join[i], f.side[i]))) return r;
+        if ((r = (int)hipStreamWaitEvent(s, f.ev_join[i], 0))) return r;
+    }
+    return (int)hipGetLastError();
+}
+
+}  // extern "C"

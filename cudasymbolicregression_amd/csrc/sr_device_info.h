@@ -33,4 +33,18 @@ inline size_t lds_limit() {
     return v;
 }
 
+// The device's own per-workgroup LDS limit (160 KiB on gfx950), for the
+// kernels that are sized to use it.
+inline size_t lds_limit_device() {
+    static size_t v = 0;
+    if (v == 0) {
+        int dev = 0, c = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || c <= 0)
+            c = 64 * 1024;
+        v = (size_t)c;
+    }
+    return v;
+}
+
 }  // namespace sr

@@ -101,111 +101,151 @@ template <typename T> __device__ inline T log_w(T a) { return (a <= T(0)) ? Mth<
 template <typename T> __device__ inline T sqrt_w(T a) { return (a < T(0)) ? Mth<T>::qnan() : Mth<T>::sqrt_(a); }
 template <typename T> __device__ inline T pow_w(T a, T b) { return (a < T(0)) ? Mth<T>::qnan() : Mth<T>::pow_(a, b); }
 
-// Forward of one non-NO_OP node whose opcode is known to be an operator or a
-// constant (leaves VAR / LEARNABLE_CONSTANT / UNKNOWN_LEAF are resolved by the
-// caller).  `op` is wave-uniform, so the switch compiles to scalar branches.
-template <typename T>
-__device__ inline T fwd_node(int op, T a0, T a1) {
+// Forward of one non-NO_OP node for P points at once (P values per lane),
+// whose opcode is an operator or a constant (leaves VAR / LEARNABLE_CONSTANT /
+// UNKNOWN_LEAF are resolved by the caller).  `op` is wave-uniform, so the
+// switch compiles to scalar branches taken once for all P points.
+#define SR_EACH(expr)                                                                              \
+    {                                                                                              \
+        _Pragma("unroll") for (int p = 0; p < P; ++p) r[p] = (expr);                                \
+    }                                                                                              \
+    break
+template <typename T, int P>
+__device__ inline void fwd_nodes(int op, const T *a0, const T *a1, T *r) {
     const T nan = Mth<T>::qnan();
     switch (op) {
-    case CONST_1: return T(1.0);
-    case CONST_2: return T(2.0);
-    case CONST_3: return T(3.0);
-    case CONST_4: return T(4.0);
-    case CONST_5: return T(5.0);
-    case PI: return T(3.14159265358979323846);
-    case E: return T(2.71828182845904523536);
-    case SIN: return Mth<T>::sin_(a0);
-    case COS: return Mth<T>::cos_(a0);
-    case EXP: return Mth<T>::exp_(a0);
-    case LOG: return (a0 <= T(0)) ? nan : Mth<T>::log_(a0);
-    case SQUARE: return a0 * a0;
-    case SQRT: return (a0 < T(0)) ? nan : Mth<T>::sqrt_(a0);
-    case TAN: return Mth<T>::tan_(a0);
-    case ARCSIN: return (a0 < T(-1) || a0 > T(1)) ? nan : Mth<T>::asin_(a0);
-    case ARCCOS: return (a0 < T(-1) || a0 > T(1)) ? nan : Mth<T>::acos_(a0);
-    case ARCTAN: return Mth<T>::atan_(a0);
-    case SINH: return Mth<T>::sinh_(a0);
-    case COSH: return Mth<T>::cosh_(a0);
-    case TANH: return Mth<T>::tanh_(a0);
-    case FLOOR: return Mth<T>::floor_(a0);
-    case CEIL: return Mth<T>::ceil_(a0);
-    case LN: return (a0 <= T(0)) ? nan : Mth<T>::log_(a0);
-    case LOG10: return (a0 <= T(0)) ? nan : Mth<T>::log10_(a0);
-    case NEG: return -a0;
-    case INV: return (a0 == T(0)) ? nan : T(1.0) / a0;
-    case CUBE: return a0 * a0 * a0;
-    case FOURTH: { T sq = a0 * a0; return sq * sq; }
-    case FIFTH: { T sq = a0 * a0; return sq * sq * a0; }
-    case ADD: return a0 + a1;
-    case SUB: return a0 - a1;
-    case MUL: return a0 * a1;
-    case DIV: return (a1 == T(0)) ? nan : a0 / a1;
-    case POW: return (a0 < T(0)) ? nan : Mth<T>::pow_(a0, a1);
-    default: return T(0);
+    case CONST_1: SR_EACH(T(1.0));
+    case CONST_2: SR_EACH(T(2.0));
+    case CONST_3: SR_EACH(T(3.0));
+    case CONST_4: SR_EACH(T(4.0));
+    case CONST_5: SR_EACH(T(5.0));
+    case PI: SR_EACH(T(3.14159265358979323846));
+    case E: SR_EACH(T(2.71828182845904523536));
+    case SIN: SR_EACH(Mth<T>::sin_(a0[p]));
+    case COS: SR_EACH(Mth<T>::cos_(a0[p]));
+    case EXP: SR_EACH(Mth<T>::exp_(a0[p]));
+    case LOG: SR_EACH((a0[p] <= T(0)) ? nan : Mth<T>::log_(a0[p]));
+    case SQUARE: SR_EACH(a0[p] * a0[p]);
+    case SQRT: SR_EACH((a0[p] < T(0)) ? nan : Mth<T>::sqrt_(a0[p]));
+    case TAN: SR_EACH(Mth<T>::tan_(a0[p]));
+    case ARCSIN: SR_EACH((a0[p] < T(-1) || a0[p] > T(1)) ? nan : Mth<T>::asin_(a0[p]));
+    case ARCCOS: SR_EACH((a0[p] < T(-1) || a0[p] > T(1)) ? nan : Mth<T>::acos_(a0[p]));
+    case ARCTAN: SR_EACH(Mth<T>::atan_(a0[p]));
+    case SINH: SR_EACH(Mth<T>::sinh_(a0[p]));
+    case COSH: SR_EACH(Mth<T>::cosh_(a0[p]));
+    case TANH: SR_EACH(Mth<T>::tanh_(a0[p]));
+    case FLOOR: SR_EACH(Mth<T>::floor_(a0[p]));
+    case CEIL: SR_EACH(Mth<T>::ceil_(a0[p]));
+    case LN: SR_EACH((a0[p] <= T(0)) ? nan : Mth<T>::log_(a0[p]));
+    case LOG10: SR_EACH((a0[p] <= T(0)) ? nan : Mth<T>::log10_(a0[p]));
+    case NEG: SR_EACH(-a0[p]);
+    case INV: SR_EACH((a0[p] == T(0)) ? nan : T(1.0) / a0[p]);
+    case CUBE: SR_EACH(a0[p] * a0[p] * a0[p]);
+    case FOURTH: SR_EACH((a0[p] * a0[p]) * (a0[p] * a0[p]));
+    case FIFTH: SR_EACH((a0[p] * a0[p]) * (a0[p] * a0[p]) * a0[p]);
+    case ADD: SR_EACH(a0[p] + a1[p]);
+    case SUB: SR_EACH(a0[p] - a1[p]);
+    case MUL: SR_EACH(a0[p] * a1[p]);
+    case DIV: SR_EACH((a1[p] == T(0)) ? nan : a0[p] / a1[p]);
+    case POW: SR_EACH((a0[p] < T(0)) ? nan : Mth<T>::pow_(a0[p], a1[p]));
+    default: SR_EACH(T(0));
     }
 }
+#undef SR_EACH
 
-// Backward of one operator node (arity >= 1) for one point.
-// g is the node's adjoint.  On return *add0 / *add1 tell whether go0 / go1
-// are added to the children (the reference skips the add when g == 0, except
-// SQRT and POW which always add).  Returns ERR_DIV_ZERO or 0.
 template <typename T>
-__device__ inline int32_t bwd_node(int op, T g, T a0, T a1, T &go0, bool &add0, T &go1, bool &add1) {
+__device__ inline T fwd_node(int op, T a0, T a1) {
+    T r;
+    fwd_nodes<T, 1>(op, &a0, &a1, &r);
+    return r;
+}
+
+// Backward of one operator node (arity >= 1) for P points at once.
+// g[p] is the node's adjoint at point p.  On return add0[p] / add1[p] tell
+// whether go0[p] / go1[p] are added to the children (the reference skips the
+// add when g == 0, except SQRT and POW which always add), and e[p] holds
+// ERR_DIV_ZERO or 0.
+template <typename T, int P>
+__device__ inline void bwd_nodes(int op, const T *g, const T *a0, const T *a1, T *go0, bool *add0, T *go1,
+                                 bool *add1, int32_t *e) {
     const T zero = T(0);
     const T nan = Mth<T>::qnan();
-    const bool gz = (g == zero);
-    add0 = false;
-    add1 = false;
-    switch (op) {
-    case SIN: go0 = g * Mth<T>::cos_(a0); add0 = !gz; return 0;
-    case COS: go0 = g * -Mth<T>::sin_(a0); add0 = !gz; return 0;
-    case EXP: go0 = g * Mth<T>::exp_(a0); add0 = !gz; return 0;
-    case LOG: case LN: go0 = div_w(g, a0); add0 = !gz; return 0;
-    case SQUARE: go0 = g * (T(2.0) * a0); add0 = !gz; return 0;
-    case SQRT:
-        go0 = (a0 < zero) ? nan : (gz ? zero : div_w(g * T(0.5), sqrt_w(a0)));
-        add0 = true;
-        return 0;
-    case TAN: { T c = Mth<T>::cos_(a0); go0 = div_w(g, c * c); add0 = !gz; return 0; }
-    case ARCSIN:
-        go0 = (a0 < T(-1) || a0 > T(1)) ? nan : div_w(g, sqrt_w(T(1.0) - a0 * a0));
-        add0 = !gz;
-        return 0;
-    case ARCCOS:
-        go0 = (a0 < T(-1) || a0 > T(1)) ? nan : -div_w(g, sqrt_w(T(1.0) - a0 * a0));
-        add0 = !gz;
-        return 0;
-    case ARCTAN: go0 = div_w(g, T(1.0) + a0 * a0); add0 = !gz; return 0;
-    case SINH: go0 = g * Mth<T>::cosh_(a0); add0 = !gz; return 0;
-    case COSH: go0 = g * Mth<T>::sinh_(a0); add0 = !gz; return 0;
-    case TANH: { T t = Mth<T>::tanh_(a0); go0 = g * (T(1.0) - t * t); add0 = !gz; return 0; }
-    case FLOOR: case CEIL: return 0;
-    case LOG10: go0 = div_w(g, a0 * log_w(T(10.0))); add0 = !gz; return 0;
-    case NEG: go0 = -g; add0 = !gz; return 0;
-    case INV: go0 = (a0 == zero) ? nan : -div_w(g, a0 * a0); add0 = !gz; return 0;
-    case CUBE: go0 = g * (T(3.0) * (a0 * a0)); add0 = !gz; return 0;
-    case FOURTH: go0 = g * (T(4.0) * (a0 * a0 * a0)); add0 = !gz; return 0;
-    case FIFTH: { T sq = a0 * a0; go0 = g * (T(5.0) * (sq * sq)); add0 = !gz; return 0; }
-    case ADD: go0 = g; go1 = g; add0 = add1 = !gz; return 0;
-    case SUB: go0 = g; go1 = -g; add0 = add1 = !gz; return 0;
-    case MUL: go0 = g * a1; go1 = g * a0; add0 = add1 = !gz; return 0;
-    case DIV:
-        if (gz) return 0;
-        add0 = add1 = true;
-        if (a1 == zero) { go0 = zero; go1 = zero; return ERR_DIV_ZERO; }
-        go0 = div_w(g, a1);
-        go1 = -div_w(g * a0, a1 * a1);
-        return 0;
-    case POW:
-        add0 = add1 = true;
-        if (a0 < zero) { go0 = nan; go1 = nan; return 0; }
-        if (gz) { go0 = zero; go1 = zero; return 0; }
-        go0 = g * (pow_w(a0, a1 - T(1.0)) * a1);
-        go1 = g * (log_w(a0) * pow_w(a0, a1));
-        return 0;
-    default: return 0;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        add0[p] = false;
+        add1[p] = false;
+        e[p] = 0;
+        go0[p] = zero;
+        go1[p] = zero;
     }
+#define SR_EACH(body)                                                                              \
+    {                                                                                              \
+        _Pragma("unroll") for (int p = 0; p < P; ++p) {                                            \
+            const bool gz = (g[p] == zero);                                                        \
+            (void)gz;                                                                              \
+            body                                                                                   \
+        }                                                                                          \
+    }                                                                                              \
+    break
+    switch (op) {
+    case SIN: SR_EACH(go0[p] = g[p] * Mth<T>::cos_(a0[p]); add0[p] = !gz;);
+    case COS: SR_EACH(go0[p] = g[p] * -Mth<T>::sin_(a0[p]); add0[p] = !gz;);
+    case EXP: SR_EACH(go0[p] = g[p] * Mth<T>::exp_(a0[p]); add0[p] = !gz;);
+    case LOG:
+    case LN: SR_EACH(go0[p] = div_w(g[p], a0[p]); add0[p] = !gz;);
+    case SQUARE: SR_EACH(go0[p] = g[p] * (T(2.0) * a0[p]); add0[p] = !gz;);
+    case SQRT:
+        SR_EACH(go0[p] = (a0[p] < zero) ? nan : (gz ? zero : div_w(g[p] * T(0.5), sqrt_w(a0[p]))); add0[p] = true;);
+    case TAN: SR_EACH(const T c = Mth<T>::cos_(a0[p]); go0[p] = div_w(g[p], c * c); add0[p] = !gz;);
+    case ARCSIN:
+        SR_EACH(go0[p] = (a0[p] < T(-1) || a0[p] > T(1)) ? nan : div_w(g[p], sqrt_w(T(1.0) - a0[p] * a0[p]));
+                add0[p] = !gz;);
+    case ARCCOS:
+        SR_EACH(go0[p] = (a0[p] < T(-1) || a0[p] > T(1)) ? nan : -div_w(g[p], sqrt_w(T(1.0) - a0[p] * a0[p]));
+                add0[p] = !gz;);
+    case ARCTAN: SR_EACH(go0[p] = div_w(g[p], T(1.0) + a0[p] * a0[p]); add0[p] = !gz;);
+    case SINH: SR_EACH(go0[p] = g[p] * Mth<T>::cosh_(a0[p]); add0[p] = !gz;);
+    case COSH: SR_EACH(go0[p] = g[p] * Mth<T>::sinh_(a0[p]); add0[p] = !gz;);
+    case TANH: SR_EACH(const T t = Mth<T>::tanh_(a0[p]); go0[p] = g[p] * (T(1.0) - t * t); add0[p] = !gz;);
+    case FLOOR:
+    case CEIL: break;
+    case LOG10: SR_EACH(go0[p] = div_w(g[p], a0[p] * log_w(T(10.0))); add0[p] = !gz;);
+    case NEG: SR_EACH(go0[p] = -g[p]; add0[p] = !gz;);
+    case INV: SR_EACH(go0[p] = (a0[p] == zero) ? nan : -div_w(g[p], a0[p] * a0[p]); add0[p] = !gz;);
+    case CUBE: SR_EACH(go0[p] = g[p] * (T(3.0) * (a0[p] * a0[p])); add0[p] = !gz;);
+    case FOURTH: SR_EACH(go0[p] = g[p] * (T(4.0) * (a0[p] * a0[p] * a0[p])); add0[p] = !gz;);
+    case FIFTH: SR_EACH(const T sq = a0[p] * a0[p]; go0[p] = g[p] * (T(5.0) * (sq * sq)); add0[p] = !gz;);
+    case ADD: SR_EACH(go0[p] = g[p]; go1[p] = g[p]; add0[p] = add1[p] = !gz;);
+    case SUB: SR_EACH(go0[p] = g[p]; go1[p] = -g[p]; add0[p] = add1[p] = !gz;);
+    case MUL: SR_EACH(go0[p] = g[p] * a1[p]; go1[p] = g[p] * a0[p]; add0[p] = add1[p] = !gz;);
+    case DIV:
+        SR_EACH(if (!gz) {
+            add0[p] = add1[p] = true;
+            if (a1[p] == zero) {
+                e[p] = ERR_DIV_ZERO;
+            } else {
+                go0[p] = div_w(g[p], a1[p]);
+                go1[p] = -div_w(g[p] * a0[p], a1[p] * a1[p]);
+            }
+        });
+    case POW:
+        SR_EACH(add0[p] = add1[p] = true; if (a0[p] < zero) {
+            go0[p] = nan;
+            go1[p] = nan;
+        } else if (!gz) {
+            go0[p] = g[p] * (pow_w(a0[p], a1[p] - T(1.0)) * a1[p]);
+            go1[p] = g[p] * (log_w(a0[p]) * pow_w(a0[p], a1[p]));
+        });
+    default: break;
+    }
+#undef SR_EACH
+}
+
+template <typename T>
+__device__ inline int32_t bwd_node(int op, T g, T a0, T a1, T &go0, bool &add0, T &go1, bool &add1) {
+    int32_t e;
+    bwd_nodes<T, 1>(op, &g, &a0, &a1, &go0, &add0, &go1, &add1, &e);
+    return e;
 }
 
 }  // namespace sr

@@ -44,6 +44,18 @@ __device__ inline double uniform_f64(double v) {
     return __longlong_as_double(((int64_t)(uint32_t)lo) | ((int64_t)hi << 32));
 }
 
+// Next work item of a wave from a global counter.  Every lane issues the
+// atomic and only lane 0 adds 1, so there is no lane-dependent branch: with
+// an `if (lane == 0)` around the atomic, the optimiser may correlate it with
+// another lane-0 branch at the end of the previous item and let lanes 1..63
+// re-enter the loop by a shortcut edge that skips the fetch, which turns the
+// work loop into an exec-masked loop that re-reads a stale index.  Lane 0's
+// ticket is returned to every lane as a wave-uniform value.
+__device__ inline int32_t wave_fetch_add(int32_t *counter) {
+    const int32_t got = atomicAdd(counter, (threadIdx.x & (64 - 1)) == 0 ? 1 : 0);
+    return __builtin_amdgcn_readlane(got, 0);
+}
+
 // Butterfly sum: every lane computes the same additions in a commuted order,
 // so all lanes hold the bit-identical total; it is returned as a uniform.
 __device__ inline double wave_sum(double v) {
@@ -58,6 +70,47 @@ __device__ inline int32_t wave_max_i32(int32_t v) {
         v = o > v ? o : v;
     }
     return uniform_i32(v);
+}
+
+// DPP reductions.  A row is 16 lanes; quad_perm [1,0,3,2] and [2,3,0,1]
+// sum each quad, row_ror:4 and row_ror:8 then leave every lane of a row
+// holding its row's total.  These are VALU moves with no LDS round trip.
+// All 64 lanes must be active (the library calls them from wave-uniform
+// control flow only).
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+    const int64_t b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((int64_t)(uint32_t)lo) | ((int64_t)hi << 32));
+}
+constexpr int DPP_QUAD_SWAP1 = 0xB1;  // quad_perm [1,0,3,2]
+constexpr int DPP_QUAD_SWAP2 = 0x4E;  // quad_perm [2,3,0,1]
+constexpr int DPP_ROW_ROR4 = 0x124;
+constexpr int DPP_ROW_ROR8 = 0x128;
+
+__device__ inline double row_sums(double v) {
+    v += dpp_f64<DPP_QUAD_SWAP1>(v);
+    v += dpp_f64<DPP_QUAD_SWAP2>(v);
+    v += dpp_f64<DPP_ROW_ROR4>(v);
+    v += dpp_f64<DPP_ROW_ROR8>(v);
+    return v;
+}
+
+// Sum over the 64 lanes as a wave-uniform value, in a fixed order
+// ((row0 + row1) + (row2 + row3)); `rows` = 1 when only lanes 0..15 can be
+// non-zero.
+__device__ inline double wave_sum_rows(double v, int rows) {
+    v = row_sums(v);
+    if (rows == 1) return uniform_f64(rl(v, 0));
+    return uniform_f64((rl(v, 0) + rl(v, 16)) + (rl(v, 32) + rl(v, 48)));
+}
+
+// Largest per-lane error code among {0, 201, 204} without a shuffle.
+__device__ inline int32_t wave_max_err(int32_t e) {
+    if (__ballot(e >= 204) != 0ull) return 204;
+    if (__ballot(e >= 201) != 0ull) return 201;
+    return 0;
 }
 
 // Program of one expression held in registers (M <= 128).

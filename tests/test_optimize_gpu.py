@@ -122,3 +122,62 @@ def test_optimize_raises_div_by_zero():
         st.optimize(X, Y, Ops, Ch, C0)
     Cf, Yf, info = st.optimize(X, Y, Ops, Ch, C0, raise_errors=False, return_info=True)
     assert int(info["err"][0]) == 204
+
+
+def _bucket_batch(seed):
+    """Programs in every length bucket of the fit kernels (<= 16, <= 32,
+    <= 64 and > 64 nodes) plus a 64-constant sum (65 live slots incl. the
+    shared non-constant one), padded to M = 128."""
+    rng = np.random.default_rng(seed)
+    M, n_x = 128, 2
+    rows = []
+    for lo, hi, depth, count in ((3, 16, 4, 10), (17, 32, 6, 10), (33, 64, 7, 10), (65, 127, 8, 8)):
+        for _ in range(count):
+            o, c = helpers.random_postfix(rng, M, n_x, max_len=hi, min_len=lo, unary=[10, 11, 14],
+                                          binary=[40, 41, 42], leaf_const_p=0.4, hard_const_p=0.0, depth=depth)
+            rows.append((o, c))
+    o = np.zeros(M, np.int64)
+    o[0] = 1
+    for k in range(1, 64):
+        o[2 * k - 1] = 1
+        o[2 * k] = 40
+    rows.append((o, helpers.children_of(o)))
+    ops = np.stack([r[0] for r in rows])
+    ch = np.stack([r[1] for r in rows])
+    return ops, ch, rng
+
+
+def test_optimize_length_buckets_match_oracle():
+    """Short runs (3 iterations) keep the two L-BFGS trajectories close, so
+    the fitted constants of every bucket are compared directly:
+    |C_gpu - C_oracle| <= 1e-3 * (1 + |C_oracle|) for >= 90% of the problems
+    that end finite in both, and exact agreement on non-constant slots."""
+    import cudasymbolicregression_amd as st
+
+    ops, ch, rng = _bucket_batch(11)
+    B, M = ops.shape
+    lengths = (ops != 0).sum(1)
+    print("lengths per bucket:", [(lengths <= 16).sum(), ((lengths > 16) & (lengths <= 32)).sum(),
+                                  ((lengths > 32) & (lengths <= 64)).sum(), (lengths > 64).sum()])
+    N = 70  # a full and a partial tile of 64 points
+    X = rng.uniform(-2, 2, (N, 2)).astype(np.float32)
+    C_true = helpers.constants_like(ops, rng, scale=1.0)
+    Y = lbfgs_oracle.oracle.fused_root(X, ops, ch, C_true, np.zeros((B, N), np.float32), mode=0)["y"]
+    C0 = helpers.constants_like(ops, rng, scale=1.0)
+    C0[ops != 1] = 0.25  # non-constant slots must come back unchanged
+    C_o, y_o, f_o, _ = lbfgs_oracle.optimize(X, Y, ops, ch, C0, max_iter=3)
+    Cg, Yg, info = st.optimize(_t(X), _t(Y), _t(ops), _t(ch), _t(C0), max_iter=3, return_info=True,
+                               raise_errors=False)
+    Cg = Cg.cpu().numpy()
+    assert (info["err"].cpu().numpy() == 0).all()
+    nonc = ops != 1
+    np.testing.assert_array_equal(Cg[nonc], C_o[nonc])  # C_init, or NaN where the reference's go NaN
+    fin = np.isfinite(C_o).all(1) & np.isfinite(Cg).all(1)
+    close = (np.abs(Cg - C_o) <= 1e-3 * (1 + np.abs(C_o))).all(1)
+    frac = close[fin].mean()
+    print(f"{fin.sum()} finite problems, {frac:.3f} within tolerance; per-problem max |dC|:",
+          np.round(np.abs(Cg - C_o).max(1), 6).tolist())
+    assert fin.sum() >= B * 3 // 4
+    assert frac >= 0.9
+    # the 64-constant sum: every constant gets the same gradient
+    assert close[-1]
