@@ -208,10 +208,11 @@ def unstable_tape_points(X, ops_bm, ch_bm, C_bm, delta, rtol, atol):
     return (sens & live).any(axis=0).T  # (B, N)
 
 
-def rounding_bound(X, ops_bm, ch_bm, C_bm, ulps=8):
-    """(B, N) first-order bound on how far two FP32 evaluations of the root
-    may drift apart when every node's libm result may differ by `ulps` units
-    in the last place: ulps * 2^-24 * sum_k |dy/dv_k| * |v_k|, with values
+def rounding_bound(X, ops_bm, ch_bm, C_bm, ulps=8, eps=2.0 ** -24):
+    """(B, N) first-order bound on how far two evaluations of the root
+    (unit roundoff eps: 2^-24 for FP32, 2^-53 for FP64) may drift apart when
+    every node's libm result may differ by `ulps` units in the last place:
+    ulps * eps * sum_k |dy/dv_k| * |v_k|, with values
     and adjoints from the FP64 oracle (non-finite terms give inf).  This is
     the classical forward error bound; it depends on the program and the
     point only, never on the implementation under test."""
@@ -232,4 +233,4 @@ def rounding_bound(X, ops_bm, ch_bm, C_bm, ulps=8):
         t[~np.isfinite(t) & (om != 0)[:, None, :]] = np.inf
         t[(om == 0)[:, None, :].repeat(N, axis=1)] = 0.0
         s = t.sum(axis=0).T  # (B, N)
-    return ulps * 2.0 ** -24 * s
+    return ulps * eps * s

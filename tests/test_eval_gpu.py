@@ -253,8 +253,10 @@ def test_fused_root_matches_oracle(M, mode):
     # FP64 path against the FP64 oracle
     out64 = fused.fused_root(_t(X.astype(np.float64)), _t(ops_bm), _t(ch_bm), _t(C_bm.astype(np.float64)),
                              _t(dy.astype(np.float64)), mode=mode, raise_errors=False)
-    ok, msg = helpers.check_stable(out64["y"].cpu().numpy(), r64["y"], np.zeros(r64["y"].shape, bool) | uy,
-                                   1e-10, 1e-12)
+    X64, C64 = X.astype(np.float64), C_bm.astype(np.float64)
+    uy64 = helpers.unstable_tape_points(X64, ops_bm, ch_bm, C64, 2.0 ** -50, 1e-6, 1e-12)
+    rb64 = helpers.rounding_bound(X, ops_bm, ch_bm, C_bm, eps=2.0 ** -53)
+    ok, msg = helpers.check_stable(out64["y"].cpu().numpy(), r64["y"], uy64, 1e-10, 1e-12, extra_tol=rb64)
     assert ok, "y64 " + msg
 
 
