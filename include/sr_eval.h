@@ -1,0 +1,138 @@
+/*
+ * sr_eval.h -- C-ABI of libsr_hip.so: batched postfix-expression evaluation
+ * (forward, backward, fused root-only) and device-resident batched L-BFGS
+ * constant fitting on gfx950 (MI355X).
+ *
+ * Conventions (every entry point):
+ *   - all pointers are caller-owned DEVICE memory unless stated otherwise;
+ *     nothing here allocates memory the caller sees;
+ *   - `stream` is a hipStream_t passed as void* (torch's current stream);
+ *     every call only enqueues work on it and returns without waiting;
+ *   - return 0 on success, a hipError_t value on a HIP failure, or
+ *       -1  invalid arguments (sizes out of the supported range),
+ *       -2  workspace smaller than the matching *_workspace_bytes(),
+ *       -3  a launch would need more LDS than one workgroup may have;
+ *   - expression-level error codes (error_codes.h of the reference:
+ *     101 child >= parent, 102 arity mismatch, 201 gradient into a NO_OP
+ *     node, 204 division by zero in the backward pass) are written to device
+ *     flags/arrays; the caller turns them into the reference's exception
+ *     texts (symbolic_evaluation.cpp:34, :160, :300).
+ *
+ * Layouts: "mb" = the reference backend's node-first layouts Ops (M,B),
+ * Ch (M,B,2), C (M,B) [, K], cache (M,N,B) [, K]; the fused and fit entry
+ * points take the public batch-first layouts Ops (B,M), Ch (B,M,2), C (B,M).
+ * Opcodes are the reference's Operator enum (include/operators.h:10-125).
+ * At most 128 nodes per expression (M <= 128).
+ */
+#ifndef SR_EVAL_H
+#define SR_EVAL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Validation of every (m, b) node: replaces validate_inputs /
+ * validate_inputs_kernel (src/symbolic_evaluation.cpp:7-37,
+ * src/evaluation_kernels_cuda.cu:13-49).  *d_first receives
+ * ((m*B + b) << 10 | code) of the first failing node in row-major (m, b)
+ * order, or all ones when every node is valid. */
+int sr_validate_mb(const int64_t *ops, const int64_t *ch, int64_t M, int64_t B, unsigned long long *d_first,
+                   void *stream);
+
+/* Full-cache forward: cache[k][n][b] = value of node k of expression b at
+ * point n.  Replaces SymbolicEvaluation::forward + the per-node forward
+ * kernels (src/symbolic_evaluation.cpp:39-92,
+ * src/evaluation_kernels_cuda.cu:51-216; CPU spec
+ * src/evaluation_kernels_cpu.cpp:55-219). */
+int sr_forward_mb_f32(const float *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch, const float *C,
+                      int64_t M, int64_t B, float *cache, void *stream);
+int sr_forward_mb_f64(const double *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                      const double *C, int64_t M, int64_t B, double *cache, void *stream);
+
+/* Full-cache backward: grad_cache (M,N,B) holds the upstream gradient on
+ * entry and the node adjoints on exit; writes grad_C (M,B) and grad_X
+ * (N,n_x, nullable).  Replaces SymbolicEvaluation::backward + the backward
+ * kernels (src/symbolic_evaluation.cpp:94-168,
+ * src/evaluation_kernels_cuda.cu:218-632; CPU spec
+ * src/evaluation_kernels_cpu.cpp:222-662).  *d_err receives the largest
+ * code (0, 201, 204). */
+int64_t sr_backward_mb_workspace_bytes(int64_t B, int64_t N, int64_t n_x, int is_f64);
+int sr_backward_mb_f32(float *grad_cache, const float *cache, const int64_t *ops, const int64_t *ch, int64_t M,
+                       int64_t B, int64_t N, int64_t n_x, float *grad_C, float *grad_X, void *workspace,
+                       int64_t ws_bytes, int32_t *d_err, void *stream);
+int sr_backward_mb_f64(double *grad_cache, const double *cache, const int64_t *ops, const int64_t *ch, int64_t M,
+                       int64_t B, int64_t N, int64_t n_x, double *grad_C, double *grad_X, void *workspace,
+                       int64_t ws_bytes, int32_t *d_err, void *stream);
+
+/* K constant sets per expression: C (M,B,K), cache (M,N,B,K).  Replaces
+ * evaluate_multiple_constant_backend and its kernels
+ * (src/symbolic_evaluation.cpp:171-305, src/evaluation_kernels_cuda.cu:761-1364). */
+int sr_multi_forward_mb_f32(const float *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                            const float *C, int64_t M, int64_t B, int64_t K, float *cache, void *stream);
+int sr_multi_forward_mb_f64(const double *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                            const double *C, int64_t M, int64_t B, int64_t K, double *cache, void *stream);
+int64_t sr_multi_backward_mb_workspace_bytes(int64_t B, int64_t N, int64_t n_x, int64_t K, int is_f64);
+int sr_multi_backward_mb_f32(float *grad_cache, const float *cache, const int64_t *ops, const int64_t *ch,
+                             int64_t M, int64_t B, int64_t N, int64_t n_x, int64_t K, float *grad_C, float *grad_X,
+                             void *workspace, int64_t ws_bytes, int32_t *d_err, void *stream);
+int sr_multi_backward_mb_f64(double *grad_cache, const double *cache, const int64_t *ops, const int64_t *ch,
+                             int64_t M, int64_t B, int64_t N, int64_t n_x, int64_t K, double *grad_C,
+                             double *grad_X, void *workspace, int64_t ws_bytes, int32_t *d_err, void *stream);
+
+/* Fused root-only forward + backward over batch-first layouts.  Computes
+ * what optimize's closure needs (symbolic_torch/optimization.py:63-101:
+ * root = #non-NO_OP - 1, loss = mean_n (y - Y)^2, loss.sum().backward())
+ * without materialising the (M,N,B) cache:
+ *   y_out (B,N); loss_out (B,) FP64 [mode 1]; grad_C (B,M) [!fwd_only];
+ *   grad_X (N,n_x) [nullable]; err_b (B,) per-expression code;
+ *   flags[0] = max validation code, flags[1] = max backward code.
+ * mode 0: dy (B,N) is the upstream gradient of y; mode 1: dy is the target
+ * Y and the upstream gradient is (2*(y - Y))*(1/N).  Loss and dL/dC are
+ * reduced in FP64. */
+int64_t sr_fused_workspace_bytes(int64_t N, int64_t n_x, int64_t B, int64_t M, int want_gx, int is_f64);
+int sr_fused_root_f32(const float *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch, const float *C,
+                      int64_t M, int64_t B, const float *dy, int mode, int fwd_only, float *y_out, double *loss_out,
+                      float *grad_C, float *grad_X, int32_t *err_b, int32_t *flags, void *workspace,
+                      int64_t workspace_bytes, void *stream);
+int sr_fused_root_f64(const double *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                      const double *C, int64_t M, int64_t B, const double *dy, int mode, int fwd_only,
+                      double *y_out, double *loss_out, double *grad_C, double *grad_X, int32_t *err_b,
+                      int32_t *flags, void *workspace, int64_t workspace_bytes, void *stream);
+
+/* Device-resident batched L-BFGS over the LEARNABLE_CONSTANT slots of B
+ * batch-first expressions.  Replaces optimize + LBFGS_batch.step
+ * (symbolic_torch/optimization.py:9-112, symbolic_torch/lbfgs_batch.py:61-326)
+ * with the same options: max_iter, history (1..16), tol_grad, line_search
+ * (0 strong Wolfe, 1 backtracking), ls_max_iter, eps.
+ *   X (N,n_x) shared, or rows X[x_off[b] ...] per problem (ragged, nullable);
+ *   Y (B,N) targets, or Y[y_off[b] ...] with n_pts[b] points (ragged);
+ *   stop_loss (B,) optional early stop on the FP64 loss (nullable);
+ *   outputs C_out (B,M) (non-constant slots keep C_init, or NaN where the
+ *   reference's go NaN), loss_out (B,) FP64 final loss, iters_out (B,),
+ *   nclos_out (B,) objective evaluations (nullable), err_out (B,), flags (2).
+ * n_x <= 2048. */
+int64_t sr_lbfgs_workspace_bytes(int64_t B, int64_t M);
+int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                 const float *C_init, int64_t M, int64_t B, const int64_t *x_off, const int64_t *y_off,
+                 const int32_t *n_pts, const double *stop_loss, float *C_out, double *loss_out, int32_t *iters_out,
+                 int32_t *nclos_out, int32_t *err_out, int32_t *flags, int max_iter, int history, double tol_grad,
+                 int line_search, int ls_max_iter, double eps, void *workspace, int64_t ws_bytes, void *stream);
+
+/* Launch variants of the fit kernels (no counterpart in the reference; used
+ * by the autotuner).  Problems are bucketed by instruction count; each
+ * bucket has variants (waves per problem x points per lane per dispatch)
+ * of which one is active per process.  Host-only calls; return -1 on an
+ * out-of-range bucket or variant. */
+int sr_fit_num_buckets(void);
+int sr_fit_num_variants(int bucket);
+int sr_fit_get_variant(int bucket);
+int sr_fit_set_variant(int bucket, int variant);
+int sr_fit_variant_info(int bucket, int variant, int *waves, int *ppt);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SR_EVAL_H */
