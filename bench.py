@@ -65,6 +65,7 @@ OPCOST = {
 }
 LOSS_SLOTS = 5  # d = y - Y, d*d, accumulate, 2*d, *1/N
 FP32_PEAK_TFLOPS = 157.3
+VALU_LANE_PEAK = 256 * 4 * 16 * 2.4e9  # unpacked FP32 lane-instructions/s (3.93e13)
 HBM_PEAK_GBS = 8000.0
 
 
@@ -131,6 +132,51 @@ def rel_mse_fp64(w, C):
     return torch.where(torch.isfinite(r), r, torch.full_like(r, float("inf")))
 
 
+def bench_evaluate(dev, steps: int, seed: int = 1, B: int = 65536, N: int = 1024):
+    """Plain evaluate forward+backward at scale: B G2 expressions (padded 64,
+    n_x = 3), X ~ U(-3,3) (N,3), C ~ N(0,1) on the constant slots, upstream
+    dL/dy ~ N(0,1) on the root output; one fused root-only forward+backward
+    with grad_X per step, timed with HIP events."""
+    import torch
+
+    import cudasymbolicregression_amd as st
+    from cudasymbolicregression_amd import fused
+
+    g = st.ProbabilisticContextFreeGrammar(G2, "E", 64, 3, torch.device("cpu"))
+    torch.manual_seed(seed)
+    ops, ch = g.sample(B)
+    gen = torch.Generator(device=dev).manual_seed(seed)
+    X = torch.rand(N, 3, generator=gen, device=dev) * 6 - 3
+    ops, ch = ops.to(dev), ch.to(dev)
+    C = torch.zeros(ops.shape, device=dev)
+    m = ops == 1
+    C[m] = torch.randn(int(m.sum()), generator=gen, device=dev)
+    dy = torch.randn(B, N, generator=gen, device=dev)
+    out = {}
+    fused.fused_root(X, ops, ch, C, dy, mode=0, want_grad_X=True, raise_errors=False, out=out)
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        r = fused.fused_root(X, ops, ch, C, dy, mode=0, want_grad_X=True, raise_errors=False, out=out)
+    end.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / steps
+    M = ops.shape[1]
+    nbytes = (X.numel() * 4 + ops.numel() * 8 + ch.numel() * 8 + C.numel() * 4 + dy.numel() * 4 + B * N * 4
+              + B * M * 4 + X.numel() * 4)
+    slots = float(sum(expression_cost(row) for row in ops.cpu().tolist())) * N
+    t = ms / 1e3
+    return dict(workload=f"G2 B={B}, padded 64, N={N}, n_x=3, fused fwd+bwd with grad_X", ms=round(ms, 4),
+                hbm=dict(achieved_GBps=round(nbytes / t / 1e9, 1), peak_GBps=HBM_PEAK_GBS,
+                         frac=round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4), bytes=nbytes),
+                valu=dict(achieved_TFLOPs=round(slots / t / 1e12, 4), peak_TFLOPs=FP32_PEAK_TFLOPS,
+                          frac=round(slots / t / 1e12 / FP32_PEAK_TFLOPS, 5),
+                          frac_unpacked_lane_peak=round(slots / t / VALU_LANE_PEAK, 4), slots=slots),
+                mean_len=round(float((ops != 0).sum(1).double().mean()), 3),
+                errors=int((r["err"] != 0).sum()))
+
+
 def fit(w, C_start, max_iter=100):
     from cudasymbolicregression_amd.optimization import _fit
 
@@ -173,6 +219,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-evaluate", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): fitting problems are independent, "
@@ -214,6 +261,11 @@ def main() -> int:
                     frac=round(achieved / FP32_PEAK_TFLOPS, 5), traffic=None,
                     kernel="k_lbfgs (all length buckets of one fit)", flop_per_launch=flops)
 
+    ev = None
+    if not args.no_evaluate:
+        ev = bench_evaluate(dev, steps=5)
+        log(f"evaluate (C2): {ev['ms']:.3f} ms, {ev['hbm']['achieved_GBps']} GB/s, VALU frac {ev['valu']['frac']}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, args.cpu_sample)
@@ -243,6 +295,7 @@ def main() -> int:
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "evaluate": ev,
     }
     print(json.dumps(out))
     return 0

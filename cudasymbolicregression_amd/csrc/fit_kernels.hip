@@ -40,8 +40,9 @@
 //     runs the (deterministic) L-BFGS logic on identical values.
 //   * Problems are bucketed by program length so every kernel is sized for
 //     its bucket; the buckets run concurrently on forked streams.
-#include "sr_wave.h"
+#include "sr_code.h"
 #include "sr_device_info.h"
+#include "sr_streams.h"
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -85,207 +86,13 @@ struct FitArgs {
     float *gtape;            // global tape slots (programs longer than 64 nodes)
 };
 
-// ------------------------------------------------------------------ slots
-template <int SL>
-struct SVec {
-    float v[SL];
-};
-
-template <int SL>
-__device__ inline float slot_get(const SVec<SL> &a, int j) {
-    return (SL == 1 || j < WAVE) ? rl(a.v[0], j & (WAVE - 1)) : rl(a.v[SL - 1], j - WAVE);
-}
-
 // FP64 inner product over the K live slots; lanes past K never contribute.
 template <int SL>
-__device__ inline double sdot(const SVec<SL> &a, const SVec<SL> &b, int K) {
+__device__ inline double sdot(const SVec<float, SL> &a, const SVec<float, SL> &b, int K) {
     const int lane = lane_id();
     double s = (lane < K) ? (double)a.v[0] * (double)b.v[0] : 0.0;
     if (SL > 1 && lane + WAVE < K) s += (double)a.v[SL - 1] * (double)b.v[SL - 1];
     return wave_sum_rows(s, K <= 16 ? 1 : 4);
-}
-
-// ---------------------------------------------------------------- program
-// One problem's program, node-lane words; LEARNABLE_CONSTANT nodes carry
-// their compact ordinal in the first-child field.
-struct FitProg {
-    uint32_t w0, w1;
-    float cinit0, cinit1;   // C_init of node lane k (for the non-constant output slots)
-    uint64_t cm0, cm1;      // LEARNABLE_CONSTANT nodes
-    int L, root, nc, K;     // K = nc + (M > nc): live slots incl. the shared one
-    int32_t verr;
-    bool shared_nan_init;   // every non-constant slot starts as NaN
-    __device__ inline uint32_t word(int k) const { return k < WAVE ? rl_u32(w0, k) : rl_u32(w1, k - WAVE); }
-};
-
-__device__ inline FitProg load_fit_program(const FitArgs &a, int64_t b) {
-    Program<float> p = load_program<float>(a.ops, a.ch, a.C_init, b, a.M, 1, a.M, 1, a.M, a.n_x);
-    FitProg f;
-    const int lane = lane_id();
-    f.cm0 = p.cmask0;
-    f.cm1 = p.cmask1;
-    f.L = p.L;
-    f.root = p.root;
-    f.verr = p.verr;
-    f.cinit0 = p.c0;
-    f.cinit1 = p.c1;
-    const int n0 = __popcll(p.cmask0);
-    f.nc = n0 + __popcll(p.cmask1);
-    f.K = f.nc + (a.M > f.nc ? 1 : 0);
-    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
-    const int o0 = __popcll(p.cmask0 & below);
-    const int o1 = n0 + __popcll(p.cmask1 & below);
-    const uint32_t cw = (uint32_t)LEARNABLE_CONSTANT | (NULL_SLOT << 24);
-    f.w0 = ((p.cmask0 >> lane) & 1ull) ? (cw | ((uint32_t)o0 << 16)) : p.w0;
-    f.w1 = ((p.cmask1 >> lane) & 1ull) ? (cw | ((uint32_t)o1 << 16)) : p.w1;
-    const bool nonconst_finite0 = lane < a.M && !((p.cmask0 >> lane) & 1ull) && !(p.c0 != p.c0);
-    const bool nonconst_finite1 = lane + WAVE < a.M && !((p.cmask1 >> lane) & 1ull) && !(p.c1 != p.c1);
-    f.shared_nan_init = (__ballot(nonconst_finite0) | __ballot(nonconst_finite1)) == 0ull;
-    return f;
-}
-
-// Compact start point: lane j <- C_init of the j-th constant; shared slot 0.
-template <int SL>
-__device__ inline SVec<SL> compact_start(const FitProg &p) {
-    const int lane = lane_id();
-    SVec<SL> x;
-#pragma unroll
-    for (int h = 0; h < SL; ++h) x.v[h] = 0.0f;
-    int j = 0;
-    for (uint64_t m = p.cm0; m; m &= m - 1, ++j) {
-        const int bit = __ffsll((long long)m) - 1;
-        const float v = rl(p.cinit0, bit);
-        if (j < WAVE) { if (lane == j) x.v[0] = v; }
-        else if (lane == j - WAVE) x.v[SL - 1] = v;
-    }
-    for (uint64_t m = p.cm1; m; m &= m - 1, ++j) {
-        const int bit = __ffsll((long long)m) - 1;
-        const float v = rl(p.cinit1, bit);
-        if (j < WAVE) { if (lane == j) x.v[0] = v; }
-        else if (lane == j - WAVE) x.v[SL - 1] = v;
-    }
-    return x;
-}
-
-// ------------------------------------------------------------------ tapes
-// ------------------------------------------------------------ instructions
-// A problem's program is compiled once into an instruction list: one
-// instruction per operator node up to the root (leaves are folded into the
-// operand descriptors of their parents), plus a copy instruction when the
-// root itself is a leaf.  Instruction i writes tape slot i, so the root is
-// always the last slot.  Word: bits 0-5 opcode (the reference's operator id,
-// or OP_COPY), bits 6-18 and 19-31 the two operand descriptors
-// (2-bit kind, 11-bit index).
-constexpr int K_TAPE = 0, K_VAR = 1, K_CONST = 2, K_IMM = 3;
-constexpr int IMM_NOOP = 8;  // a NO_OP node used as an operand: NaN, and 201 if it receives a gradient
-constexpr int OP_COPY = 63;
-constexpr int MAX_NX_CODE = 2047;
-
-__host__ __device__ inline int instr_op(uint32_t w) { return (int)(w & 63u); }
-__host__ __device__ inline uint32_t instr_d0(uint32_t w) { return (w >> 6) & 0x1FFFu; }
-__host__ __device__ inline uint32_t instr_d1(uint32_t w) { return w >> 19; }
-__host__ __device__ inline int desc_kind(uint32_t d) { return (int)(d & 3u); }
-__host__ __device__ inline int desc_idx(uint32_t d) { return (int)(d >> 2); }
-
-// Descriptor of node word w as an operand of its parent.
-__device__ inline uint32_t operand_desc(uint32_t w, uint32_t instr_index, bool is_instr) {
-    const int op = node_op(w);
-    if (is_instr) return (uint32_t)K_TAPE | (instr_index << 2);
-    if (op == LEARNABLE_CONSTANT) return (uint32_t)K_CONST | ((uint32_t)node_c0(w) << 2);
-    if (op >= VAR_START_ID && op != UNKNOWN_LEAF) return (uint32_t)K_VAR | ((uint32_t)(op - VAR_START_ID) << 2);
-    if (op >= CONST_1 && op <= E) return (uint32_t)K_IMM | ((uint32_t)(op - 1) << 2);  // 1..5, 6 = pi, 7 = e
-    if (op == NO_OP) return (uint32_t)K_IMM | ((uint32_t)IMM_NOOP << 2);
-    return (uint32_t)K_IMM;  // unknown leaf: 0
-}
-
-// Instruction count of a batch-first program row under the same rule as
-// compile_code (used to bucket problems before they are compiled).
-__device__ inline int count_instructions(const int64_t *row, int M, int n_x, int &nc) {
-    int nn = 0;
-    nc = 0;
-    for (int k = 0; k < M; ++k) {
-        if (row[k] != NO_OP) ++nn;
-        if (row[k] == LEARNABLE_CONSTANT) ++nc;
-    }
-    const int root = nn > 0 ? nn - 1 : 0;
-    int ni = 0;
-    for (int k = 0; k <= root && k < M; ++k)
-        if (arity_of(node_op(pack_node(row[k], -1, -1, n_x))) > 0) ++ni;
-    const bool root_is_op = arity_of(node_op(pack_node(row[root], -1, -1, n_x))) > 0;
-    return ni + (root_is_op ? 0 : 1);
-}
-
-struct Code {
-    uint32_t iw0, iw1;  // lane i: instruction i (and 64 + i)
-    int NI;
-    __device__ inline uint32_t word(int i) const { return i < WAVE ? rl_u32(iw0, i) : rl_u32(iw1, i - WAVE); }
-};
-
-__device__ inline void place_word(Code &c, int i, uint32_t v) {
-    const int lane = lane_id();
-    if (i < WAVE) { if (lane == i) c.iw0 = v; }
-    else if (lane == i - WAVE) c.iw1 = v;
-}
-
-__device__ inline Code compile_code(const FitProg &p, int M) {
-    const int lane = lane_id();
-    const int root = p.root;
-    const uint32_t w0 = p.w0, w1 = p.w1;
-    const bool op0 = lane < M && lane <= root && arity_of(node_op(w0)) > 0;
-    const bool op1 = lane + WAVE < M && lane + WAVE <= root && arity_of(node_op(w1)) > 0;
-    const uint64_t om0 = __ballot(op0), om1 = __ballot(op1);
-    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
-    const uint32_t ix0 = (uint32_t)__popcll(om0 & below);
-    const uint32_t ix1 = (uint32_t)(__popcll(om0) + __popcll(om1 & below));
-    const uint32_t d0 = operand_desc(w0, ix0, op0), d1 = operand_desc(w1, ix1, op1);
-    auto child_desc = [&](int c) -> uint32_t {  // every lane takes part in both shuffles
-        const int cc = c & 127;
-        const uint32_t lo = (uint32_t)__shfl((int)d0, cc & (WAVE - 1), WAVE);
-        const uint32_t hi = (uint32_t)__shfl((int)d1, cc & (WAVE - 1), WAVE);
-        return cc < WAVE ? lo : hi;
-    };
-    const uint32_t c00 = child_desc(node_c0(w0)), c01 = child_desc(node_c1(w0));
-    const uint32_t c10 = child_desc(node_c0(w1)), c11 = child_desc(node_c1(w1));
-    const uint32_t i0 = (uint32_t)node_op(w0) | (c00 << 6) | ((arity_of(node_op(w0)) == 2 ? c01 : 0u) << 19);
-    const uint32_t i1 = (uint32_t)node_op(w1) | (c10 << 6) | ((arity_of(node_op(w1)) == 2 ? c11 : 0u) << 19);
-    Code c;
-    c.iw0 = 0u;
-    c.iw1 = 0u;
-    int i = 0;
-    for (uint64_t m = om0; m; m &= m - 1, ++i) place_word(c, i, rl_u32(i0, __ffsll((long long)m) - 1));
-    for (uint64_t m = om1; m; m &= m - 1, ++i) place_word(c, i, rl_u32(i1, __ffsll((long long)m) - 1));
-    const uint32_t rw = root < WAVE ? rl_u32(w0, root) : rl_u32(w1, root - WAVE);
-    if (arity_of(node_op(rw)) == 0) {
-        const uint32_t rd = root < WAVE ? rl_u32(d0, root) : rl_u32(d1, root - WAVE);
-        place_word(c, i, (uint32_t)OP_COPY | (rd << 6));
-        ++i;
-    }
-    c.NI = i;
-    return c;
-}
-
-// Tape of one wave: the value of instruction i at point p of a tile group is
-// v[(i*PPT + p)*64 + lane], its adjoint a[...] likewise; a holds one extra
-// row group (index LT) that absorbs the adjoint writes of non-tape operands.
-// The arrays are in LDS (bank-conflict free: 64 lanes touch 64 consecutive
-// words) or, for the longest programs, in a global workspace slot.  Each lane
-// only touches its own column, so a write needs no synchronisation before
-// the same lane reads it back.
-template <int PPT>
-struct Tape {
-    float *v;
-    float *a;
-    __device__ inline float val(int i, int p) const { return v[(i * PPT + p) * WAVE + lane_id()]; }
-    __device__ inline void set(int i, int p, float x) const { v[(i * PPT + p) * WAVE + lane_id()] = x; }
-    __device__ inline float adj(int i, int p) const { return a[(i * PPT + p) * WAVE + lane_id()]; }
-    __device__ inline void set_adj(int i, int p, float x) const { a[(i * PPT + p) * WAVE + lane_id()] = x; }
-};
-
-__device__ inline float imm_value(int idx) {
-    return idx == 0 ? 0.0f
-                    : (idx <= 5 ? (float)idx
-                                : (idx == 6 ? (float)3.14159265358979323846
-                                            : (idx == 7 ? (float)2.71828182845904523536 : __builtin_nanf(""))));
 }
 
 // ---------------------------------------------------------------- closure
@@ -295,9 +102,9 @@ __device__ inline float imm_value(int idx) {
 // compact lanes (gc), and its error code (uniform).  Per instruction there is
 // one opcode dispatch for all PPT points; operand fetch and adjoint scatter
 // are selects on the (uniform) operand descriptors.
-template <int SL, int NX, int PPT, int LT>
-__device__ inline void closure_code(const Code &c, int nc, const SVec<SL> &xt, const float *X, const float *Yb,
-                                    int64_t N, int n_x, int wslot, int W, const Tape<PPT> &t, double &loss,
+template <int SL, int NX, int PPT, int LT, bool GX>
+__device__ inline void closure_code(const Code &c, int nc, const SVec<float, SL> &xt, const float *X, const float *Yb,
+                                    int64_t N, int n_x, int wslot, int W, const Tape<float, PPT> &t, double &loss,
                                     double (&gc)[SL], int32_t &err) {
     const int lane = lane_id();
     const int NI = c.NI;
@@ -327,24 +134,25 @@ __device__ inline void closure_code(const Code &c, int nc, const SVec<SL> &xt, c
             const int kind = desc_kind(d), idx = desc_idx(d);
             const int tidx = kind == K_TAPE ? idx : 0;
             const int vidx = kind == K_VAR ? idx : 0;
-            const float cv = slot_get<SL>(xt, kind == K_CONST ? idx : 0);
-            const float iv = imm_value(kind == K_IMM ? idx : 0);
+            const float cv = slot_get<float, SL>(xt, kind == K_CONST ? idx : 0);
+            const float iv = imm_value<float>(kind == K_IMM ? idx : 0);
 #pragma unroll
             for (int p = 0; p < PPT; ++p) {
                 const float tv = t.val(tidx, p);
                 float vv = 0.0f;
-                if (n_x <= NX) {
+                if constexpr (!GX) {
 #pragma unroll
                     for (int v = 0; v < NX; ++v) vv = (v == vidx) ? xv[p][v] : vv;
-                } else if (kind == K_VAR) {
-                    vv = act[p] ? X[n[p] * (int64_t)n_x + vidx] : 0.0f;
+                } else {
+                    // more variables than registers: read them from the X row
+                    if (kind == K_VAR) vv = act[p] ? X[n[p] * (int64_t)n_x + vidx] : 0.0f;
                 }
                 a[p] = kind == K_TAPE ? tv : (kind == K_VAR ? vv : (kind == K_CONST ? cv : iv));
             }
         };
         // ---- forward; adjoints are zeroed on the way
         for (int i = 0; i < NI; ++i) {
-            const uint32_t w = c.word(i);
+            const uint32_t w = c.word_lt<LT>(i);
             const int op = instr_op(w);
             float a0[PPT], a1[PPT], r[PPT];
             fetch(instr_d0(w), a0);
@@ -375,7 +183,7 @@ __device__ inline void closure_code(const Code &c, int nc, const SVec<SL> &xt, c
         }
         // ---- reverse sweep
         for (int i = NI - 1; i >= 0; --i) {
-            const uint32_t w = c.word(i);
+            const uint32_t w = c.word_lt<LT>(i);
             const int op = instr_op(w);
             const bool binary = op >= ADD && op <= POW;
             float g[PPT], a0[PPT], a1[PPT], go0[PPT], go1[PPT];
@@ -462,9 +270,8 @@ __host__ __device__ inline size_t part_bytes(int W, int SL) {
 }
 // Tape of one wave: LT value rows and LT + 1 adjoint rows per point of a
 // tile group, 64 floats each.
-__host__ __device__ inline size_t tape_floats(int LT, int PPT) { return (size_t)(2 * LT + 1) * PPT * WAVE; }
 __host__ __device__ inline size_t tape_bytes(int LT, int PPT, bool mem) {
-    return mem ? 0 : tape_floats(LT, PPT) * sizeof(float);
+    return mem ? 0 : tape_elems(LT, PPT) * sizeof(float);
 }
 __host__ __device__ inline size_t fit_lds(int LT, int PPT, bool mem, int W, int SL, int hist) {
     const int groups = W == 1 ? block_waves(W, SL) : 1;
@@ -491,10 +298,10 @@ void k_fit(FitArgs a) {
     const int group = W == 1 ? wave : 0;   // problem slot of this wave inside the block
     const int wslot = W == 1 ? 0 : wave;   // this wave's share of the points
     const int hist = a.history;
-    Tape<PPT> tape;
+    Tape<float, PPT> tape;
     {
-        float *slot = MEM ? a.gtape + ((size_t)blockIdx.x * block_waves(W, SL) + wave) * tape_floats(LT, PPT)
-                          : reinterpret_cast<float *>(smem) + (size_t)wave * tape_floats(LT, PPT);
+        float *slot = MEM ? a.gtape + ((size_t)blockIdx.x * block_waves(W, SL) + wave) * tape_elems(LT, PPT)
+                          : reinterpret_cast<float *>(smem) + (size_t)wave * tape_elems(LT, PPT);
         tape.v = slot;
         tape.a = slot + (size_t)LT * PPT * WAVE;
     }
@@ -523,7 +330,7 @@ void k_fit(FitArgs a) {
         }
         if (i >= count) break;
         const int64_t b = (int64_t)a.list[i];
-        const FitProg p = load_fit_program(a, b);
+        const CProg<float> p = load_cprog<float>(a.ops, a.ch, a.C_init, b, a.M, a.n_x);
         const bool writer = (wslot == 0);
         if (p.verr) {
             if (writer) {
@@ -556,7 +363,7 @@ void k_fit(FitArgs a) {
         const double stop = a.stop_loss ? a.stop_loss[b] : -1.0;
         const int K = p.K;
 
-        SVec<SL> x = compact_start<SL>(p), g, d, g_prev, g_lo, g_new, ga;
+        SVec<float, SL> x = compact_start<float, SL>(p), g, d, g_prev, g_lo, g_new, ga;
 #pragma unroll
         for (int h = 0; h < SL; ++h) { g.v[h] = d.v[h] = g_prev.v[h] = g_lo.v[h] = g_new.v[h] = ga.v[h] = 0.0f; }
         int st = ST_INIT;
@@ -568,12 +375,15 @@ void k_fit(FitArgs a) {
 
         for (;;) {
             // ---------------- one objective evaluation at x (+ a_eval d)
-            SVec<SL> xt;
+            SVec<float, SL> xt;
 #pragma unroll
             for (int h = 0; h < SL; ++h) xt.v[h] = (st == ST_INIT) ? x.v[h] : x.v[h] + a_eval * d.v[h];
             double phi, gcd[SL];
             int32_t e2;
-            closure_code<SL, NX, PPT, LT>(code, p.nc, xt, X, Yb, N, a.n_x, wslot, W, tape, phi, gcd, e2);
+            if (a.n_x <= NX)
+                closure_code<SL, NX, PPT, LT, false>(code, p.nc, xt, X, Yb, N, a.n_x, wslot, W, tape, phi, gcd, e2);
+            else
+                closure_code<SL, NX, PPT, LT, true>(code, p.nc, xt, X, Yb, N, a.n_x, wslot, W, tape, phi, gcd, e2);
             if (W > 1) {
                 // partial sums of the W waves meet in LDS; every wave adds them
                 // in the same order, so all waves hold identical results
@@ -693,7 +503,7 @@ void k_fit(FitArgs a) {
             }
             if (accept) {
                 // ---------------- step and history push
-                SVec<SL> s, yv;
+                SVec<float, SL> s, yv;
 #pragma unroll
                 for (int h = 0; h < SL; ++h) {
                     s.v[h] = alpha * d.v[h];
@@ -737,11 +547,11 @@ void k_fit(FitArgs a) {
                 if (__ballot(notnan) == 0ull) break;
             }
             // two-loop recursion, newest pair first; alpha_j kept in lane j
-            SVec<SL> q = g;
+            SVec<float, SL> q = g;
             double alph_lane = 0.0;
             for (int jj = nh - 1; jj >= 0; --jj) {
                 const int sl = head + jj < hist ? head + jj : head + jj - hist;
-                SVec<SL> sj, yj;
+                SVec<float, SL> sj, yj;
 #pragma unroll
                 for (int h = 0; h < SL; ++h) {
                     sj.v[h] = Sh[((size_t)sl * SL + h) * WAVE + lane];
@@ -753,10 +563,10 @@ void k_fit(FitArgs a) {
 #pragma unroll
                 for (int h = 0; h < SL; ++h) q.v[h] = q.v[h] - af * yj.v[h];
             }
-            SVec<SL> z = q;
+            SVec<float, SL> z = q;
             if (nh > 0) {
                 const int sl = head + nh - 1 < hist ? head + nh - 1 : head + nh - 1 - hist;
-                SVec<SL> sn, yn;
+                SVec<float, SL> sn, yn;
 #pragma unroll
                 for (int h = 0; h < SL; ++h) {
                     sn.v[h] = Sh[((size_t)sl * SL + h) * WAVE + lane];
@@ -767,7 +577,7 @@ void k_fit(FitArgs a) {
                 for (int h = 0; h < SL; ++h) z.v[h] = gamma * q.v[h];
                 for (int jj = 0; jj < nh; ++jj) {
                     const int sj_ = head + jj < hist ? head + jj : head + jj - hist;
-                    SVec<SL> sj, yj;
+                    SVec<float, SL> sj, yj;
 #pragma unroll
                     for (int h = 0; h < SL; ++h) {
                         sj.v[h] = Sh[((size_t)sj_ * SL + h) * WAVE + lane];
@@ -808,7 +618,7 @@ void k_fit(FitArgs a) {
 
         // ---------------- write back all M slots
         if (writer) {
-            const float shared = slot_get<SL>(x, p.nc < 2 * WAVE ? p.nc : 0);
+            const float shared = slot_get<float, SL>(x, p.nc < 2 * WAVE ? p.nc : 0);
             const bool shared_nan = (K > p.nc) && (shared != shared);
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
@@ -931,33 +741,8 @@ inline size_t fit_gt_off(int64_t B) { return (((size_t)NBUCKETS * B * 4 + 64) + 
 inline int64_t fit_ws(int64_t B, int64_t M) {
     size_t off = fit_gt_off(B);
     if (M > 64)  // more than 64 instructions or live slots needs more than 64 nodes
-        off += (size_t)b4_grid() * block_waves(1, 2) * tape_floats(128, 1) * sizeof(float);
+        off += (size_t)b4_grid() * block_waves(1, 2) * tape_elems(128, 1) * sizeof(float);
     return (int64_t)off + 256;
-}
-
-struct Fork {
-    hipStream_t side[3] = {nullptr, nullptr, nullptr};
-    hipEvent_t ev_fork = nullptr;
-    hipEvent_t ev_join[3] = {nullptr, nullptr, nullptr};
-    bool ok = false;
-};
-
-// Side streams (default priority) and events of the current device, created
-// once per device and kept for the life of the process.
-Fork &fork_for_device() {
-    static Fork forks[64];
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    Fork &f = forks[dev & 63];
-    if (!f.ok) {
-        bool good = hipEventCreateWithFlags(&f.ev_fork, hipEventDisableTiming) == hipSuccess;
-        for (int i = 0; i < 3 && good; ++i) {
-            good = hipStreamCreateWithFlags(&f.side[i], hipStreamNonBlocking) == hipSuccess &&
-                   hipEventCreateWithFlags(&f.ev_join[i], hipEventDisableTiming) == hipSuccess;
-        }
-        f.ok = good;
-    }
-    return f;
 }
 
 }  // namespace

@@ -1,0 +1,537 @@
+// Fused root-only forward + backward over batch-first layouts.
+//
+// What optimize's closure needs (symbolic_torch/optimization.py:63-101) and
+// what the reference gets from evaluate + gather + backward, without the
+// (M,N,B) cache: per expression b and point n the root value
+// (root = #non-NO_OP - 1, optimization.py:55-58), and the reverse sweep from
+// the root with upstream gradient
+//   MODE 0: dy[b][n]                  (an upstream dL/dy),
+//   MODE 1: (2 * (y - Y)) * (1 / N)   (the MSE closure; dy is the target Y,
+//                                      loss_out[b] = mean_n (y - Y)^2 in FP64).
+// Outputs: y (B,N), grad_C (B,M) summed over the points in FP64, grad_X
+// (N,n_x) summed over the batch, per-expression error codes.
+//
+// Execution (gfx950): one wave per expression; the program is compiled once
+// per wave into an instruction list (sr_code.h) and evaluated over tile
+// groups of 64*PPT points with the tape and adjoints in LDS.  Expressions are
+// bucketed by instruction count (<= 8, <= 32, the rest) so each bucket's
+// kernel is sized for its tape; the buckets run concurrently on forked
+// streams.  Variable adjoints are summed per block in LDS (float atomics)
+// and written as one partial row per block; a final kernel adds the rows in
+// FP64 in row order.
+#include "sr_code.h"
+#include "sr_device_info.h"
+#include "sr_streams.h"
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+using namespace sr;
+
+namespace {
+
+constexpr int F_NCREG = 8;  // constants accumulated in per-lane FP64 registers
+constexpr int F_NX = 4;     // variables held in registers per point
+constexpr int F_NBUCKETS = 3;
+constexpr size_t GX_LDS_MAX2 = 24 * 1024;  // largest per-block grad_X partial kept in LDS
+
+struct FArgs {
+    const void *X;
+    int64_t N;
+    int n_x;
+    const int64_t *ops;
+    const int64_t *ch;
+    const void *C;
+    int M;
+    int64_t B;
+    const void *dy;
+    void *y_out;        // (B,N) or nullptr
+    double *loss_out;   // (B) MODE 1, or nullptr
+    void *grad_C;       // (B,M) unless fwd_only
+    void *gx_partial;   // [rows][N*n_x] per-block partials, or nullptr
+    int64_t gx_row0;    // first partial row of this launch
+    void *grad_X_direct;  // atomics target when partial rows are not used (nullable)
+    int32_t *err_b;     // (B) or nullptr
+    int32_t *flags;     // [0] validation (max), [1] backward (max)
+    const int32_t *list;
+    const int32_t *list_count;
+    int32_t *work_counter;
+    void *gtape;        // global tape slots (longest bucket)
+    int mode;
+    int fwd_only;
+    int want_gx;
+};
+
+// LT: most instructions of an expression in the bucket; SL: slot registers
+// per lane; PPT: points per lane per dispatch; WPB: waves per block; MEM:
+// tape in the global workspace; GX: more variables than F_NX registers.
+template <typename T, int LT, int SL, int PPT, int WPB, bool MEM, bool GX>
+__global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int wave = uniform_i32((int)(threadIdx.x / WAVE));
+    const int lane = lane_id();
+    const T *X = (const T *)a.X;
+    const T *dy = (const T *)a.dy;
+    T *y_out = (T *)a.y_out;
+    T *grad_C = (T *)a.grad_C;
+    const int64_t N = a.N;
+    const int n_x = a.n_x;
+    const int64_t gx_len = N * n_x;
+    const bool want_gx = a.want_gx != 0 && !a.fwd_only;
+    const bool gx_lds = want_gx && a.gx_partial != nullptr;
+    T *gx_blk = reinterpret_cast<T *>(smem);  // [N*n_x] when gx_lds
+    const size_t gx_bytes = gx_lds ? (((size_t)gx_len * sizeof(T) + 15) / 16 * 16) : 0;
+    Tape<T, PPT> t;
+    {
+        T *slot = MEM ? (T *)a.gtape + ((size_t)blockIdx.x * WPB + wave) * tape_elems(LT, PPT)
+                      : reinterpret_cast<T *>(smem + gx_bytes) + (size_t)wave * tape_elems(LT, PPT);
+        t.v = slot;
+        t.a = slot + (size_t)LT * PPT * WAVE;
+    }
+    if (gx_lds)
+        for (int64_t i = threadIdx.x; i < gx_len; i += WPB * WAVE) gx_blk[i] = T(0);
+    __syncthreads();
+    const int64_t count = (int64_t)*a.list_count;
+    const T inv_n = T(1.0) / T(N);
+
+    for (;;) {
+        const int i = wave_fetch_add(a.work_counter);
+        if (i >= count) break;
+        const int64_t b = (int64_t)a.list[i];
+        const CProg<T> p = load_cprog<T>(a.ops, a.ch, (const T *)a.C, b, a.M, n_x);
+        if (p.verr) {
+            if (lane == 0) {
+                atomicMax(&a.flags[0], p.verr);
+                if (a.err_b) a.err_b[b] = p.verr;
+            }
+            continue;
+        }
+        const Code c = compile_code(p, a.M);
+        if (c.NI > LT) {  // the bucketing rule and the compiler disagree: never expected
+            if (lane == 0) {
+                atomicMax(&a.flags[1], 900);
+                if (a.err_b) a.err_b[b] = 900;
+            }
+            continue;
+        }
+        const SVec<T, SL> xc = compact_start<T, SL>(p);
+        const int NI = c.NI;
+        double lacc = 0.0;
+        double cacc[F_NCREG];
+#pragma unroll
+        for (int q = 0; q < F_NCREG; ++q) cacc[q] = 0.0;
+        double gc[SL];
+#pragma unroll
+        for (int h = 0; h < SL; ++h) gc[h] = 0.0;
+        int32_t e = 0;
+        for (int64_t base = 0; base < N; base += (int64_t)WAVE * PPT) {
+            bool act[PPT];
+            int64_t n[PPT];
+            T xv[PPT][F_NX], gxr[PPT][F_NX];
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                n[q] = base + q * WAVE + lane;
+                act[q] = n[q] < N;
+                const T *row = X + (act[q] ? n[q] : 0) * (int64_t)n_x;
+#pragma unroll
+                for (int v = 0; v < F_NX; ++v) {
+                    xv[q][v] = (!GX && act[q] && v < n_x) ? row[v] : T(0);
+                    gxr[q][v] = T(0);
+                }
+            }
+            auto fetch = [&](uint32_t d, T (&av)[PPT]) {
+                const int kind = desc_kind(d), idx = desc_idx(d);
+                const int tidx = kind == K_TAPE ? idx : 0;
+                const int vidx = kind == K_VAR ? idx : 0;
+                const T cv = slot_get<T, SL>(xc, kind == K_CONST ? idx : 0);
+                const T iv = imm_value<T>(kind == K_IMM ? idx : 0);
+#pragma unroll
+                for (int q = 0; q < PPT; ++q) {
+                    const T tv = t.val(tidx, q);
+                    T vv = T(0);
+                    if constexpr (!GX) {
+#pragma unroll
+                        for (int v = 0; v < F_NX; ++v) vv = (v == vidx) ? xv[q][v] : vv;
+                    } else {
+                        if (kind == K_VAR) vv = act[q] ? X[n[q] * (int64_t)n_x + vidx] : T(0);
+                    }
+                    av[q] = kind == K_TAPE ? tv : (kind == K_VAR ? vv : (kind == K_CONST ? cv : iv));
+                }
+            };
+            // ---- forward; adjoints are zeroed on the way
+            for (int ii = 0; ii < NI; ++ii) {
+                const uint32_t w = c.word_lt<LT>(ii);
+                const int op = instr_op(w);
+                T a0[PPT], a1[PPT], r[PPT];
+                fetch(instr_d0(w), a0);
+                if (op >= ADD && op <= POW) {
+                    fetch(instr_d1(w), a1);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < PPT; ++q) a1[q] = T(0);
+                }
+                if (op == OP_COPY) {
+#pragma unroll
+                    for (int q = 0; q < PPT; ++q) r[q] = a0[q];
+                } else {
+                    fwd_nodes<T, PPT>(op, a0, a1, r);
+                }
+#pragma unroll
+                for (int q = 0; q < PPT; ++q) {
+                    t.set(ii, q, r[q]);
+                    t.set_adj(ii, q, T(0));
+                }
+            }
+            // ---- root value, loss and upstream gradient (dy is read only in
+            //      MODE 1 or when the backward runs; it may be null otherwise)
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const T yh = t.val(NI - 1, q);
+                if (act[q] && y_out) y_out[b * N + n[q]] = yh;
+                T g0 = T(0);
+                if (a.mode == 1) {
+                    const T d = yh - (act[q] ? dy[b * N + n[q]] : T(0));
+                    if (act[q]) lacc += (double)d * (double)d;
+                    g0 = (T(2.0) * d) * inv_n;
+                } else if (!a.fwd_only) {
+                    g0 = act[q] ? dy[b * N + n[q]] : T(0);
+                }
+                t.set_adj(NI - 1, q, act[q] ? g0 : T(0));
+            }
+            if (a.fwd_only) continue;
+            // ---- reverse sweep
+            for (int ii = NI - 1; ii >= 0; --ii) {
+                const uint32_t w = c.word_lt<LT>(ii);
+                const int op = instr_op(w);
+                const bool binary = op >= ADD && op <= POW;
+                T g[PPT], a0[PPT], a1[PPT], go0[PPT], go1[PPT];
+                bool add0[PPT], add1[PPT];
+                int32_t ee[PPT];
+#pragma unroll
+                for (int q = 0; q < PPT; ++q) g[q] = t.adj(ii, q);
+                fetch(instr_d0(w), a0);
+                if (binary) {
+                    fetch(instr_d1(w), a1);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < PPT; ++q) a1[q] = T(0);
+                }
+                if (op == OP_COPY) {
+#pragma unroll
+                    for (int q = 0; q < PPT; ++q) {
+                        go0[q] = g[q];
+                        add0[q] = true;
+                        go1[q] = T(0);
+                        add1[q] = false;
+                        ee[q] = 0;
+                    }
+                } else {
+                    bwd_nodes<T, PPT>(op, g, a0, a1, go0, add0, go1, add1, ee);
+                }
+#pragma unroll
+                for (int q = 0; q < PPT; ++q)
+                    if (act[q] && ee[q] > e) e = ee[q];
+                auto scatter = [&](uint32_t d, const T (&go)[PPT], const bool (&add)[PPT]) {
+                    const int kind = desc_kind(d), idx = desc_idx(d);
+                    const int slot = kind == K_TAPE ? idx : LT;  // row LT absorbs the other kinds
+#pragma unroll
+                    for (int q = 0; q < PPT; ++q) {
+                        const T o = t.adj(slot, q);
+                        t.set_adj(slot, q, (act[q] && add[q]) ? o + go[q] : o);
+                        if (kind == K_IMM && idx == IMM_NOOP && act[q] && add[q] && go[q] != T(0))
+                            e = e > ERR_GRAD_NO_OP ? e : ERR_GRAD_NO_OP;
+                    }
+                    if (kind == K_CONST) {
+                        double gv = 0.0;
+#pragma unroll
+                        for (int q = 0; q < PPT; ++q) gv += (act[q] && add[q]) ? (double)go[q] : 0.0;
+                        if (idx < F_NCREG) {
+#pragma unroll
+                            for (int r = 0; r < F_NCREG; ++r)
+                                if (r == idx) cacc[r] += gv;
+                        } else {
+                            const double s = wave_sum_rows(gv, 4);
+                            if (idx < WAVE) { if (lane == idx) gc[0] += s; }
+                            else if (lane == idx - WAVE) gc[SL - 1] += s;
+                        }
+                    } else if (kind == K_VAR && want_gx) {
+#pragma unroll
+                        for (int q = 0; q < PPT; ++q) {
+                            const T gvx = (act[q] && add[q]) ? go[q] : T(0);
+                            if constexpr (!GX) {
+#pragma unroll
+                                for (int v = 0; v < F_NX; ++v)
+                                    if (v == idx) gxr[q][v] += gvx;
+                            } else {
+                                if (gvx != T(0)) atomicAdd(&((T *)a.grad_X_direct)[n[q] * n_x + idx], gvx);
+                            }
+                        }
+                    }
+                };
+                scatter(instr_d0(w), go0, add0);
+                if (binary) scatter(instr_d1(w), go1, add1);
+            }
+            // ---- variable adjoints of this tile group
+            if (!GX && want_gx) {
+#pragma unroll
+                for (int q = 0; q < PPT; ++q) {
+                    if (!act[q]) continue;
+                    for (int v = 0; v < n_x; ++v) {
+                        T gxv = T(0);
+#pragma unroll
+                        for (int u = 0; u < F_NX; ++u) gxv = (u == v) ? gxr[q][u] : gxv;
+                        if (gxv == T(0)) continue;
+                        if (gx_lds) atomicAdd(&gx_blk[n[q] * n_x + v], gxv);
+                        else atomicAdd(&((T *)a.grad_X_direct)[n[q] * n_x + v], gxv);
+                    }
+                }
+            }
+        }
+        if (a.mode == 1 && a.loss_out) {
+            const double s = wave_sum_rows(lacc, 4);
+            if (lane == 0) a.loss_out[b] = s / (double)N;
+        }
+        if (a.fwd_only) {
+            if (lane == 0 && a.err_b) a.err_b[b] = 0;
+            continue;
+        }
+        const int ncr = p.nc < F_NCREG ? p.nc : F_NCREG;
+#pragma unroll
+        for (int r = 0; r < F_NCREG; ++r) {
+            if (r < ncr) {
+                const double s = wave_sum_rows(cacc[r], 4);
+                if (lane == r) gc[0] = s;
+            }
+        }
+        // dL/dC back to node slots: constant node k takes its compact lane
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k = lane + h * WAVE;
+            const uint64_t cm = h == 0 ? p.cm0 : p.cm1;
+            const uint64_t below = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
+            const int o = (h == 0 ? 0 : __popcll(p.cm0)) + __popcll(cm & below);
+            const double v0 = __shfl(gc[0], o & (WAVE - 1), WAVE);
+            const double v1 = __shfl(gc[SL - 1], o & (WAVE - 1), WAVE);
+            if (k < a.M) grad_C[b * a.M + k] = ((cm >> lane) & 1ull) ? (T)((SL == 1 || o < WAVE) ? v0 : v1) : T(0);
+        }
+        e = wave_max_err(e);
+        if (lane == 0) {
+            if (e) atomicMax(&a.flags[1], e);
+            if (a.err_b) a.err_b[b] = e;
+        }
+    }
+    __syncthreads();
+    if (gx_lds) {
+        T *dst = (T *)a.gx_partial + (a.gx_row0 + blockIdx.x) * gx_len;
+        for (int64_t i = threadIdx.x; i < gx_len; i += WPB * WAVE) dst[i] = gx_blk[i];
+    }
+}
+
+// Buckets by instruction count NI and constant count nc: NI <= 8, NI <= 32
+// (both with nc <= 64), the rest.
+__global__ void k_classify_fused(const int64_t *__restrict__ ops, int M, int64_t B, int n_x, int32_t *lists,
+                                 int32_t *counts) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    int nc = 0;
+    const int ni = count_instructions(ops + b * M, M, n_x, nc);
+    const int bucket = nc > 64 ? 2 : (ni <= 8 ? 0 : (ni <= 32 ? 1 : 2));
+    const int slot = atomicAdd(&counts[bucket], 1);
+    lists[(int64_t)bucket * B + slot] = (int32_t)b;
+}
+
+__global__ void k_fill_i32_e(int32_t *p, int32_t v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_u64_e(uint64_t *p, uint64_t v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Sums P partial rows of length len in FP64, in row order.
+template <typename T>
+__global__ void k_sum_rows(const T *__restrict__ part, int64_t P, int64_t len, T *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    double s = 0.0;
+    for (int64_t r = 0; r < P; ++r) s += (double)part[r * len + i];
+    out[i] = (T)s;
+}
+
+// Bucket kernels <LT, SL, PPT, WPB, MEM>.
+template <typename T> struct FCfg;
+template <> struct FCfg<float> {
+    static constexpr int B0_PPT = 2;
+};
+template <> struct FCfg<double> {
+    static constexpr int B0_PPT = 1;
+};
+
+template <typename T, int LT, int SL, int PPT, int WPB, bool MEM>
+size_t fused_lds(bool gx_lds, int64_t gx_len) {
+    return (gx_lds ? (((size_t)gx_len * sizeof(T) + 15) / 16 * 16) : 0) +
+           (MEM ? 0 : (size_t)WPB * tape_elems(LT, PPT) * sizeof(T));
+}
+
+template <typename T, int LT, int SL, int PPT, int WPB, bool MEM>
+int fused_grid(bool gx_lds, int64_t gx_len, int n_x) {
+    int per_cu = 0;
+    const size_t lds = fused_lds<T, LT, SL, PPT, WPB, MEM>(gx_lds, gx_len);
+    const void *fn = n_x <= F_NX ? (const void *)k_fused_code<T, LT, SL, PPT, WPB, MEM, false>
+                                 : (const void *)k_fused_code<T, LT, SL, PPT, WPB, MEM, true>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, WPB * 64, lds) != hipSuccess || per_cu <= 0)
+        per_cu = 1;
+    return num_cus() * per_cu;
+}
+
+template <typename T, int LT, int SL, int PPT, int WPB, bool MEM>
+int fused_launch(const FArgs &a, int grid, bool gx_lds, hipStream_t s) {
+    const size_t lds = fused_lds<T, LT, SL, PPT, WPB, MEM>(gx_lds, a.N * a.n_x);
+    if (lds > lds_limit_device()) return -3;
+    if (a.n_x <= F_NX)
+        hipLaunchKernelGGL((k_fused_code<T, LT, SL, PPT, WPB, MEM, false>), dim3((unsigned)grid), dim3(WPB * 64), lds,
+                           s, a);
+    else
+        hipLaunchKernelGGL((k_fused_code<T, LT, SL, PPT, WPB, MEM, true>), dim3((unsigned)grid), dim3(WPB * 64), lds,
+                           s, a);
+    return (int)hipGetLastError();
+}
+
+#define SR_F0(T) T, 8, 1, FCfg<T>::B0_PPT, 4, false
+#define SR_F1(T) T, 32, 1, 1, 4, false
+#define SR_F2(T) T, 128, 2, 1, 2, true
+
+constexpr int F2_BLOCKS_PER_CU = 2;
+
+// Workspace: lists [3][B] int32, counters [3]+[3] | grad_X partial rows | global tape (bucket 2).
+struct FWs {
+    size_t gx_off, gt_off, total;
+    int g0, g1, g2;
+    bool gx_lds;
+};
+template <typename T>
+FWs fused_layout(int64_t N, int64_t n_x, int64_t B, int64_t M, bool want_gx) {
+    FWs w;
+    const int64_t gx_len = N * n_x;
+    w.gx_lds = want_gx && n_x <= F_NX && (size_t)gx_len * sizeof(T) <= GX_LDS_MAX2;
+    w.g0 = fused_grid<SR_F0(T)>(w.gx_lds, gx_len, (int)n_x);
+    w.g1 = M > 8 ? fused_grid<SR_F1(T)>(w.gx_lds, gx_len, (int)n_x) : 0;
+    w.g2 = M > 32 ? num_cus() * F2_BLOCKS_PER_CU : 0;
+    size_t off = (((size_t)F_NBUCKETS * B * 4 + 64) + 255) / 256 * 256;
+    w.gx_off = off;
+    if (w.gx_lds) off += (size_t)(w.g0 + w.g1 + w.g2) * gx_len * sizeof(T);
+    off = (off + 255) / 256 * 256;
+    w.gt_off = off;
+    if (w.g2) off += (size_t)w.g2 * 2 * tape_elems(128, 1) * sizeof(T);
+    w.total = off + 256;
+    return w;
+}
+
+template <typename T>
+int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch, const T *C, int64_t M,
+                    int64_t B, const T *dy, int mode, int fwd_only, T *y_out, double *loss_out, T *grad_C,
+                    T *grad_X, int32_t *err_b, int32_t *flags, void *workspace, int64_t workspace_bytes,
+                    hipStream_t s) {
+    if (M <= 0 || M > 128 || B < 0 || N <= 0 || n_x < 0 || n_x > MAX_NX_CODE + 1 || (mode != 0 && mode != 1))
+        return -1;
+    if (dy == nullptr && (mode == 1 || !fwd_only)) return -1;  // the kernel reads dy in these cases
+    if (grad_C == nullptr && !fwd_only) return -1;
+    const bool want_gx = grad_X != nullptr && !fwd_only;
+    const FWs L = fused_layout<T>(N, n_x, B, M, want_gx);
+    if (workspace_bytes < (int64_t)L.total) return -2;
+    hipLaunchKernelGGL(k_fill_i32_e, dim3(1), dim3(64), 0, s, flags, 0, (int64_t)2);
+    const int64_t gx_len = N * n_x;
+    if (want_gx && !L.gx_lds) {  // direct atomics into grad_X
+        const int64_t nb = (gx_len + 255) / 256;
+        if (sizeof(T) == 4)
+            hipLaunchKernelGGL(k_fill_i32_e, dim3((unsigned)nb), dim3(256), 0, s, (int32_t *)grad_X, 0, gx_len);
+        else
+            hipLaunchKernelGGL(k_fill_u64_e, dim3((unsigned)nb), dim3(256), 0, s, (uint64_t *)grad_X, 0ull, gx_len);
+    }
+    if (B == 0) {
+        if (want_gx && L.gx_lds) {
+            const int64_t nb = (gx_len + 255) / 256;
+            if (sizeof(T) == 4)
+                hipLaunchKernelGGL(k_fill_i32_e, dim3((unsigned)nb), dim3(256), 0, s, (int32_t *)grad_X, 0, gx_len);
+            else
+                hipLaunchKernelGGL(k_fill_u64_e, dim3((unsigned)nb), dim3(256), 0, s, (uint64_t *)grad_X, 0ull,
+                                   gx_len);
+        }
+        return (int)hipGetLastError();
+    }
+    char *ws = (char *)workspace;
+    int32_t *lists = (int32_t *)ws;
+    int32_t *counters = (int32_t *)(ws + (size_t)F_NBUCKETS * B * 4);  // [0..2] counts, [3..5] work
+    hipLaunchKernelGGL(k_fill_i32_e, dim3(1), dim3(64), 0, s, counters, 0, (int64_t)(2 * F_NBUCKETS));
+    hipLaunchKernelGGL(k_classify_fused, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, ops, (int)M, B, (int)n_x,
+                       lists, counters);
+    FArgs a;
+    memset(&a, 0, sizeof(a));
+    a.X = X; a.N = N; a.n_x = (int)n_x; a.ops = ops; a.ch = ch; a.C = C; a.M = (int)M; a.B = B; a.dy = dy;
+    a.y_out = y_out; a.loss_out = loss_out; a.grad_C = grad_C;
+    a.gx_partial = L.gx_lds ? (void *)(ws + L.gx_off) : nullptr;
+    a.grad_X_direct = (want_gx && !L.gx_lds) ? grad_X : nullptr;
+    a.err_b = err_b; a.flags = flags;
+    a.gtape = L.g2 ? (void *)(ws + L.gt_off) : nullptr;
+    a.mode = mode; a.fwd_only = fwd_only; a.want_gx = want_gx ? 1 : 0;
+    auto bucket_args = [&](int k, int64_t row0) {
+        FArgs r = a;
+        r.list = lists + (size_t)k * B;
+        r.list_count = counters + k;
+        r.work_counter = counters + F_NBUCKETS + k;
+        r.gx_row0 = row0;
+        return r;
+    };
+    int r = (int)hipGetLastError();
+    if (r) return r;
+    if (M <= 8) {
+        if ((r = fused_launch<SR_F0(T)>(bucket_args(0, 0), L.g0, L.gx_lds, s))) return r;
+    } else {
+        // buckets 2 and 1 on side streams, bucket 0 on the caller's stream
+        Fork &f = fork_for_device();
+        if (!f.ok) return (int)hipErrorOutOfMemory;
+        if ((r = (int)hipEventRecord(f.ev_fork, s))) return r;
+        for (int i = 0; i < 2; ++i)
+            if ((r = (int)hipStreamWaitEvent(f.side[i], f.ev_fork, 0))) return r;
+        if (L.g2 && (r = fused_launch<SR_F2(T)>(bucket_args(2, L.g0 + L.g1), L.g2, L.gx_lds, f.side[0]))) return r;
+        if ((r = fused_launch<SR_F1(T)>(bucket_args(1, L.g0), L.g1, L.gx_lds, f.side[1]))) return r;
+        if ((r = fused_launch<SR_F0(T)>(bucket_args(0, 0), L.g0, L.gx_lds, s))) return r;
+        for (int i = 0; i < 2; ++i) {
+            if ((r = (int)hipEventRecord(f.ev_join[i], f.side[i]))) return r;
+            if ((r = (int)hipStreamWaitEvent(s, f.ev_join[i], 0))) return r;
+        }
+    }
+    if (L.gx_lds) {
+        const int64_t rows = (int64_t)L.g0 + L.g1 + L.g2;
+        hipLaunchKernelGGL(k_sum_rows<T>, dim3((unsigned)((gx_len + 255) / 256)), dim3(256), 0, s,
+                           (const T *)(ws + L.gx_off), rows, gx_len, grad_X);
+    }
+    return (int)hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t sr_fused_workspace_bytes(int64_t N, int64_t n_x, int64_t B, int64_t M, int want_gx, int is_f64) {
+    return is_f64 ? (int64_t)fused_layout<double>(N, n_x, B, M, want_gx != 0).total
+                  : (int64_t)fused_layout<float>(N, n_x, B, M, want_gx != 0).total;
+}
+
+int sr_fused_root_f32(const float *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch, const float *C,
+                      int64_t M, int64_t B, const float *dy, int mode, int fwd_only, float *y_out, double *loss_out,
+                      float *grad_C, float *grad_X, int32_t *err_b, int32_t *flags, void *workspace,
+                      int64_t workspace_bytes, void *stream) {
+    return fused_root_code<float>(X, N, n_x, ops, ch, C, M, B, dy, mode, fwd_only, y_out, loss_out, grad_C, grad_X,
+                                  err_b, flags, workspace, workspace_bytes, (hipStream_t)stream);
+}
+int sr_fused_root_f64(const double *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                      const double *C, int64_t M, int64_t B, const double *dy, int mode, int fwd_only,
+                      double *y_out, double *loss_out, double *grad_C, double *grad_X, int32_t *err_b,
+                      int32_t *flags, void *workspace, int64_t workspace_bytes, void *stream) {
+    return fused_root_code<double>(X, N, n_x, ops, ch, C, M, B, dy, mode, fwd_only, y_out, loss_out, grad_C, grad_X,
+                                   err_b, flags, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+}  // extern "C"
