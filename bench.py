@@ -220,7 +220,18 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-evaluate", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-MiB device buffer (calibrates the HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        src = torch.ones(args.copy_probe * (1 << 20) // 4, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 4}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): fitting problems are independent, "
               "so the path shards by problem with no collective; run one process per GPU.", file=sys.stderr)

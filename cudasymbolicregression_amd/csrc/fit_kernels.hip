@@ -734,7 +734,12 @@ const FitVariant *const kVariants[NBUCKETS] = {kB0, kB1, kB2, kB3, kB4};
 constexpr int kNumVariants[NBUCKETS] = {4, 3, 3, 2, 1};
 int g_variant[NBUCKETS] = {1, 2, 1, 1, 0};  // fastest on the bench workload (autotune.tune_fit)
 
-inline int launch_bucket(int k, const FitArgs &a, hipStream_t s) { return kVariants[k][g_variant[k]].launch(a, s); }
+// The selected variant, or the bucket's first one when the selected one
+// needs more LDS than a workgroup may have at this history size.
+inline int launch_bucket(int k, const FitArgs &a, hipStream_t s) {
+    const int r = kVariants[k][g_variant[k]].launch(a, s);
+    return r == -3 ? kVariants[k][0].launch(a, s) : r;
+}
 
 // Workspace: lists [5][B] int32 | counters [5] counts + [5] work | global tape (bucket 4).
 inline size_t fit_gt_off(int64_t B) { return (((size_t)NBUCKETS * B * 4 + 64) + 255) / 256 * 256; }
