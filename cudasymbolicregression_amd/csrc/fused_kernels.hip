@@ -14,11 +14,11 @@
 // Execution (gfx950): one wave per expression; the program is compiled once
 // per wave into an instruction list (sr_code.h) and evaluated over tile
 // groups of 64*PPT points with the tape and adjoints in LDS.  Expressions are
-// bucketed by instruction count (<= 8, <= 32, the rest) so each bucket's
+// bucketed by instruction count (<= 8, <= 16, <= 64, the rest) so each bucket's
 // kernel is sized for its tape; the buckets run concurrently on forked
 // streams.  Variable adjoints are summed per block in LDS (float atomics)
-// and written as one partial row per block; a final kernel adds the rows in
-// FP64 in row order.
+// and written as one partial row per block; two final kernels add the rows
+// in FP64 in a fixed order.
 #include "sr_code.h"
 #include "sr_device_info.h"
 #include "sr_streams.h"
@@ -32,7 +32,7 @@ namespace {
 
 constexpr int F_NCREG = 8;  // constants accumulated in per-lane FP64 registers
 constexpr int F_NX = 4;     // variables held in registers per point
-constexpr int F_NBUCKETS = 3;
+constexpr int F_NBUCKETS = 4;
 constexpr size_t GX_LDS_MAX2 = 24 * 1024;  // largest per-block grad_X partial kept in LDS
 
 struct FArgs {
@@ -327,17 +327,19 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
     }
 }
 
-// Buckets by instruction count NI and constant count nc: NI <= 8, NI <= 32
-// (both with nc <= 64), the rest.
-__global__ void k_classify_fused(const int64_t *__restrict__ ops, int M, int64_t B, int n_x, int32_t *lists,
-                                 int32_t *counts) {
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Buckets by instruction count NI and constant count nc: NI <= 8, <= 16,
+// <= 64 (all with nc <= 64), the rest.  One wave per expression.
+__global__ __launch_bounds__(256) void k_classify_fused(const int64_t *__restrict__ ops, int M, int64_t B, int n_x,
+                                                        int32_t *lists, int32_t *counts) {
+    const int64_t b = (int64_t)blockIdx.x * 4 + uniform_i32((int)(threadIdx.x / WAVE));
     if (b >= B) return;
     int nc = 0;
-    const int ni = count_instructions(ops + b * M, M, n_x, nc);
-    const int bucket = nc > 64 ? 2 : (ni <= 8 ? 0 : (ni <= 32 ? 1 : 2));
-    const int slot = atomicAdd(&counts[bucket], 1);
-    lists[(int64_t)bucket * B + slot] = (int32_t)b;
+    const int ni = count_instructions_wave(ops + b * M, M, n_x, nc);
+    const int bucket = nc > 64 ? 3 : (ni <= 8 ? 0 : (ni <= 16 ? 1 : (ni <= 64 ? 2 : 3)));
+    if (lane_id() == 0) {
+        const int slot = atomicAdd(&counts[bucket], 1);
+        lists[(int64_t)bucket * B + slot] = (int32_t)b;
+    }
 }
 
 __global__ void k_fill_i32_e(int32_t *p, int32_t v, int64_t n) {
@@ -349,13 +351,27 @@ __global__ void k_fill_u64_e(uint64_t *p, uint64_t v, int64_t n) {
     if (i < n) p[i] = v;
 }
 
-// Sums P partial rows of length len in FP64, in row order.
+// Sums P partial rows of length len in FP64 in two deterministic stages:
+// chunk c of ROW_CHUNK rows -> part64[c][i] (row order), then the chunks in
+// order -> out[i].
+constexpr int ROW_CHUNK = 32;
+
 template <typename T>
-__global__ void k_sum_rows(const T *__restrict__ part, int64_t P, int64_t len, T *__restrict__ out) {
+__global__ void k_sum_rows_1(const T *__restrict__ part, int64_t P, int64_t len, double *__restrict__ part64) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    const int64_t r0 = (int64_t)blockIdx.y * ROW_CHUNK;
+    const int64_t r1 = r0 + ROW_CHUNK < P ? r0 + ROW_CHUNK : P;
+    double s = 0.0;
+    for (int64_t r = r0; r < r1; ++r) s += (double)part[r * len + i];
+    part64[(int64_t)blockIdx.y * len + i] = s;
+}
+template <typename T>
+__global__ void k_sum_rows_2(const double *__restrict__ part64, int64_t chunks, int64_t len, T *__restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= len) return;
     double s = 0.0;
-    for (int64_t r = 0; r < P; ++r) s += (double)part[r * len + i];
+    for (int64_t c = 0; c < chunks; ++c) s += part64[c * len + i];
     out[i] = (T)s;
 }
 
@@ -399,15 +415,17 @@ int fused_launch(const FArgs &a, int grid, bool gx_lds, hipStream_t s) {
 }
 
 #define SR_F0(T) T, 8, 1, FCfg<T>::B0_PPT, 4, false
-#define SR_F1(T) T, 32, 1, 1, 4, false
-#define SR_F2(T) T, 128, 2, 1, 2, true
+#define SR_F1(T) T, 16, 1, FCfg<T>::B0_PPT, 4, false
+#define SR_F2(T) T, 64, 1, 1, 2, false
+#define SR_F3(T) T, 128, 2, 1, 2, true
 
-constexpr int F2_BLOCKS_PER_CU = 2;
+constexpr int F3_BLOCKS_PER_CU = 2;
 
-// Workspace: lists [3][B] int32, counters [3]+[3] | grad_X partial rows | global tape (bucket 2).
+// Workspace: lists [4][B] int32, counters [4]+[4] | grad_X partial rows |
+// FP64 chunk sums of the rows | global tape (bucket 3).
 struct FWs {
-    size_t gx_off, gt_off, total;
-    int g0, g1, g2;
+    size_t gx_off, g64_off, gt_off, total;
+    int g0, g1, g2, g3;
     bool gx_lds;
 };
 template <typename T>
@@ -417,13 +435,18 @@ FWs fused_layout(int64_t N, int64_t n_x, int64_t B, int64_t M, bool want_gx) {
     w.gx_lds = want_gx && n_x <= F_NX && (size_t)gx_len * sizeof(T) <= GX_LDS_MAX2;
     w.g0 = fused_grid<SR_F0(T)>(w.gx_lds, gx_len, (int)n_x);
     w.g1 = M > 8 ? fused_grid<SR_F1(T)>(w.gx_lds, gx_len, (int)n_x) : 0;
-    w.g2 = M > 32 ? num_cus() * F2_BLOCKS_PER_CU : 0;
+    w.g2 = M > 16 ? fused_grid<SR_F2(T)>(w.gx_lds, gx_len, (int)n_x) : 0;
+    w.g3 = M > 64 ? num_cus() * F3_BLOCKS_PER_CU : 0;
+    const int64_t rows = (int64_t)w.g0 + w.g1 + w.g2 + w.g3;
     size_t off = (((size_t)F_NBUCKETS * B * 4 + 64) + 255) / 256 * 256;
     w.gx_off = off;
-    if (w.gx_lds) off += (size_t)(w.g0 + w.g1 + w.g2) * gx_len * sizeof(T);
+    if (w.gx_lds) off += (size_t)rows * gx_len * sizeof(T);
+    off = (off + 255) / 256 * 256;
+    w.g64_off = off;
+    if (w.gx_lds) off += (size_t)((rows + ROW_CHUNK - 1) / ROW_CHUNK) * gx_len * sizeof(double);
     off = (off + 255) / 256 * 256;
     w.gt_off = off;
-    if (w.g2) off += (size_t)w.g2 * 2 * tape_elems(128, 1) * sizeof(T);
+    if (w.g3) off += (size_t)w.g3 * 2 * tape_elems(128, 1) * sizeof(T);
     w.total = off + 256;
     return w;
 }
@@ -464,7 +487,7 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
     int32_t *lists = (int32_t *)ws;
     int32_t *counters = (int32_t *)(ws + (size_t)F_NBUCKETS * B * 4);  // [0..2] counts, [3..5] work
     hipLaunchKernelGGL(k_fill_i32_e, dim3(1), dim3(64), 0, s, counters, 0, (int64_t)(2 * F_NBUCKETS));
-    hipLaunchKernelGGL(k_classify_fused, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, ops, (int)M, B, (int)n_x,
+    hipLaunchKernelGGL(k_classify_fused, dim3((unsigned)((B + 3) / 4)), dim3(256), 0, s, ops, (int)M, B, (int)n_x,
                        lists, counters);
     FArgs a;
     memset(&a, 0, sizeof(a));
@@ -473,7 +496,7 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
     a.gx_partial = L.gx_lds ? (void *)(ws + L.gx_off) : nullptr;
     a.grad_X_direct = (want_gx && !L.gx_lds) ? grad_X : nullptr;
     a.err_b = err_b; a.flags = flags;
-    a.gtape = L.g2 ? (void *)(ws + L.gt_off) : nullptr;
+    a.gtape = L.g3 ? (void *)(ws + L.gt_off) : nullptr;
     a.mode = mode; a.fwd_only = fwd_only; a.want_gx = want_gx ? 1 : 0;
     auto bucket_args = [&](int k, int64_t row0) {
         FArgs r = a;
@@ -488,12 +511,14 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
     if (M <= 8) {
         if ((r = fused_launch<SR_F0(T)>(bucket_args(0, 0), L.g0, L.gx_lds, s))) return r;
     } else {
-        // buckets 2 and 1 on side streams, bucket 0 on the caller's stream
+        // buckets 3+2 and 1 on side streams, bucket 0 on the caller's stream
         Fork &f = fork_for_device();
         if (!f.ok) return (int)hipErrorOutOfMemory;
         if ((r = (int)hipEventRecord(f.ev_fork, s))) return r;
         for (int i = 0; i < 2; ++i)
             if ((r = (int)hipStreamWaitEvent(f.side[i], f.ev_fork, 0))) return r;
+        if (L.g3 && (r = fused_launch<SR_F3(T)>(bucket_args(3, L.g0 + L.g1 + L.g2), L.g3, L.gx_lds, f.side[0])))
+            return r;
         if (L.g2 && (r = fused_launch<SR_F2(T)>(bucket_args(2, L.g0 + L.g1), L.g2, L.gx_lds, f.side[0]))) return r;
         if ((r = fused_launch<SR_F1(T)>(bucket_args(1, L.g0), L.g1, L.gx_lds, f.side[1]))) return r;
         if ((r = fused_launch<SR_F0(T)>(bucket_args(0, 0), L.g0, L.gx_lds, s))) return r;
@@ -503,9 +528,13 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
         }
     }
     if (L.gx_lds) {
-        const int64_t rows = (int64_t)L.g0 + L.g1 + L.g2;
-        hipLaunchKernelGGL(k_sum_rows<T>, dim3((unsigned)((gx_len + 255) / 256)), dim3(256), 0, s,
-                           (const T *)(ws + L.gx_off), rows, gx_len, grad_X);
+        const int64_t rows = (int64_t)L.g0 + L.g1 + L.g2 + L.g3;
+        const int64_t chunks = (rows + ROW_CHUNK - 1) / ROW_CHUNK;
+        const unsigned nb = (unsigned)((gx_len + 255) / 256);
+        hipLaunchKernelGGL(k_sum_rows_1<T>, dim3(nb, (unsigned)chunks), dim3(256), 0, s, (const T *)(ws + L.gx_off),
+                           rows, gx_len, (double *)(ws + L.g64_off));
+        hipLaunchKernelGGL(k_sum_rows_2<T>, dim3(nb), dim3(256), 0, s, (const double *)(ws + L.g64_off), chunks,
+                           gx_len, grad_X);
     }
     return (int)hipGetLastError();
 }

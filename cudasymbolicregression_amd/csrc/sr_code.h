@@ -121,19 +121,23 @@ __device__ inline uint32_t operand_desc(uint32_t w, uint32_t instr_index, bool i
 }
 
 // Instruction count of a batch-first program row under the same rule as
-// compile_code (used to bucket problems before they are compiled).
-__device__ inline int count_instructions(const int64_t *row, int M, int n_x, int &nc) {
-    int nn = 0;
-    nc = 0;
-    for (int k = 0; k < M; ++k) {
-        if (row[k] != NO_OP) ++nn;
-        if (row[k] == LEARNABLE_CONSTANT) ++nc;
-    }
+// compile_code (used to bucket problems before they are compiled): one wave
+// per row, lane k reading node k (and 64 + k), so the reads are coalesced.
+// Returns wave-uniform values.
+__device__ inline int count_instructions_wave(const int64_t *row, int M, int n_x, int &nc) {
+    const int lane = lane_id();
+    int64_t op0 = NO_OP, op1 = NO_OP;
+    if (lane < M) op0 = row[lane];
+    if (lane + WAVE < M) op1 = row[lane + WAVE];
+    const uint64_t nn0 = __ballot(op0 != NO_OP), nn1 = __ballot(op1 != NO_OP);
+    nc = __popcll(__ballot(op0 == LEARNABLE_CONSTANT)) + __popcll(__ballot(op1 == LEARNABLE_CONSTANT));
+    const int nn = __popcll(nn0) + __popcll(nn1);
     const int root = nn > 0 ? nn - 1 : 0;
-    int ni = 0;
-    for (int k = 0; k <= root && k < M; ++k)
-        if (arity_of(node_op(pack_node(row[k], -1, -1, n_x))) > 0) ++ni;
-    const bool root_is_op = arity_of(node_op(pack_node(row[root], -1, -1, n_x))) > 0;
+    const bool is0 = lane < M && lane <= root && arity_of(node_op(pack_node(op0, -1, -1, n_x))) > 0;
+    const bool is1 = lane + WAVE < M && lane + WAVE <= root && arity_of(node_op(pack_node(op1, -1, -1, n_x))) > 0;
+    const uint64_t m0 = __ballot(is0), m1 = __ballot(is1);
+    const int ni = __popcll(m0) + __popcll(m1);
+    const bool root_is_op = root < WAVE ? ((m0 >> root) & 1ull) : ((m1 >> (root - WAVE)) & 1ull);
     return ni + (root_is_op ? 0 : 1);
 }
 
