@@ -67,6 +67,7 @@ LOSS_SLOTS = 5  # d = y - Y, d*d, accumulate, 2*d, *1/N
 FP32_PEAK_TFLOPS = 157.3
 VALU_LANE_PEAK = 256 * 4 * 16 * 2.4e9  # unpacked FP32 lane-instructions/s (3.93e13)
 HBM_PEAK_GBS = 8000.0
+FIT_TRAFFIC_PROFILED = 81.2e6  # HBM bytes per fit call, rocprofv3 FETCH_SIZE/WRITE_SIZE (calibrated)
 
 
 def log(msg: str) -> None:
@@ -270,7 +271,9 @@ def main() -> int:
     achieved = flops / (ms / 1000.0) / 1e12
     roofline = dict(bound="mfma", achieved=round(achieved, 4), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=round(achieved / FP32_PEAK_TFLOPS, 5), traffic=None,
-                    kernel="k_lbfgs (all length buckets of one fit)", flop_per_launch=flops)
+                    traffic_profiled_bytes=FIT_TRAFFIC_PROFILED, traffic_source="profiles/fit_hbm_traffic.md",
+                    kernel="sr_lbfgs_fit: the k_fit bucket kernels of one call (concurrent)",
+                    flop_per_launch=flops, frac_unpacked_lane_peak=round(flops / (ms / 1e3) / VALU_LANE_PEAK, 5))
 
     ev = None
     if not args.no_evaluate:
