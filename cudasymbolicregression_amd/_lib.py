@@ -13,7 +13,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import c_double, c_int, c_int64, c_void_p, POINTER
+from ctypes import c_double, c_float, c_int, c_int64, c_void_p, POINTER
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 HIP_LIB_PATH = os.path.join(_HERE, "libsr_hip.so")
@@ -78,6 +78,18 @@ def hip() -> ctypes.CDLL:
         _declare(lib, "sr_fit_get_variant", I, [I])
         _declare(lib, "sr_fit_set_variant", I, [I, I])
         _declare(lib, "sr_fit_variant_info", I, [I, I, POINTER(c_int), POINTER(c_int)])
+        F = c_float
+        _declare(lib, "sr_mlp_gemm_nt", I, [P, P, P, I64, I64, I64, I, P, P, P, P])
+        _declare(lib, "sr_mlp_transpose", I, [P, P, I64, I64, P])
+        _declare(lib, "sr_mlp_gelu_bwd", I, [P, P, I64, P, P, P])
+        _declare(lib, "sr_mlp_col_sum_workspace_bytes", I64, [I64, I64])
+        _declare(lib, "sr_mlp_col_sum", I, [P, I64, I64, P, P, I64, P])
+        _declare(lib, "sr_mlp_adam", I, [P, P, P, P, I64, F, F, F, F, I64, F, P, P])
+        _declare(lib, "sr_mlp_to_bf16", I, [P, I64, P, P])
+        _declare(lib, "sr_mlp_features", I, [P, I64, I64, I64, I, I, P, I64, I, I, P, P])
+        _declare(lib, "sr_mlp_mse_head", I, [P, P, P, I64, I64, I64, F, P, P, P, P])
+        _declare(lib, "sr_mlp_predict", I, [P, P, I64, I64, P, P])
+        _declare(lib, "sr_mlp_select_start", I, [P, P, P, P, I64, I64, P, P, P])
         _hip = lib
     return _hip
 

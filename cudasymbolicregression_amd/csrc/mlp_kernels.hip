@@ -1,0 +1,322 @@
+// Warm-start MLP kernels (gfx950): BF16 operands on MFMA with FP32
+// accumulation, hand-written forward, backward and Adam.
+//
+// The network maps an expression and its targets to starting constants:
+//   input  x = one-hot(Ops) over 64 slots x V opcodes  (+)  g(Y) at 256 points
+//   h1 = gelu(x W1^T + b1), h2 = gelu(h1 W2^T + b2), h3 = gelu(h2 W3^T + b3)
+//   out = h3 W4^T + b4, masked to the LEARNABLE_CONSTANT slots.
+// g(y) = sign(y) * log1p(|y|) (bounded growth for exp-type targets).
+//
+// Every matrix product is C[M,N] = A[M,K] * Bt[N,K]^T ("NT", both operands
+// K-contiguous), which is the natural layout of a linear layer (W stored
+// out x in); the backward products use it after explicit transposes.
+// One wave computes a 32x32 tile with v_mfma_f32_32x32x16_bf16; a block of
+// four waves covers 64x64.  Operand fragments are read straight from global
+// memory (16 contiguous bytes per lane and k-step), which is enough for this
+// network's size (the fitting kernels dominate the time).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int WAVE = 64;
+constexpr int N_OUT = 64;  // output slots of the network (padded expression length)
+
+// Round-to-nearest-even FP32 -> BF16 bits (NaN stays NaN).
+__device__ __host__ inline uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ inline float gelu_grad(float z) {
+    const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
+    const float pdf = 0.39894228040143268f * expf(-0.5f * z * z);
+    return cdf + z * pdf;
+}
+
+enum Epi : int { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2 };
+
+// C = A * Bt^T (+ bias), M and N multiples of 32, K a multiple of 16.
+// MFMA 32x32x16 lane layout (per wave): lane l holds A[i = l % 32][k0 + 8*(l/32) + e]
+// and Bt[j = l % 32][k0 + 8*(l/32) + e] for e = 0..7; accumulator register r of
+// lane l is C[8*(r/4) + 4*(l/32) + r%4][l % 32].
+__global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t *__restrict__ A, const uint16_t *__restrict__ Bt,
+                                                const float *__restrict__ bias, int M, int N, int K, int epi,
+                                                float *__restrict__ out_f32, uint16_t *__restrict__ out_bf16,
+                                                float *__restrict__ pre_f32) {
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int i0 = blockIdx.y * 64 + (wave / 2) * 32;
+    const int j0 = blockIdx.x * 64 + (wave % 2) * 32;
+    if (i0 >= M || j0 >= N) return;  // whole wave leaves together
+    const int r = lane % 32, kh = (lane / 32) * 8;
+    const uint16_t *pa = A + (size_t)(i0 + r) * K + kh;
+    const uint16_t *pb = Bt + (size_t)(j0 + r) * K + kh;
+    f32x16 acc;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.0f;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(pa + k0);
+        const bf16x8 b = *reinterpret_cast<const bf16x8 *>(pb + k0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    const int j = j0 + r;
+    const float bj = (epi != EPI_STORE && bias) ? bias[j] : 0.0f;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int i = i0 + 8 * (q / 4) + 4 * (lane / 32) + q % 4;
+        const float z = acc[q] + bj;
+        const size_t o = (size_t)i * N + j;
+        if (epi == EPI_BIAS_GELU) {
+            if (pre_f32) pre_f32[o] = z;
+            const float h = gelu(z);
+            if (out_f32) out_f32[o] = h;
+            if (out_bf16) out_bf16[o] = f2bf(h);
+        } else {
+            if (out_f32) out_f32[o] = z;
+            if (out_bf16) out_bf16[o] = f2bf(z);
+        }
+    }
+}
+
+// out[c][r] = in[r][c] for a rows x cols BF16 matrix (32 x 32 tiles in LDS).
+__global__ __launch_bounds__(256) void k_transpose(const uint16_t *__restrict__ in, uint16_t *__restrict__ out,
+                                                  int rows, int cols) {
+    __shared__ uint16_t tile[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 32 x 8
+    for (int y = ty; y < 32; y += 8) {
+        const int r = r0 + y, c = c0 + tx;
+        tile[y][tx] = (r < rows && c < cols) ? in[(size_t)r * cols + c] : (uint16_t)0;
+    }
+    __syncthreads();
+    for (int y = ty; y < 32; y += 8) {
+        const int c = c0 + y, r = r0 + tx;
+        if (c < cols && r < rows) out[(size_t)c * rows + r] = tile[tx][y];
+    }
+}
+
+// dZ = dH * gelu'(Z) (FP32 in, BF16 and FP32 out).
+__global__ void k_gelu_bwd(const float *__restrict__ dH, const float *__restrict__ Z, int64_t n,
+                           float *__restrict__ dZ_f32, uint16_t *__restrict__ dZ_bf16) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float d = dH[i] * gelu_grad(Z[i]);
+    if (dZ_f32) dZ_f32[i] = d;
+    if (dZ_bf16) dZ_bf16[i] = f2bf(d);
+}
+
+// db[n] = sum_m dZ[m][n]: FP64 partial sums over chunks of COL_ROWS rows,
+// then the chunks in order (deterministic).
+constexpr int COL_ROWS = 64;
+__global__ void k_col_sum_part(const float *__restrict__ dZ, int M, int N, double *__restrict__ part) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int r0 = blockIdx.y * COL_ROWS, r1 = min(r0 + COL_ROWS, M);
+    double s = 0.0;
+    for (int m = r0; m < r1; ++m) s += (double)dZ[(size_t)m * N + n];
+    part[(size_t)blockIdx.y * N + n] = s;
+}
+__global__ void k_col_sum_fin(const double *__restrict__ part, int chunks, int N, float *__restrict__ db) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double s = 0.0;
+    for (int c = 0; c < chunks; ++c) s += part[(size_t)c * N + n];
+    db[n] = (float)s;
+}
+
+// Adam on FP32 master weights; also refreshes their BF16 copy.
+__global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, int64_t n, float lr, float b1, float b2, float eps, float bc1,
+                       float bc2, float wd, uint16_t *__restrict__ w_bf16) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i] + wd * w[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    w[i] = wi;
+    if (w_bf16) w_bf16[i] = f2bf(wi);
+}
+
+__global__ void k_to_bf16(const float *__restrict__ x, int64_t n, uint16_t *__restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = f2bf(x[i]);
+}
+
+// Input rows: one-hot over (slot, opcode) for the first `slots` nodes
+// (opcodes outside [0, vocab) encode nothing), then g(Y[b][p]) for p <
+// n_pts, zero padding up to row_len.  Rows b >= B (batch padding up to a
+// multiple of 32) are all zero.  One block per row.
+__global__ void k_features(const int64_t *__restrict__ ops, int M, int64_t B, int slots, int vocab,
+                           const float *__restrict__ Y, int64_t ldy, int n_pts, int row_len,
+                           uint16_t *__restrict__ out) {
+    const int64_t b = blockIdx.x;
+    uint16_t *row = out + (size_t)b * row_len;
+    const int onehot = slots * vocab;
+    for (int i = threadIdx.x; i < row_len; i += blockDim.x) {
+        float v = 0.0f;
+        if (b < B && i < onehot) {
+            const int k = i / vocab, c = i % vocab;
+            v = (k < M && ops[b * M + k] == c) ? 1.0f : 0.0f;
+        } else if (b < B && i < onehot + n_pts) {
+            const float y = Y[b * ldy + (i - onehot)];
+            v = (y == y) ? copysignf(log1pf(fminf(fabsf(y), 3.0e38f)), y) : 0.0f;
+        }
+        row[i] = f2bf(v);
+    }
+}
+
+// Masked MSE head: d = (out - label) on the constant slots (Ops == 1) of rows
+// b < rows, 0 elsewhere (padding rows up to rows_p included);
+// dO = 2 d * inv_count (FP32 and BF16), loss[b] = sum_k d^2.
+// Block (64, 4): one wave per row, lane k = output slot k (N_OUT == WAVE).
+__global__ void k_mse_head(const float *__restrict__ out, const float *__restrict__ label,
+                           const int64_t *__restrict__ ops, int M, int64_t rows, int64_t rows_p, float inv_count,
+                           float *__restrict__ dO_f32, uint16_t *__restrict__ dO_bf16, float *__restrict__ loss) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.y + threadIdx.y;
+    if (b >= rows_p) return;
+    const int k = threadIdx.x;
+    const bool is_c = b < rows && k < M && ops[b * M + k] == 1;
+    const size_t o = (size_t)b * N_OUT + k;
+    const float d = is_c ? out[o] - label[b * M + k] : 0.0f;
+    const float g = 2.0f * d * inv_count;
+    if (dO_f32) dO_f32[o] = g;
+    if (dO_bf16) dO_bf16[o] = f2bf(g);
+    if (loss) {
+        float s = d * d;
+        for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (k == 0) loss[b] = s;
+    }
+}
+
+// Predicted constants: C[b][k] = out[b][k] on the constant slots, 0 elsewhere.
+__global__ void k_predict(const float *__restrict__ out, const int64_t *__restrict__ ops, int M, int64_t B,
+                          float *__restrict__ C) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B * M) return;
+    const int64_t b = i / M;
+    const int k = (int)(i % M);
+    C[i] = ops[i] == 1 ? out[b * N_OUT + k] : 0.0f;
+}
+
+// Start point per problem: the warm start where its loss is finite and not
+// above the random start's loss, else the random start.  used[b] = 1 when
+// the warm start was taken.
+__global__ void k_select_start(const float *__restrict__ C_warm, const float *__restrict__ C_rand,
+                               const double *__restrict__ loss_warm, const double *__restrict__ loss_rand, int M,
+                               int64_t B, float *__restrict__ C_start, int32_t *__restrict__ used) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B * M) return;
+    const int64_t b = i / M;
+    const double lw = loss_warm[b], lr = loss_rand[b];
+    const bool warm = (lw == lw) && !(lw > lr);  // NaN warm loss never wins; NaN random loss always loses
+    C_start[i] = warm ? C_warm[i] : C_rand[i];
+    if (used && i % M == 0) used[b] = warm ? 1 : 0;
+}
+
+inline int last() { return (int)hipGetLastError(); }
+inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+extern "C" {
+
+int sr_mlp_gemm_nt(const void *A, const void *Bt, const float *bias, int64_t M, int64_t N, int64_t K, int epi,
+                   float *out_f32, void *out_bf16, float *pre_f32, void *stream) {
+    if (M <= 0 || N <= 0 || K <= 0 || M % 32 || N % 32 || K % 16 || epi < 0 || epi > 2) return -1;
+    hipLaunchKernelGGL(k_gemm_nt, dim3(blocks(N, 64), blocks(M, 64)), dim3(256), 0, (hipStream_t)stream,
+                       (const uint16_t *)A, (const uint16_t *)Bt, bias, (int)M, (int)N, (int)K, epi, out_f32,
+                       (uint16_t *)out_bf16, pre_f32);
+    return last();
+}
+
+int sr_mlp_transpose(const void *in, void *out, int64_t rows, int64_t cols, void *stream) {
+    if (rows <= 0 || cols <= 0) return -1;
+    hipLaunchKernelGGL(k_transpose, dim3(blocks(cols, 32), blocks(rows, 32)), dim3(256), 0, (hipStream_t)stream,
+                       (const uint16_t *)in, (uint16_t *)out, (int)rows, (int)cols);
+    return last();
+}
+
+int sr_mlp_gelu_bwd(const float *dH, const float *Z, int64_t n, float *dZ_f32, void *dZ_bf16, void *stream) {
+    if (n < 0) return -1;
+    if (n) hipLaunchKernelGGL(k_gelu_bwd, dim3(blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, dH, Z, n, dZ_f32,
+                              (uint16_t *)dZ_bf16);
+    return last();
+}
+
+int64_t sr_mlp_col_sum_workspace_bytes(int64_t M, int64_t N) {
+    return M <= 0 || N <= 0 ? 0 : ((M + COL_ROWS - 1) / COL_ROWS) * N * (int64_t)sizeof(double);
+}
+
+int sr_mlp_col_sum(const float *dZ, int64_t M, int64_t N, float *db, void *workspace, int64_t workspace_bytes,
+                   void *stream) {
+    if (M <= 0 || N <= 0 || M > INT32_MAX / 2 || N > INT32_MAX / 2) return -1;
+    if (workspace_bytes < sr_mlp_col_sum_workspace_bytes(M, N)) return -2;
+    const int chunks = (int)((M + COL_ROWS - 1) / COL_ROWS);
+    hipLaunchKernelGGL(k_col_sum_part, dim3(blocks(N, 256), chunks), dim3(256), 0, (hipStream_t)stream, dZ, (int)M,
+                       (int)N, (double *)workspace);
+    hipLaunchKernelGGL(k_col_sum_fin, dim3(blocks(N, 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const double *)workspace, chunks, (int)N, db);
+    return last();
+}
+
+int sr_mlp_adam(float *w, const float *g, float *m, float *v, int64_t n, float lr, float b1, float b2, float eps,
+                int64_t step, float weight_decay, void *w_bf16, void *stream) {
+    if (n < 0 || step < 1) return -1;
+    const float bc1 = 1.0f - powf(b1, (float)step), bc2 = 1.0f - powf(b2, (float)step);
+    if (n) hipLaunchKernelGGL(k_adam, dim3(blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, w, g, m, v, n, lr, b1,
+                              b2, eps, bc1, bc2, weight_decay, (uint16_t *)w_bf16);
+    return last();
+}
+
+int sr_mlp_to_bf16(const float *x, int64_t n, void *y, void *stream) {
+    if (n < 0) return -1;
+    if (n) hipLaunchKernelGGL(k_to_bf16, dim3(blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, x, n,
+                              (uint16_t *)y);
+    return last();
+}
+
+int sr_mlp_features(const int64_t *ops, int64_t M, int64_t B, int64_t B_pad, int slots, int vocab, const float *Y,
+                    int64_t ldy, int n_pts, int row_len, void *out, void *stream) {
+    if (B < 0 || B_pad < B || M <= 0 || slots < 0 || vocab <= 0 || n_pts < 0 || n_pts > ldy ||
+        (int64_t)slots * vocab + n_pts > row_len)
+        return -1;
+    if (B_pad) hipLaunchKernelGGL(k_features, dim3((unsigned)B_pad), dim3(256), 0, (hipStream_t)stream, ops, (int)M,
+                                  B, slots, vocab, Y, ldy, n_pts, row_len, (uint16_t *)out);
+    return last();
+}
+
+int sr_mlp_mse_head(const float *out, const float *label, const int64_t *ops, int64_t M, int64_t rows,
+                    int64_t rows_p, float inv_count, float *dO_f32, void *dO_bf16, float *loss, void *stream) {
+    if (M <= 0 || M > N_OUT || rows < 0 || rows_p < rows) return -1;
+    if (rows_p) hipLaunchKernelGGL(k_mse_head, dim3(blocks(rows_p, 4)), dim3(64, 4), 0, (hipStream_t)stream, out,
+                                   label, ops, (int)M, rows, rows_p, inv_count, dO_f32, (uint16_t *)dO_bf16, loss);
+    return last();
+}
+
+int sr_mlp_predict(const float *out, const int64_t *ops, int64_t M, int64_t B, float *C, void *stream) {
+    if (M <= 0 || M > N_OUT || B < 0) return -1;
+    if (B) hipLaunchKernelGGL(k_predict, dim3(blocks(B * M, 256)), dim3(256), 0, (hipStream_t)stream, out, ops,
+                              (int)M, B, C);
+    return last();
+}
+
+int sr_mlp_select_start(const float *C_warm, const float *C_rand, const double *loss_warm, const double *loss_rand,
+                        int64_t M, int64_t B, float *C_start, int32_t *used, void *stream) {
+    if (M <= 0 || B < 0) return -1;
+    if (B) hipLaunchKernelGGL(k_select_start, dim3(blocks(B * M, 256)), dim3(256), 0, (hipStream_t)stream, C_warm,
+                              C_rand, loss_warm, loss_rand, (int)M, B, C_start, used);
+    return last();
+}
+
+}  // extern "C"

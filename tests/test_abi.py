@@ -8,7 +8,7 @@ import subprocess
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = {"sr_eval.h": "libsr_hip.so", "sr_pcfg.h": "libsr_host.so"}
+HEADERS = {"sr_eval.h": "libsr_hip.so", "sr_mlp.h": "libsr_hip.so", "sr_pcfg.h": "libsr_host.so"}
 
 
 def _declared(header):
@@ -47,7 +47,7 @@ def test_python_loader_binds_every_symbol():
     _lib.host()
     assert _lib.MISSING_SYMBOLS == []
     exported = _exported("libsr_hip.so") | _exported("libsr_host.so")
-    declared = set(_declared("sr_eval.h")) | set(_declared("sr_pcfg.h"))
+    declared = set().union(*(_declared(h) for h in HEADERS))
     # every sr_* function the libraries export is part of the documented boundary
     assert {n for n in exported if n.startswith("sr_")} <= declared
 
