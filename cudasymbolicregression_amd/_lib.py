@@ -78,6 +78,7 @@ def hip() -> ctypes.CDLL:
         _declare(lib, "sr_fit_get_variant", I, [I])
         _declare(lib, "sr_fit_set_variant", I, [I, I])
         _declare(lib, "sr_fit_variant_info", I, [I, I, POINTER(c_int), POINTER(c_int)])
+        _declare(lib, "sr_fit_set_stationary_exit", I, [I])
         F = c_float
         _declare(lib, "sr_mlp_gemm_nt", I, [P, P, P, I64, I64, I64, I, P, P, P, P])
         _declare(lib, "sr_mlp_transpose", I, [P, P, I64, I64, P])

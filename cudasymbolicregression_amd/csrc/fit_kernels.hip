@@ -84,6 +84,7 @@ struct FitArgs {
     const int32_t *list_count;
     int32_t *work_counter;
     float *gtape;            // global tape slots (programs longer than 64 nodes)
+    int stationary_exit;     // end a problem once its iteration is an exact fixed point
 };
 
 // FP64 inner product over the K live slots; lanes past K never contribute.
@@ -370,6 +371,8 @@ void k_fit(FitArgs a) {
         float a_eval = 0.0f, ai = 1.0f, a_prev = 0.0f, lo = 0.0f, hi = 0.0f, alpha = 0.0f;
         double f = 0.0, f_new = 0.0, dphi0 = 0.0, phi_prev = 0.0, phi_lo = 0.0;
         int ls = 0, zi = 0, it = 0, nh = 0, head = 0, nclos = 0;
+        int zrun = 0;          // newest history pairs with s == 0 and y == 0, consecutively
+        bool fz_probe = false; // this line search starts from an all-zero direction
         bool d_finite = true;
         int32_t err = 0;
 
@@ -414,6 +417,21 @@ void k_fit(FitArgs a) {
             for (int h = 0; h < SL; ++h) ga.v[h] = (float)gcd[h];
             ++nclos;
             if (e2) { err = e2; break; }
+            if (fz_probe) {
+                // All stored pairs are zero, so the direction is zero and x + d == x.
+                // If the objective at x reproduces (f, g) bit for bit, this
+                // iteration accepts alpha = 1 and leaves (x, f, g) unchanged while
+                // pushing another zero pair: every remaining iteration repeats it.
+                fz_probe = false;
+                bool same = (phi == f);
+#pragma unroll
+                for (int h = 0; h < SL; ++h)
+                    if (lane + h * WAVE < K && __float_as_uint(ga.v[h]) != __float_as_uint(g.v[h])) same = false;
+                if (__ballot(!same) == 0ull) {
+                    it = a.max_iter;
+                    break;
+                }
+            }
 
             // ---------------- advance the iteration's state
             bool accept = false;  // (alpha, f_new, g_new) decided
@@ -525,6 +543,13 @@ void k_fit(FitArgs a) {
                     Yh[((size_t)slot * SL + h) * WAVE + lane] = yv.v[h];
                 }
                 if (lane == 0) rho_l[slot] = rho_new;
+                {
+                    bool nz = false;
+#pragma unroll
+                    for (int h = 0; h < SL; ++h)
+                        if (lane + h * WAVE < K && (s.v[h] != 0.0f || yv.v[h] != 0.0f)) nz = true;
+                    zrun = __ballot(nz) == 0ull ? (zrun < hist ? zrun + 1 : hist) : 0;
+                }
                 g = g_new;
                 f = f_new;
                 ++it;
@@ -598,6 +623,15 @@ void k_fit(FitArgs a) {
                 for (int h = 0; h < SL; ++h)
                     if (lane + h * WAVE < K && !isfinite(d.v[h])) bad = true;
                 d_finite = __ballot(bad) == 0ull;
+            }
+            if (a.stationary_exit && nh > 0 && zrun >= nh) {
+                // zero direction and no -0 in x (x + (+-0) == x bit for bit)
+                bool moves = false;
+#pragma unroll
+                for (int h = 0; h < SL; ++h)
+                    if (lane + h * WAVE < K && (d.v[h] != 0.0f || __float_as_uint(x.v[h]) == 0x80000000u))
+                        moves = true;
+                fz_probe = __ballot(moves) == 0ull;
             }
             ls = 0;
             ai = 1.0f;
@@ -735,6 +769,7 @@ const FitVariant kB4[] = {{launch_variant<128, 2, 1, true, 2, 1>, 1, 1}};
 const FitVariant *const kVariants[NBUCKETS] = {kB0, kB1, kB2, kB3, kB4};
 constexpr int kNumVariants[NBUCKETS] = {4, 3, 3, 2, 1};
 int g_variant[NBUCKETS] = {1, 2, 1, 1, 0};  // fastest on the bench workload (autotune.tune_fit)
+int g_stationary_exit = 1;
 
 // The selected variant, or the bucket's first one when the selected one
 // needs more LDS than a workgroup may have at this history size.
@@ -762,6 +797,11 @@ int64_t sr_lbfgs_workspace_bytes(int64_t B, int64_t M) { return fit_ws(B, M); }
 int sr_fit_num_buckets(void) { return NBUCKETS; }
 int sr_fit_num_variants(int bucket) { return (bucket < 0 || bucket >= NBUCKETS) ? -1 : kNumVariants[bucket]; }
 int sr_fit_get_variant(int bucket) { return (bucket < 0 || bucket >= NBUCKETS) ? -1 : g_variant[bucket]; }
+int sr_fit_set_stationary_exit(int on) {
+    const int prev = g_stationary_exit;
+    g_stationary_exit = on ? 1 : 0;
+    return prev;
+}
 int sr_fit_set_variant(int bucket, int variant) {
     if (bucket < 0 || bucket >= NBUCKETS || variant < 0 || variant >= kNumVariants[bucket]) return -1;
     g_variant[bucket] = variant;
@@ -803,6 +843,7 @@ int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const i
     a.flags = flags;
     a.max_iter = max_iter; a.history = history; a.tol_grad = tol_grad; a.line_search = line_search;
     a.ls_max_iter = ls_max_iter; a.eps = eps;
+    a.stationary_exit = g_stationary_exit;
     a.gtape = M > 64 ? (float *)(ws + fit_gt_off(B)) : nullptr;
     auto bucket_args = [&](int k) {
         FitArgs r = a;

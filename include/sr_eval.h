@@ -131,6 +131,16 @@ int sr_fit_get_variant(int bucket);
 int sr_fit_set_variant(int bucket, int variant);
 int sr_fit_variant_info(int bucket, int variant, int *waves, int *ppt);
 
+/* Stationary exit of sr_lbfgs_fit (host-only, process-global, on by
+ * default; returns the previous setting).  When every stored history pair
+ * of a problem is exactly zero, its two-loop direction is zero; if the
+ * objective at x then reproduces (f, g) bit for bit, each remaining
+ * iteration of LBFGS_batch.step (symbolic_torch/lbfgs_batch.py:266-326)
+ * repeats the same accepted zero step, so the problem ends there with the
+ * results of running them: same C, loss and iteration count (max_iter).
+ * Only the closure count differs.  Off runs every iteration. */
+int sr_fit_set_stationary_exit(int on);
+
 #ifdef __cplusplus
 }
 #endif

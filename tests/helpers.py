@@ -8,6 +8,26 @@ BINARY = [40, 41, 42, 43, 44]
 G2_UNARY = [10, 11, 12, 13, 15]          # sin cos exp log sqrt
 G2_BINARY = [40, 41, 42, 43]             # + - * /
 
+# Grammar G2 of the benchmark workloads (3 variables, sin/cos/exp/log/sqrt).
+G2 = """E -> E + T [0.3]
+E -> E - T [0.1]
+E -> T [0.6]
+T -> T * F [0.3]
+T -> T / F [0.1]
+T -> F [0.6]
+F -> R ( E ) [0.15]
+F -> V [0.85]
+R -> sin [0.2]
+R -> cos [0.2]
+R -> exp [0.2]
+R -> log [0.2]
+R -> sqrt [0.2]
+V -> X_0 [0.25]
+V -> X_1 [0.25]
+V -> X_2 [0.25]
+V -> C [0.25]
+"""
+
 
 def _gen_tree(rng, depth, n_x, unary, binary, p_leaf, leaf_const_p, hard_const_p, out):
     if depth == 0 or rng.random() < p_leaf:

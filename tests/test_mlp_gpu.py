@@ -15,28 +15,11 @@ import numpy as np
 import pytest
 import torch
 
+import helpers
 import mlp_oracle as mo
 
 pytestmark = pytest.mark.gpu
 
-G2 = """E -> E + T [0.3]
-E -> E - T [0.1]
-E -> T [0.6]
-T -> T * F [0.3]
-T -> T / F [0.1]
-T -> F [0.6]
-F -> R ( E ) [0.15]
-F -> V [0.85]
-R -> sin [0.2]
-R -> cos [0.2]
-R -> exp [0.2]
-R -> log [0.2]
-R -> sqrt [0.2]
-V -> X_0 [0.25]
-V -> X_1 [0.25]
-V -> X_2 [0.25]
-V -> C [0.25]
-"""
 
 
 def _dev():
@@ -162,7 +145,7 @@ def _problems(P, seed, n_pts=512):
     import cudasymbolicregression_amd as st
     from cudasymbolicregression_amd import warmstart
 
-    g = st.ProbabilisticContextFreeGrammar(G2, "E", 64, 3, torch.device("cpu"))
+    g = st.ProbabilisticContextFreeGrammar(helpers.G2, "E", 64, 3, torch.device("cpu"))
     gen = torch.Generator().manual_seed(0)
     X = (torch.rand(n_pts, 3, generator=gen) * 6 - 3).to(_dev())
     ops, ch, Ct, Y = warmstart.sample_problems(g, X, P, seed)

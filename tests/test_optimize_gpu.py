@@ -181,3 +181,41 @@ def test_optimize_length_buckets_match_oracle():
     assert frac >= 0.9
     # the 64-constant sum: every constant gets the same gradient
     assert close[-1]
+
+
+@pytest.mark.parametrize("line_search", ["strong_wolfe", "backtracking"])
+def test_stationary_exit_matches_running_every_iteration(line_search):
+    """Problems whose history is all zero pairs sit at an exact fixed point of
+    the iteration; ending them there must give the same constants, loss,
+    iteration count and error code, bit for bit, as running all max_iter
+    iterations (badly scaled G2 problems started near their optimum stall
+    like that under the reference's unscaled first step)."""
+    import cudasymbolicregression_amd as st
+    from cudasymbolicregression_amd import _lib as L
+    from cudasymbolicregression_amd import warmstart
+    from cudasymbolicregression_amd.optimization import _fit
+
+    g = st.ProbabilisticContextFreeGrammar(helpers.G2, "E", 64, 3, torch.device("cpu"))
+    gen = torch.Generator().manual_seed(0)
+    X = (torch.rand(512, 3, generator=gen) * 6 - 3).to(_dev())
+    ops, ch, Ct, Y = warmstart.sample_problems(g, X, 4096, seed=21)
+    torch.manual_seed(22)
+    C0 = (Ct + 0.01 * torch.randn_like(Ct) * (ops == 1)).contiguous()
+    lib = L.hip()
+    res = {}
+    try:
+        for on in (0, 1):
+            lib.sr_fit_set_stationary_exit(on)
+            res[on] = _fit(X, Y, ops, ch, C0, 100, 10, 1e-7, line_search)
+            torch.cuda.synchronize()
+    finally:
+        lib.sr_fit_set_stationary_exit(1)
+    (C_a, l_a, it_a, e_a, f_a, n_a), (C_b, l_b, it_b, e_b, f_b, n_b) = res[0], res[1]
+    assert torch.equal(C_a.view(torch.int32), C_b.view(torch.int32))
+    assert torch.equal(l_a.view(torch.int64), l_b.view(torch.int64))
+    assert torch.equal(it_a, it_b) and torch.equal(e_a, e_b) and torch.equal(f_a, f_b)
+    saved = int(n_a.sum() - n_b.sum())
+    ended = int((n_b < n_a).sum())
+    print(f"{line_search}: {ended} problems ended early, {saved} of {int(n_a.sum())} closures saved", flush=True)
+    assert (n_b <= n_a).all()
+    assert ended > 0
