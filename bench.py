@@ -1,24 +1,31 @@
 #!/usr/bin/env python3
-"""Headline benchmark: expressions fitted per second on one MI355X.
+"""Headline benchmark: expressions fitted per second on one MI355X, learned
+warm start versus plain (randn) starts.
 
 Workload (the warm-start test set): P problems, each a G2 expression (postfix
 length <= 64, at least one learnable constant) with its own noiseless target
 Y = f(X; C*) on 512 shared points X ~ U(-3, 3) (n_x = 3), true constants
-C* ~ N(0, 2^2).  A step fits every problem's constants from a start point with
-the device-resident batched L-BFGS (max_iter 100, history 10, strong Wolfe,
-tol_grad 1e-7).  A problem counts as fitted when its relative MSE
-sum_n (f(X; C) - Y)^2 / sum_n (Y - mean Y)^2 <= 1e-6, with f evaluated in
+C* ~ N(0, 2^2).  A step fits every problem's constants with the
+device-resident batched L-BFGS (max_iter 100, history 10, strong Wolfe,
+tol_grad 1e-7), each problem stopping once its loss is <= 5e-7 x Var(Y)
+(the time to rel-MSE <= 1e-6).  A problem counts as fitted when its relative
+MSE sum_n (f(X; C) - Y)^2 / sum_n (Y - mean Y)^2 <= 1e-6, with f evaluated in
 FP64 at the fitted FP32 constants (the FP64 evaluator on the GPU).
 
-value = fitted problems per second of the timed fit (inputs resident in HBM
-before the timed region starts; the FP64 judging runs after it).  Start
-points: the learned warm start (MLP, with randn fallback) when --mode warm,
-randn otherwise; both are reported when the warm-start model is available.
+Two methods, timed alike (inputs resident in HBM, FP64 judging after the
+timed region):
+  plain: L-BFGS from randn starts;
+  warm:  the MLP's predicted constants (features + network on MFMA), the
+         loss at both starts, the per-problem choice of the lower one, then
+         the same L-BFGS.
+value = fitted problems per second of the warm-start step.  The network is
+trained beforehand (untimed, reported separately) on (Ops, Y, fitted C)
+tuples labelled by the plain L-BFGS, from a disjoint sample of problems.
 
 Also reported: the roofline of the fitting kernel (algorithmic FP32 slots per
-closure x closures / kernel time, against the FP32 peak) and a CPU baseline
-(the oracle's restatement of the reference's CPU algorithm on a bounded
-sample, single core).
+closure x closures / kernel time, against the FP32 peak), the C2 evaluate
+measurement, and a CPU baseline (the oracle's restatement of the
+reference's CPU algorithm on a bounded sample, single core).
 """
 from __future__ import annotations
 
@@ -78,61 +85,6 @@ def expression_cost(ops_row) -> int:
     return sum(sum(OPCOST.get(int(o), (0, 1 if int(o) >= 50 else 0))) for o in ops_row if int(o) != 0)
 
 
-def build_workload(P: int, seed: int, dev):
-    """P fitting problems (see module docstring), resident on `dev`."""
-    import torch
-
-    import cudasymbolicregression_amd as st
-    from cudasymbolicregression_amd import fused
-
-    g = st.ProbabilisticContextFreeGrammar(G2, "E", 64, 3, torch.device("cpu"))
-    gen = torch.Generator().manual_seed(seed)
-    X = (torch.rand(512, 3, generator=gen) * 6 - 3).to(dev)
-    ops_l, ch_l, C_l, Y_l = [], [], [], []
-    have = 0
-    torch.manual_seed(seed)
-    while have < P:
-        ops, ch = g.sample(2 * P)
-        keep = (ops == 1).any(dim=1)
-        ops, ch = ops[keep], ch[keep]
-        Ct = torch.zeros(ops.shape, dtype=torch.float32)
-        m = ops == 1
-        Ct[m] = torch.randn(int(m.sum()), generator=gen) * 2.0
-        ops_d, ch_d, Ct_d = ops.to(dev), ch.to(dev), Ct.to(dev)
-        y64 = fused.fused_root(X.double(), ops_d, ch_d, Ct_d.double(), None, fwd_only=True, raise_errors=False)["y"]
-        y = y64.float()
-        ok = torch.isfinite(y).all(dim=1) & (y.abs().amax(dim=1) < 1e6) & (y.double().var(dim=1) > 1e-8)
-        ops_l.append(ops_d[ok])
-        ch_l.append(ch_d[ok])
-        C_l.append(Ct_d[ok])
-        Y_l.append(y[ok])
-        have += int(ok.sum())
-    ops = torch.cat(ops_l)[:P].contiguous()
-    ch = torch.cat(ch_l)[:P].contiguous()
-    Ctrue = torch.cat(C_l)[:P].contiguous()
-    Y = torch.cat(Y_l)[:P].contiguous()
-    gen_d = torch.Generator(device=dev).manual_seed(seed + 1)
-    C0 = torch.zeros_like(Ctrue)
-    m = ops == 1
-    C0[m] = torch.randn(int(m.sum()), generator=gen_d, device=dev)
-    return dict(X=X, ops=ops, ch=ch, Ctrue=Ctrue, Y=Y, C0=C0, grammar=g)
-
-
-def rel_mse_fp64(w, C):
-    """Relative MSE of each problem, evaluated in FP64 at the FP32 constants."""
-    import torch
-
-    from cudasymbolicregression_amd import fused
-
-    y = fused.fused_root(w["X"].double(), w["ops"], w["ch"], C.double(), None, fwd_only=True, raise_errors=False)["y"]
-    Y = w["Y"].double()
-    num = ((y - Y) ** 2).sum(dim=1)
-    den = ((Y - Y.mean(dim=1, keepdim=True)) ** 2).sum(dim=1)
-    den = torch.where(den > 0, den, (Y ** 2).sum(dim=1))
-    r = num / den
-    return torch.where(torch.isfinite(r), r, torch.full_like(r, float("inf")))
-
-
 def bench_evaluate(dev, steps: int, seed: int = 1, B: int = 65536, N: int = 1024):
     """Plain evaluate forward+backward at scale: B G2 expressions (padded 64,
     n_x = 3), X ~ U(-3,3) (N,3), C ~ N(0,1) on the constant slots, upstream
@@ -178,15 +130,20 @@ def bench_evaluate(dev, steps: int, seed: int = 1, B: int = 65536, N: int = 1024
                 errors=int((r["err"] != 0).sum()))
 
 
-def fit(w, C_start, max_iter=100):
+def fit(w, C_start, stop=True):
     from cudasymbolicregression_amd.optimization import _fit
 
-    return _fit(w["X"], w["Y"], w["ops"], w["ch"], C_start, max_iter, 10, 1e-7, "strong_wolfe")
+    return _fit(w["X"], w["Y"], w["ops"], w["ch"], C_start, 100, 10, 1e-7, "strong_wolfe",
+                stop_loss=w["stop"] if stop else None)
 
 
 def cpu_baseline(w, sample: int):
-    """The oracle's CPU restatement of the reference algorithm on `sample` problems."""
+    """The oracle's CPU restatement of the reference algorithm (randn starts,
+    every iteration, no early stop) on `sample` problems."""
     import numpy as np
+    import torch
+
+    from cudasymbolicregression_amd import warmstart
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import lbfgs_oracle
@@ -200,15 +157,30 @@ def cpu_baseline(w, sample: int):
     with np.errstate(all="ignore"):
         C, _, _, calls = lbfgs_oracle.optimize(X, Y, ops, ch, C0, max_iter=100)
     dt = time.perf_counter() - t0
-    import torch
-
-    r = rel_mse_fp64({"X": w["X"], "ops": w["ops"][:sample], "ch": w["ch"][:sample], "Y": w["Y"][:sample]},
-                     torch.from_numpy(C).to(w["X"].device))
+    r = warmstart.rel_mse_fp64(w["X"], w["Y"][:sample], w["ops"][:sample], w["ch"][:sample],
+                               torch.from_numpy(C).to(w["X"].device))
     fitted = int((r <= 1e-6).sum())
     return dict(value=fitted / dt, unit="expressions fitted/s", cores=1, kind="port",
-                sample=f"{sample} problems of the same workload, full fits (max_iter 100), {dt:.1f} s, "
+                sample=f"{sample} problems of the same workload, randn starts, full fits (max_iter 100), {dt:.1f} s, "
                        f"{fitted} fitted, {calls} batched closures; oracle/lbfgs_oracle.py + oracle/eval_oracle.c, "
                        f"single thread")
+
+
+def timed(fn, steps: int, warmup: int):
+    """ms per call of fn() between HIP events on the current stream (every
+    kernel of a call is joined back to it), after `warmup` untimed calls."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        out = fn()
+    end.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / steps, out
 
 
 def main() -> int:
@@ -217,7 +189,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--problems", type=int, default=16384)
-    ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--seed", type=int, default=5, help="test problems (training uses seed - 1)")
+    ap.add_argument("--train-tuples", type=int, default=2_000_000)
+    ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-evaluate", action="store_true")
@@ -240,40 +214,62 @@ def main() -> int:
 
     import torch
 
+    import cudasymbolicregression_amd as st
+    from cudasymbolicregression_amd import warmstart
+
     dev = torch.device("cuda:0")
     t0 = time.time()
-    w = build_workload(args.problems, args.seed, dev)
+    g = st.ProbabilisticContextFreeGrammar(G2, "E", 64, 3, torch.device("cpu"))
+    gen = torch.Generator().manual_seed(0)
+    X = (torch.rand(512, 3, generator=gen) * 6 - 3).to(dev)  # the seed-fixed points of every problem
+    ops, ch, Ctrue, Y = warmstart.sample_problems(g, X, args.problems, args.seed)
+    var = Y.double().var(dim=1, unbiased=False)
+    w = dict(X=X, ops=ops, ch=ch, Ctrue=Ctrue, Y=Y, C0=warmstart.random_start(ops, args.seed + 1),
+             stop=(5e-7 * var).contiguous())
     torch.cuda.synchronize()
-    log(f"workload ready: {args.problems} problems in {time.time() - t0:.1f} s")
+    log(f"test set ready: {args.problems} problems in {time.time() - t0:.1f} s")
 
-    # untimed warmup
-    for _ in range(args.warmup):
-        fit(w, w["C0"])
+    # ---- training (untimed): labels from the plain L-BFGS on a disjoint sample
+    t0 = time.time()
+    tops, tch, tY, label, tstats = warmstart.make_training_set(g, X, args.train_tuples, args.seed - 1)
+    model = warmstart.WarmStartMLP(dev, seed=0)
     torch.cuda.synchronize()
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record()
-    for _ in range(args.steps):
-        C_fit, loss, iters, err, flags, nclos = fit(w, w["C0"])
-    end.record()
+    t1 = time.time()
+    hist = warmstart.train(model, tops, tY, label, epochs=args.epochs, batch=4096, lr=1e-3, lr_final=1e-5)
     torch.cuda.synchronize()
-    ms = start.elapsed_time(end) / args.steps
-    log(f"plain fit: {ms:.2f} ms/step")
+    tstats.update(train_s=round(time.time() - t1, 2), epochs=args.epochs, batch=4096,
+                  loss_first_epoch=round(hist[0], 4), loss_last_epoch=round(hist[-1], 4),
+                  zero_predictor_loss=round(float((label.double() ** 2).sum()) / max(1, label.shape[0]), 4))
+    log(f"training: {tstats} ({time.time() - t0:.1f} s)")
+    del tops, tch, tY, label
 
-    r = rel_mse_fp64(w, C_fit)
-    fitted = int((r <= 1e-6).sum())
-    value = fitted / (ms / 1000.0)
-    ok_err = int((err == 0).sum())
+    # ---- plain: randn starts
+    ms_p, (C_p, loss_p, iters_p, err_p, _, ncl_p) = timed(lambda: fit(w, w["C0"]), args.steps, args.warmup)
+    fit_p = int((warmstart.rel_mse_fp64(X, Y, ops, ch, C_p) <= 1e-6).sum())
+    log(f"plain: {ms_p:.2f} ms/step, {fit_p} fitted")
 
-    # roofline of the fitting kernels: algorithmic slots / kernel time
-    ops_cpu = w["ops"].cpu()
-    per_point = torch.tensor([expression_cost(row) + LOSS_SLOTS for row in ops_cpu.tolist()], dtype=torch.float64)
-    flops = float((per_point * nclos.cpu().double() * w["X"].shape[0]).sum())
-    achieved = flops / (ms / 1000.0) / 1e12
+    # ---- warm: network prediction, start selection, fit
+    def warm_step():
+        C_start, used, _, _ = warmstart.warm_start(model, X, Y, ops, ch, w["C0"])
+        return fit(w, C_start) + (used,)
+
+    ms_w, (C_w, loss_w, iters_w, err_w, _, ncl_w, used) = timed(warm_step, args.steps, args.warmup)
+    fit_w = int((warmstart.rel_mse_fp64(X, Y, ops, ch, C_w) <= 1e-6).sum())
+    ms_ws, _ = timed(lambda: warmstart.warm_start(model, X, Y, ops, ch, w["C0"]), args.steps, 1)
+    log(f"warm: {ms_w:.2f} ms/step ({ms_ws:.2f} ms start selection), {fit_w} fitted")
+    value = fit_w / (ms_w / 1000.0)
+    plain_value = fit_p / (ms_p / 1000.0)
+
+    # roofline of the fitting kernels (warm step): algorithmic slots / fit time
+    fit_ms = ms_w - ms_ws
+    per_point = torch.tensor([expression_cost(row) + LOSS_SLOTS for row in ops.cpu().tolist()], dtype=torch.float64)
+    flops = float((per_point * ncl_w.cpu().double() * X.shape[0]).sum())
+    achieved = flops / (fit_ms / 1000.0) / 1e12
     roofline = dict(bound="mfma", achieved=round(achieved, 4), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=round(achieved / FP32_PEAK_TFLOPS, 5), traffic=None,
                     traffic_profiled_bytes=FIT_TRAFFIC_PROFILED, traffic_source="profiles/fit_hbm_traffic.md",
                     kernel="sr_lbfgs_fit: the k_fit bucket kernels of one call (concurrent)",
-                    flop_per_launch=flops, frac_unpacked_lane_peak=round(flops / (ms / 1e3) / VALU_LANE_PEAK, 5))
+                    flop_per_launch=flops, frac_unpacked_lane_peak=round(flops / (fit_ms / 1e3) / VALU_LANE_PEAK, 5))
 
     ev = None
     if not args.no_evaluate:
@@ -292,7 +288,7 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 3),
+        "ms_per_step": round(ms_w, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -300,12 +296,18 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": "warm-start test set: G2 expressions (len<=64, >=1 constant), own noiseless targets on 512 "
-                        "shared points X~U(-3,3), n_x=3, C*~N(0,2^2); randn start",
+                        "seed-fixed points X~U(-3,3), n_x=3, C*~N(0,2^2); MLP warm start with randn fallback",
             "problems": args.problems, "n_points": 512, "n_x": 3, "max_iter": 100, "history": 10,
-            "line_search": "strong_wolfe", "tol_grad": 1e-7, "init": "randn",
-            "fitted": fitted, "fitted_fraction": round(fitted / args.problems, 4), "problems_without_error": ok_err,
-            "mean_iters": round(float(iters.double().mean()), 2),
-            "mean_closures": round(float(nclos.double().mean()), 2),
+            "line_search": "strong_wolfe", "tol_grad": 1e-7, "stop_rel_loss": 5e-7,
+            "warm": {"ms": round(ms_w, 3), "start_selection_ms": round(ms_ws, 3), "fitted": fit_w,
+                     "fitted_per_s": round(value, 1), "warm_start_taken": round(float(used.double().mean()), 4),
+                     "mean_iters": round(float(iters_w.double().mean()), 2),
+                     "closures": int(ncl_w.sum()), "problems_without_error": int((err_w == 0).sum())},
+            "plain": {"ms": round(ms_p, 3), "fitted": fit_p, "fitted_per_s": round(plain_value, 1),
+                      "mean_iters": round(float(iters_p.double().mean()), 2), "closures": int(ncl_p.sum()),
+                      "problems_without_error": int((err_p == 0).sum())},
+            "warm_over_plain": round(value / plain_value, 4) if plain_value > 0 else None,
+            "training": tstats,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
