@@ -116,3 +116,74 @@ def optimize(
     if return_info:
         return C_out, Y_final, dict(loss=loss, iters=iters, err=err, closures=nclos)
     return C_out, Y_final
+
+
+def optimize_ragged(
+    X: torch.Tensor,
+    Y: torch.Tensor,
+    n_pts: torch.Tensor,
+    Ops: torch.Tensor,
+    Ch: torch.Tensor,
+    C_init: torch.Tensor,
+    max_iter: int = 100,
+    history_size: int = 10,
+    tolerance_grad: float = 1e-7,
+    line_search_fn: Literal["strong_wolfe", "backtracking"] = "strong_wolfe",
+    *,
+    stop_rel_loss: Optional[float] = None,
+    raise_errors: bool = True,
+) -> Tuple[torch.Tensor, Dict[str, torch.Tensor]]:
+    """`optimize` for problems that each have their own dataset (no
+    counterpart in the reference, whose optimize shares one X across the
+    batch, symbolic_torch/optimization.py:9-112).
+
+    Args:
+        X: (sum_b N_b, n_x) float32, the points of problem 0, then problem 1, ...
+        Y: (sum_b N_b,) float32, the targets in the same order.
+        n_pts: (B,) integer N_b >= 1, the number of points of each problem.
+        Ops (B, M), Ch (B, M, 2), C_init (B, M) and the options as in optimize;
+        each problem's loss is the mean over its own N_b points.
+
+    Returns:
+        C_final (B, M) and info = dict(loss (B,) FP64, iters (B,), err (B,),
+        closures (B,)).  One launch fits the whole batch.
+    """
+    _require_gpu(X, "X")
+    dev = X.device
+    if X.dtype != torch.float32 or X.dim() != 2:
+        raise RuntimeError("X must be a (sum N_b, n_x) float32 tensor")
+    B, M = Ops.shape
+    n = n_pts.to(device=dev, dtype=torch.int64)
+    if n.shape != (B,):
+        raise RuntimeError("n_pts must have shape (B,)")
+    if B and int(n.min()) < 1:
+        raise RuntimeError("every problem needs at least one point")
+    total = int(n.sum()) if B else 0
+    if X.shape[0] != total or Y.numel() != total:
+        raise RuntimeError(f"X has {X.shape[0]} rows and Y {Y.numel()} targets; n_pts sums to {total}")
+    n_x = X.shape[1]
+    start = torch.cumsum(n, 0) - n
+    x_off = (start * n_x).contiguous()
+    y_off = start.contiguous()
+    n32 = n.to(torch.int32).contiguous()
+    Xc = X.contiguous()
+    Yc = Y.reshape(-1).to(torch.float32).contiguous()
+    stop = None
+    if stop_rel_loss is not None:
+        seg = torch.repeat_interleave(torch.arange(B, device=dev), n)
+        Yd = Yc.double()
+        mean = torch.zeros(B, dtype=torch.float64, device=dev).index_add_(0, seg, Yd) / n
+        var = torch.zeros(B, dtype=torch.float64, device=dev).index_add_(0, seg, (Yd - mean[seg]) ** 2) / n
+        msq = torch.zeros(B, dtype=torch.float64, device=dev).index_add_(0, seg, Yd * Yd) / n
+        stop = (float(stop_rel_loss) * torch.where(var > 0, var, msq)).contiguous()
+    C_out, loss, iters, err, flags, nclos = _fit(
+        Xc, Yc, Ops.to(device=dev, dtype=torch.int64).contiguous(), Ch.to(device=dev, dtype=torch.int64).contiguous(),
+        C_init.to(device=dev, dtype=torch.float32).contiguous(), max_iter, history_size, tolerance_grad,
+        line_search_fn, stop_loss=stop, x_off=x_off, y_off=y_off, n_pts=n32)
+    if raise_errors:
+        f = flags.tolist()
+        if f[0]:
+            raise RuntimeError(get_error_message(f[0], "input validation"))
+        if f[1]:
+            raise RuntimeError(get_error_message(f[1], "symbolic evaluation backward pass"))
+    return C_out, dict(loss=loss, iters=iters, err=err, closures=nclos)
