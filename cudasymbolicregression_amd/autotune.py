@@ -59,15 +59,17 @@ def tune_fit(X: torch.Tensor, Y: torch.Tensor, Ops: torch.Tensor, Ch: torch.Tens
              max_iter: int = 100, reps: int = 2, rounds: int = 1, verbose: bool = False,
              **kw) -> Dict[str, object]:
     """Coordinate descent over the buckets' variants, timing `optimize` on
-    the given batch; leaves the fastest configuration active and returns
-    {config, ms, trials}."""
+    the given batch (keyword options: history_size, tolerance_grad,
+    line_search_fn, stop_loss as in the fit); leaves the fastest
+    configuration active and returns {config, ms, trials}."""
     from .optimization import _fit
 
     Xc, Yc = X.contiguous(), Y.to(torch.float32).contiguous()
     Oc, Cc = Ops.to(torch.int64).contiguous(), Ch.to(torch.int64).contiguous()
     C0 = C_init.to(torch.float32).contiguous()
+    stop = kw.get("stop_loss")
     run = lambda: _fit(Xc, Yc, Oc, Cc, C0, max_iter, kw.get("history_size", 10), kw.get("tolerance_grad", 1e-7),
-                       kw.get("line_search_fn", "strong_wolfe"))
+                       kw.get("line_search_fn", "strong_wolfe"), stop_loss=stop)
     best = get_config()
     best_t = _time(run, reps)
     trials = [(list(best), best_t)]

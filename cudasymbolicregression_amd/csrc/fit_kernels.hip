@@ -755,20 +755,23 @@ struct FitVariant {
 const FitVariant kB0[] = {{launch_variant<8, 1, 1, false, 3, 2>, 1, 2},
                           {launch_variant<8, 1, 2, false, 3, 2>, 2, 2},
                           {launch_variant<8, 1, 4, false, 3, 2>, 4, 2},
-                          {launch_variant<8, 1, 1, false, 2, 4>, 1, 4}};
+                          {launch_variant<8, 1, 1, false, 2, 4>, 1, 4},
+                          {launch_variant<8, 1, 8, false, 2, 1>, 8, 1}};
 const FitVariant kB1[] = {{launch_variant<16, 1, 1, false, 3, 2>, 1, 2},
                           {launch_variant<16, 1, 2, false, 3, 2>, 2, 2},
-                          {launch_variant<16, 1, 4, false, 3, 2>, 4, 2}};
+                          {launch_variant<16, 1, 4, false, 3, 2>, 4, 2},
+                          {launch_variant<16, 1, 8, false, 2, 1>, 8, 1}};
 const FitVariant kB2[] = {{launch_variant<32, 1, 2, false, 3, 1>, 2, 1},
                           {launch_variant<32, 1, 4, false, 3, 1>, 4, 1},
-                          {launch_variant<32, 1, 4, false, 3, 2>, 4, 2}};
+                          {launch_variant<32, 1, 4, false, 3, 2>, 4, 2},
+                          {launch_variant<32, 1, 8, false, 2, 1>, 8, 1}};
 const FitVariant kB3[] = {{launch_variant<64, 1, 4, false, 2, 1>, 4, 1},
                           {launch_variant<64, 1, 2, false, 2, 1>, 2, 1}};
 const FitVariant kB4[] = {{launch_variant<128, 2, 1, true, 2, 1>, 1, 1}};
 
 const FitVariant *const kVariants[NBUCKETS] = {kB0, kB1, kB2, kB3, kB4};
-constexpr int kNumVariants[NBUCKETS] = {4, 3, 3, 2, 1};
-int g_variant[NBUCKETS] = {1, 2, 1, 1, 0};  // fastest on the bench workload (autotune.tune_fit)
+constexpr int kNumVariants[NBUCKETS] = {5, 4, 4, 2, 1};
+int g_variant[NBUCKETS] = {2, 2, 2, 0, 0};  // fastest on the bench workload (autotune.tune_fit)
 int g_stationary_exit = 1;
 
 // The selected variant, or the bucket's first one when the selected one
