@@ -44,12 +44,14 @@ __device__ inline float gelu_grad(float z) {
 
 enum Epi : int { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2 };
 
-// C = A * Bt^T (+ bias), M and N multiples of 32, K a multiple of 16.
+// C = A * Bt^T (+ bias), M and N multiples of 32, K a multiple of 16; A and
+// Bt rows start lda / ldb elements apart (multiples of 8).
 // MFMA 32x32x16 lane layout (per wave): lane l holds A[i = l % 32][k0 + 8*(l/32) + e]
 // and Bt[j = l % 32][k0 + 8*(l/32) + e] for e = 0..7; accumulator register r of
 // lane l is C[8*(r/4) + 4*(l/32) + r%4][l % 32].
 __global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t *__restrict__ A, const uint16_t *__restrict__ Bt,
-                                                const float *__restrict__ bias, int M, int N, int K, int epi,
+                                                const float *__restrict__ bias, int M, int N, int K, int lda, int ldb,
+                                                int epi,
                                                 float *__restrict__ out_f32, uint16_t *__restrict__ out_bf16,
                                                 float *__restrict__ pre_f32) {
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
@@ -57,8 +59,8 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t *__restrict__ A,
     const int j0 = blockIdx.x * 64 + (wave % 2) * 32;
     if (i0 >= M || j0 >= N) return;  // whole wave leaves together
     const int r = lane % 32, kh = (lane / 32) * 8;
-    const uint16_t *pa = A + (size_t)(i0 + r) * K + kh;
-    const uint16_t *pb = Bt + (size_t)(j0 + r) * K + kh;
+    const uint16_t *pa = A + (size_t)(i0 + r) * lda + kh;
+    const uint16_t *pb = Bt + (size_t)(j0 + r) * ldb + kh;
     f32x16 acc;
 #pragma unroll
     for (int q = 0; q < 16; ++q) acc[q] = 0.0f;
@@ -83,6 +85,39 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t *__restrict__ A,
             if (out_f32) out_f32[o] = z;
             if (out_bf16) out_bf16[o] = f2bf(z);
         }
+    }
+}
+
+// First layer with a one-hot block in its input: z = zy + b + sum_k WT[k V +
+// Ops[b][k]] (rows of the transposed BF16 weights for slots k < slots and
+// opcodes in [0, V)), i.e. the product of the one-hot columns with the
+// weights without multiplying the zeros; then Z = z and H = gelu(z) (BF16).
+// zy holds the product of the other input columns.  One block per row;
+// rows b >= B (batch padding) take no one-hot term.
+__global__ __launch_bounds__(256) void k_onehot_gelu(const float *__restrict__ zy, const uint16_t *__restrict__ WT,
+                                                    const float *__restrict__ bias, const int64_t *__restrict__ ops,
+                                                    int M, int64_t B, int slots, int vocab, int H,
+                                                    float *__restrict__ pre, uint16_t *__restrict__ h_out) {
+    __shared__ int rows[64];
+    const int64_t b = blockIdx.x;
+    if (threadIdx.x < 64) {
+        const int k = threadIdx.x;
+        int r = -1;
+        if (b < B && k < slots && k < M) {
+            const int64_t c = ops[b * M + k];
+            if (c >= 0 && c < vocab) r = k * vocab + (int)c;
+        }
+        rows[k] = r;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < H; j += blockDim.x) {
+        float z = zy[(size_t)b * H + j] + bias[j];
+        for (int k = 0; k < 64; ++k) {
+            const int r = rows[k];
+            if (r >= 0) z += bf2f(WT[(size_t)r * H + j]);
+        }
+        pre[(size_t)b * H + j] = z;
+        h_out[(size_t)b * H + j] = f2bf(gelu(z));
     }
 }
 
@@ -231,12 +266,22 @@ inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 extern "C" {
 
-int sr_mlp_gemm_nt(const void *A, const void *Bt, const float *bias, int64_t M, int64_t N, int64_t K, int epi,
-                   float *out_f32, void *out_bf16, float *pre_f32, void *stream) {
+int sr_mlp_gemm_nt(const void *A, const void *Bt, const float *bias, int64_t M, int64_t N, int64_t K, int64_t lda,
+                   int64_t ldb, int epi, float *out_f32, void *out_bf16, float *pre_f32, void *stream) {
     if (M <= 0 || N <= 0 || K <= 0 || M % 32 || N % 32 || K % 16 || epi < 0 || epi > 2) return -1;
+    if (lda < K || ldb < K || lda % 8 || ldb % 8 || M > INT32_MAX / 2 || N > INT32_MAX / 2) return -1;
     hipLaunchKernelGGL(k_gemm_nt, dim3(blocks(N, 64), blocks(M, 64)), dim3(256), 0, (hipStream_t)stream,
-                       (const uint16_t *)A, (const uint16_t *)Bt, bias, (int)M, (int)N, (int)K, epi, out_f32,
-                       (uint16_t *)out_bf16, pre_f32);
+                       (const uint16_t *)A, (const uint16_t *)Bt, bias, (int)M, (int)N, (int)K, (int)lda, (int)ldb,
+                       epi, out_f32, (uint16_t *)out_bf16, pre_f32);
+    return last();
+}
+
+int sr_mlp_onehot_gelu(const float *zy, const void *WT, const float *bias, const int64_t *ops, int64_t M, int64_t B,
+                       int64_t B_pad, int slots, int vocab, int64_t H, float *pre, void *h_out, void *stream) {
+    if (M <= 0 || B < 0 || B_pad < B || slots < 0 || slots > 64 || vocab <= 0 || H <= 0) return -1;
+    if (B_pad) hipLaunchKernelGGL(k_onehot_gelu, dim3((unsigned)B_pad), dim3(256), 0, (hipStream_t)stream, zy,
+                                  (const uint16_t *)WT, bias, ops, (int)M, B, slots, vocab, (int)H, pre,
+                                  (uint16_t *)h_out);
     return last();
 }
 

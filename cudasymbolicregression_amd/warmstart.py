@@ -14,10 +14,15 @@ with V = VAR_START_ID + n_x opcodes (53 for n_x = 3, the reference's
 `n_operators`, src/pcfg.cpp:228).  Weights and activations are BF16 on the
 MFMA units with FP32 accumulation; master weights and Adam moments are FP32.
 
+The first layer never multiplies the one-hot zeros: the MFMA GEMM covers the
+target columns only (K = 256), and the one-hot block adds, per row, the 64
+weight rows it selects (from a transposed BF16 copy of W_1).  The weight
+gradient of the first layer is still the dense dZ_1^T x product.
+
 Every kernel is hand-written HIP in csrc/mlp_kernels.hip (GEMM on
-v_mfma_f32_32x32x16_bf16, transposes, GELU backward, column sums, masked MSE
-head, Adam, feature build, prediction mask and start selection); torch only
-allocates device memory and provides the stream.
+v_mfma_f32_32x32x16_bf16, one-hot gather + GELU, transposes, GELU backward,
+column sums, masked MSE head, Adam, feature build, prediction mask and start
+selection); torch only allocates device memory and provides the stream.
 """
 from __future__ import annotations
 
@@ -64,6 +69,10 @@ class WarmStartMLP:
             self.W.append(((torch.rand(self.dims[i + 1], self.dims[i], generator=gen) * 2 - 1) * bound).to(dev))
             self.b.append(((torch.rand(self.dims[i + 1], generator=gen) * 2 - 1) * bound).to(dev))
         self.Wb = [torch.empty(w.shape, dtype=torch.bfloat16, device=dev) for w in self.W]
+        # first layer's BF16 weights transposed (one row per input column): the
+        # one-hot block of the input is applied as a sum of gathered rows
+        self.WT0 = torch.empty(self.dims[0], self.dims[1], dtype=torch.bfloat16, device=dev)
+        self.onehot = self.slots * self.vocab
         self.mW = [torch.zeros_like(w) for w in self.W]
         self.vW = [torch.zeros_like(w) for w in self.W]
         self.mb = [torch.zeros_like(b) for b in self.b]
@@ -87,6 +96,11 @@ class WarmStartMLP:
         lib = _lib.hip()
         for w, wb in zip(self.W, self.Wb):
             _ck(lib.sr_mlp_to_bf16(_lib.ptr(w), w.numel(), _lib.ptr(wb), self._stream()), "sr_mlp_to_bf16")
+        self._refresh_wt0()
+
+    def _refresh_wt0(self) -> None:
+        _ck(_lib.hip().sr_mlp_transpose(_lib.ptr(self.Wb[0]), _lib.ptr(self.WT0), self.dims[1], self.dims[0],
+                                        self._stream()), "sr_mlp_transpose")
 
     def _buffers(self, Bp: int, train: bool) -> Dict[str, object]:
         key = Bp * 2 + int(train)
@@ -145,17 +159,27 @@ class WarmStartMLP:
         lib, s = _lib.hip(), self._stream()
         _ck(lib.sr_mlp_features(_lib.ptr(ops), M, B, Bp, self.slots, self.vocab, _lib.ptr(Y), Y.shape[1],
                                 self.n_pts, self.row_len, _lib.ptr(buf["x"]), s), "sr_mlp_features")
-        h = buf["x"]
-        for l in range(self.layers):
+        # layer 1: MFMA product over the target columns, then the one-hot
+        # columns as gathered weight rows, bias and GELU
+        x, H1, oh = buf["x"], self.dims[1], self.onehot
+        ky = self.row_len - oh
+        _ck(lib.sr_mlp_gemm_nt(x.data_ptr() + 2 * oh, self.Wb[0].data_ptr() + 2 * oh, None, Bp, H1, ky,
+                               self.row_len, self.row_len, EPI_STORE, _lib.ptr(buf["z"][0]), None, None, s),
+            "sr_mlp_gemm_nt")
+        _ck(lib.sr_mlp_onehot_gelu(_lib.ptr(buf["z"][0]), _lib.ptr(self.WT0), _lib.ptr(self.b[0]), _lib.ptr(ops), M, B,
+                                   Bp, self.slots, self.vocab, H1, _lib.ptr(buf["z"][0]), _lib.ptr(buf["h"][0]), s),
+            "sr_mlp_onehot_gelu")
+        h = buf["h"][0]
+        for l in range(1, self.layers):
             n_in, n_out = self.dims[l], self.dims[l + 1]
             if l < self.layers - 1:
-                _ck(lib.sr_mlp_gemm_nt(_lib.ptr(h), _lib.ptr(self.Wb[l]), _lib.ptr(self.b[l]), Bp, n_out, n_in,
-                                       EPI_BIAS_GELU, None, _lib.ptr(buf["h"][l]), _lib.ptr(buf["z"][l]), s),
+                _ck(lib.sr_mlp_gemm_nt(_lib.ptr(h), _lib.ptr(self.Wb[l]), _lib.ptr(self.b[l]), Bp, n_out, n_in, n_in,
+                                       n_in, EPI_BIAS_GELU, None, _lib.ptr(buf["h"][l]), _lib.ptr(buf["z"][l]), s),
                     "sr_mlp_gemm_nt")
                 h = buf["h"][l]
             else:
-                _ck(lib.sr_mlp_gemm_nt(_lib.ptr(h), _lib.ptr(self.Wb[l]), _lib.ptr(self.b[l]), Bp, n_out, n_in,
-                                       EPI_BIAS, _lib.ptr(buf["out"]), None, None, s), "sr_mlp_gemm_nt")
+                _ck(lib.sr_mlp_gemm_nt(_lib.ptr(h), _lib.ptr(self.Wb[l]), _lib.ptr(self.b[l]), Bp, n_out, n_in, n_in,
+                                       n_in, EPI_BIAS, _lib.ptr(buf["out"]), None, None, s), "sr_mlp_gemm_nt")
         return B, M, Bp, buf
 
     def forward(self, ops: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
@@ -193,12 +217,12 @@ class WarmStartMLP:
                 "sr_mlp_col_sum")
             _ck(lib.sr_mlp_transpose(_lib.ptr(dz_b), _lib.ptr(buf["dzT"]), Bp, n_out, s), "sr_mlp_transpose")
             _ck(lib.sr_mlp_transpose(_lib.ptr(h), _lib.ptr(buf["hT"]), Bp, n_in, s), "sr_mlp_transpose")
-            _ck(lib.sr_mlp_gemm_nt(_lib.ptr(buf["dzT"]), _lib.ptr(buf["hT"]), None, n_out, n_in, Bp, EPI_STORE,
+            _ck(lib.sr_mlp_gemm_nt(_lib.ptr(buf["dzT"]), _lib.ptr(buf["hT"]), None, n_out, n_in, Bp, Bp, Bp, EPI_STORE,
                                    _lib.ptr(self.gW[l]), None, None, s), "sr_mlp_gemm_nt")
             if l == 0:
                 break
             _ck(lib.sr_mlp_transpose(_lib.ptr(self.Wb[l]), _lib.ptr(buf["WT"]), n_out, n_in, s), "sr_mlp_transpose")
-            _ck(lib.sr_mlp_gemm_nt(_lib.ptr(dz_b), _lib.ptr(buf["WT"]), None, Bp, n_in, n_out, EPI_STORE,
+            _ck(lib.sr_mlp_gemm_nt(_lib.ptr(dz_b), _lib.ptr(buf["WT"]), None, Bp, n_in, n_out, n_out, n_out, EPI_STORE,
                                    _lib.ptr(buf["dH"]), None, None, s), "sr_mlp_gemm_nt")
             nxt = l % 2
             _ck(lib.sr_mlp_gelu_bwd(_lib.ptr(buf["dH"]), _lib.ptr(buf["z"][l - 1]), Bp * n_in,
@@ -218,6 +242,7 @@ class WarmStartMLP:
             _ck(lib.sr_mlp_adam(_lib.ptr(self.b[l]), _lib.ptr(self.gb[l]), _lib.ptr(self.mb[l]),
                                 _lib.ptr(self.vb[l]), self.b[l].numel(), lr, betas[0], betas[1], eps, self.step,
                                 weight_decay, None, s), "sr_mlp_adam")
+        self._refresh_wt0()
 
     def train_step(self, ops, Y, label, lr: float, weight_decay: float = 0.0) -> torch.Tensor:
         rows = self.loss_and_grads(ops, Y, label)

@@ -27,13 +27,25 @@ extern "C" {
 #define SR_MLP_EPI_BIAS 1      /* C = A Bt^T + bias[col]                     */
 #define SR_MLP_EPI_BIAS_GELU 2 /* Z = A Bt^T + bias, C = gelu(Z) (exact erf) */
 
-/* C[M,N] = A[M,K] * Bt[N,K]^T with BF16 operands (row-major, K contiguous)
- * and FP32 accumulation on v_mfma_f32_32x32x16_bf16.  M and N multiples of
- * 32, K a multiple of 16, base pointers 16-byte aligned.  Writes whichever
- * of out_f32 (FP32), out_bf16 (BF16) are non-NULL; with EPI_BIAS_GELU,
- * pre_f32 (optional) receives Z.  bias may be NULL for EPI_STORE. */
-int sr_mlp_gemm_nt(const void *A, const void *Bt, const float *bias, int64_t M, int64_t N, int64_t K, int epi,
-                   float *out_f32, void *out_bf16, float *pre_f32, void *stream);
+/* C[M,N] = A[M,K] * Bt[N,K]^T with BF16 operands (row-major, K contiguous,
+ * rows lda / ldb elements apart) and FP32 accumulation on
+ * v_mfma_f32_32x32x16_bf16; C is dense (row stride N).  M and N multiples of
+ * 32, K a multiple of 16, lda and ldb multiples of 8 and >= K, base pointers
+ * 16-byte aligned.  Writes whichever of out_f32 (FP32), out_bf16 (BF16) are
+ * non-NULL; with EPI_BIAS_GELU, pre_f32 (optional) receives Z.  bias may be
+ * NULL for EPI_STORE. */
+int sr_mlp_gemm_nt(const void *A, const void *Bt, const float *bias, int64_t M, int64_t N, int64_t K, int64_t lda,
+                   int64_t ldb, int epi, float *out_f32, void *out_bf16, float *pre_f32, void *stream);
+
+/* First layer with a one-hot input block, without multiplying its zeros:
+ * z[b][j] = zy[b][j] + bias[j] + sum_k WT[k*vocab + Ops[b][k]][j] over
+ * slots k < min(slots, M) whose opcode is in [0, vocab) (WT: the BF16
+ * weights transposed, one row per input column); pre = z (FP32) and
+ * h_out = gelu(z) (BF16), both B_pad x H.  zy (B_pad x H, FP32, may alias
+ * pre) is the product of the remaining input columns; rows b >= B take no
+ * one-hot term. */
+int sr_mlp_onehot_gelu(const float *zy, const void *WT, const float *bias, const int64_t *ops, int64_t M, int64_t B,
+                       int64_t B_pad, int slots, int vocab, int64_t H, float *pre, void *h_out, void *stream);
 
 /* out[c][r] = in[r][c] for a rows x cols BF16 matrix. */
 int sr_mlp_transpose(const void *in, void *out, int64_t rows, int64_t cols, void *stream);

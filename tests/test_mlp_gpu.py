@@ -59,7 +59,7 @@ def test_gemm_nt_matches_fp64(M, N, K, epi):
     outb = torch.empty(M, N, dtype=torch.bfloat16, device=_dev())
     pre = torch.empty(M, N, dtype=torch.float32, device=_dev())
     print(f"gemm M={M} N={N} K={K} epi={epi}", flush=True)
-    rc = L.hip().sr_mlp_gemm_nt(L.ptr(Ad), L.ptr(Bd), L.ptr(bd), M, N, K, epi, L.ptr(out), L.ptr(outb),
+    rc = L.hip().sr_mlp_gemm_nt(L.ptr(Ad), L.ptr(Bd), L.ptr(bd), M, N, K, K, K, epi, L.ptr(out), L.ptr(outb),
                                 L.ptr(pre) if epi == 2 else None, _s())
     assert rc == 0
     torch.cuda.synchronize()
@@ -77,8 +77,26 @@ def test_gemm_nt_matches_fp64(M, N, K, epi):
 
 def test_gemm_rejects_unaligned_shapes():
     L = _lib()
-    for M, N, K in [(33, 32, 16), (32, 40, 16), (32, 32, 24), (0, 32, 16)]:
-        assert L.hip().sr_mlp_gemm_nt(None, None, None, M, N, K, 0, None, None, None, _s()) == -1
+    for M, N, K, lda in [(33, 32, 16, 16), (32, 40, 16, 16), (32, 32, 24, 24), (0, 32, 16, 16), (32, 32, 16, 12),
+                         (32, 32, 16, 8)]:
+        assert L.hip().sr_mlp_gemm_nt(None, None, None, M, N, K, lda, K, 0, None, None, None, _s()) == -1
+
+
+def test_gemm_strided_operands():
+    """A and Bt read as column blocks of wider matrices (row strides lda, ldb)."""
+    L = _lib()
+    rng = np.random.default_rng(17)
+    M, N, K, lda, ldb, off = 64, 96, 48, 200, 136, 24
+    A = mo.bf16_round(rng.standard_normal((M, lda)))
+    Bt = mo.bf16_round(rng.standard_normal((N, ldb)))
+    Ad, Bd = _bf16_dev(A), _bf16_dev(Bt)
+    out = torch.empty(M, N, dtype=torch.float32, device=_dev())
+    assert L.hip().sr_mlp_gemm_nt(Ad.data_ptr() + 2 * off, Bd.data_ptr() + 2 * off, None, M, N, K, lda, ldb, 0,
+                                  L.ptr(out), None, None, _s()) == 0
+    torch.cuda.synchronize()
+    a, b = A[:, off:off + K], Bt[:, off:off + K]
+    bound = K * 2.0 ** -24 * (np.abs(a) @ np.abs(b).T) + 1e-6
+    assert np.all(np.abs(_np(out) - a @ b.T) <= bound)
 
 
 @pytest.mark.parametrize("rows,cols", [(70, 33), (64, 3648), (4096, 64)])
