@@ -24,13 +24,15 @@ tuples labelled by the plain L-BFGS, from a disjoint sample of problems.
 
 Also reported: the roofline of the fitting kernel (algorithmic FP32 slots per
 closure x closures / kernel time, against the FP32 peak), the C2 evaluate
-measurement, and a CPU baseline (the oracle's restatement of the
+measurement, the uneven-batching (C4) fit of 32768 problems with their own
+datasets, and a CPU baseline (the oracle's restatement of the
 reference's CPU algorithm on a bounded sample, single core).
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -130,6 +132,71 @@ def bench_evaluate(dev, steps: int, seed: int = 1, B: int = 65536, N: int = 1024
                 errors=int((r["err"] != 0).sum()))
 
 
+def bench_ragged(dev, steps: int, seed: int = 3, B: int = 32768):
+    """Uneven batching: B problems, each its own G2 expression (postfix length
+    3..127, padded 128, >= 1 constant) with its own N_i = round(exp(U(ln 64,
+    ln 4096))) points X_i ~ U(-3, 3) (n_x = 3) and noiseless targets from true
+    constants ~ N(0, 2^2); one ragged fit launch per step from randn starts
+    (stop at loss <= 5e-7 Var(Y_i)), judged in FP64."""
+    import torch
+
+    import cudasymbolicregression_amd as st
+    from cudasymbolicregression_amd import fused, warmstart
+    from cudasymbolicregression_amd.optimization import _fit
+
+    g = st.ProbabilisticContextFreeGrammar(G2, "E", 128, 3, torch.device("cpu"))
+    torch.manual_seed(seed)
+    gen = torch.Generator().manual_seed(seed)
+    parts = []
+    have = 0
+    while have < B:  # candidate rounds until B problems have finite, non-constant targets
+        ops, ch = g.sample(65536)
+        L = (ops != 0).sum(1)
+        ok = (L >= 3) & (L <= 127) & (ops == 1).any(1)
+        ops, ch = ops[ok], ch[ok]
+        P = ops.shape[0]
+        n = torch.exp(torch.empty(P).uniform_(math.log(64), math.log(4096), generator=gen)).round().to(torch.int64)
+        X = torch.rand(int(n.sum()), 3, generator=gen) * 6 - 3
+        Ct = torch.zeros(ops.shape)
+        m = ops == 1
+        Ct[m] = torch.randn(int(m.sum()), generator=gen) * 2.0
+        ops, ch, Ct, n, X = ops.to(dev), ch.to(dev), Ct.to(dev), n.to(dev), X.to(dev)
+        y = fused.fused_root_ragged(X.double(), n, ops, ch, Ct.double(), None, fwd_only=True, raise_errors=False)["y"]
+        seg = torch.repeat_interleave(torch.arange(P, device=dev), n)
+        bad = torch.zeros(P, dtype=torch.float64, device=dev).index_add_(
+            0, seg, (~torch.isfinite(y) | (y.abs() >= 1e6)).double())
+        yz = torch.nan_to_num(y)
+        mean = torch.zeros(P, dtype=torch.float64, device=dev).index_add_(0, seg, yz) / n
+        var = torch.zeros(P, dtype=torch.float64, device=dev).index_add_(0, seg, (yz - mean[seg]) ** 2) / n
+        sel = ((bad == 0) & (var > 1e-8)).nonzero().flatten()[: B - have]
+        rows = torch.isin(seg, sel)
+        parts.append((ops[sel], ch[sel], n[sel], var[sel], X[rows], y[rows].float()))
+        have += sel.numel()
+    ops, ch, n, var, X, Y = (torch.cat([p_[i] for p_ in parts]).contiguous() for i in range(6))
+    Bk = ops.shape[0]
+    x_off, y_off, n32 = fused.ragged_offsets(n, 3)
+    C0 = warmstart.random_start(ops, seed + 1)
+    stop = (5e-7 * var).contiguous()
+    run = lambda: _fit(X, Y, ops, ch, C0, 100, 10, 1e-7, "strong_wolfe", stop_loss=stop, x_off=x_off, y_off=y_off,
+                       n_pts=n32)
+    ms, (C, loss, iters, err, _, ncl) = timed(run, steps, 1)
+    yf = fused.fused_root_ragged(X.double(), n, ops, ch, C.double(), None, fwd_only=True, raise_errors=False)["y"]
+    seg = torch.repeat_interleave(torch.arange(Bk, device=dev), n)
+    num = torch.zeros(Bk, dtype=torch.float64, device=dev).index_add_(0, seg, (yf - Y.double()) ** 2)
+    r = num / (var * n)
+    fitted = int((torch.isfinite(r) & (r <= 1e-6)).sum())
+    per_pt = torch.tensor([expression_cost(row) + LOSS_SLOTS for row in ops.cpu().tolist()], dtype=torch.float64)
+    slots = float((per_pt * ncl.cpu().double() * n.cpu().double()).sum())
+    return dict(workload=f"G2 B={Bk} own expressions (postfix 3..127, padded 128), own datasets N_i log-uniform "
+                         f"[64, 4096] (mean {float(n.double().mean()):.1f}), n_x=3, randn start, stop at "
+                         f"loss <= 5e-7 Var(Y_i)",
+                ms=round(ms, 3), fitted=fitted, fitted_per_s=round(fitted / (ms / 1e3), 1),
+                points=int(n.sum()), closures=int(ncl.sum()), mean_iters=round(float(iters.double().mean()), 2),
+                problems_without_error=int((err == 0).sum()),
+                valu=dict(achieved_TFLOPs=round(slots / (ms / 1e3) / 1e12, 4), peak_TFLOPs=FP32_PEAK_TFLOPS,
+                          frac=round(slots / (ms / 1e3) / 1e12 / FP32_PEAK_TFLOPS, 5), slots=slots))
+
+
 def fit(w, C_start, stop=True):
     from cudasymbolicregression_amd.optimization import _fit
 
@@ -195,6 +262,7 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-evaluate", action="store_true")
+    ap.add_argument("--no-ragged", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer (calibrates the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -276,6 +344,11 @@ def main() -> int:
         ev = bench_evaluate(dev, steps=5)
         log(f"evaluate (C2): {ev['ms']:.3f} ms, {ev['hbm']['achieved_GBps']} GB/s, VALU frac {ev['valu']['frac']}")
 
+    rg = None
+    if not args.no_ragged:
+        rg = bench_ragged(dev, steps=2)
+        log(f"ragged (C4): {rg['ms']:.1f} ms, {rg['fitted']} fitted, {rg['fitted_per_s']:.0f} fitted/s")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, args.cpu_sample)
@@ -312,6 +385,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "evaluate": ev,
+        "ragged": rg,
     }
     print(json.dumps(out))
     return 0

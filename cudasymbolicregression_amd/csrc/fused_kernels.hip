@@ -9,7 +9,9 @@
 //   MODE 1: (2 * (y - Y)) * (1 / N)   (the MSE closure; dy is the target Y,
 //                                      loss_out[b] = mean_n (y - Y)^2 in FP64).
 // Outputs: y (B,N), grad_C (B,M) summed over the points in FP64, grad_X
-// (N,n_x) summed over the batch, per-expression error codes.
+// (N,n_x) summed over the batch, per-expression error codes.  Ragged form:
+// expression b reads its own n_pts[b] points at X + x_off[b] and its dy / y
+// at offset y_off[b] (no grad_X).
 //
 // Execution (gfx950): one wave per expression; the program is compiled once
 // per wave into an instruction list (sr_code.h) and evaluated over tile
@@ -60,6 +62,9 @@ struct FArgs {
     int mode;
     int fwd_only;
     int want_gx;
+    const int64_t *x_off;   // ragged: element offset of expression b's X rows (nullable)
+    const int64_t *y_off;   // ragged: offset of expression b's dy / y values
+    const int32_t *n_pts;   // ragged: number of points of expression b
 };
 
 // LT: most instructions of an expression in the bucket; SL: slot registers
@@ -92,7 +97,7 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
         for (int64_t i = threadIdx.x; i < gx_len; i += WPB * WAVE) gx_blk[i] = T(0);
     __syncthreads();
     const int64_t count = (int64_t)*a.list_count;
-    const T inv_n = T(1.0) / T(N);
+    const T *const X0 = X;
 
     for (;;) {
         const int i = wave_fetch_add(a.work_counter);
@@ -116,6 +121,11 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
         }
         const SVec<T, SL> xc = compact_start<T, SL>(p);
         const int NI = c.NI;
+        // this expression's points (the shared X, or its own rows when ragged)
+        const int64_t Nb = a.n_pts ? (int64_t)a.n_pts[b] : N;
+        const int64_t yo = a.y_off ? a.y_off[b] : b * N;
+        X = X0 + (a.x_off ? a.x_off[b] : 0);
+        const T inv_n = T(1.0) / T(Nb);
         double lacc = 0.0;
         double cacc[F_NCREG];
 #pragma unroll
@@ -124,14 +134,14 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
 #pragma unroll
         for (int h = 0; h < SL; ++h) gc[h] = 0.0;
         int32_t e = 0;
-        for (int64_t base = 0; base < N; base += (int64_t)WAVE * PPT) {
+        for (int64_t base = 0; base < Nb; base += (int64_t)WAVE * PPT) {
             bool act[PPT];
             int64_t n[PPT];
             T xv[PPT][F_NX], gxr[PPT][F_NX];
 #pragma unroll
             for (int q = 0; q < PPT; ++q) {
                 n[q] = base + q * WAVE + lane;
-                act[q] = n[q] < N;
+                act[q] = n[q] < Nb;
                 const T *row = X + (act[q] ? n[q] : 0) * (int64_t)n_x;
 #pragma unroll
                 for (int v = 0; v < F_NX; ++v) {
@@ -187,14 +197,14 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
 #pragma unroll
             for (int q = 0; q < PPT; ++q) {
                 const T yh = t.val(NI - 1, q);
-                if (act[q] && y_out) y_out[b * N + n[q]] = yh;
+                if (act[q] && y_out) y_out[yo + n[q]] = yh;
                 T g0 = T(0);
                 if (a.mode == 1) {
-                    const T d = yh - (act[q] ? dy[b * N + n[q]] : T(0));
+                    const T d = yh - (act[q] ? dy[yo + n[q]] : T(0));
                     if (act[q]) lacc += (double)d * (double)d;
                     g0 = (T(2.0) * d) * inv_n;
                 } else if (!a.fwd_only) {
-                    g0 = act[q] ? dy[b * N + n[q]] : T(0);
+                    g0 = act[q] ? dy[yo + n[q]] : T(0);
                 }
                 t.set_adj(NI - 1, q, act[q] ? g0 : T(0));
             }
@@ -289,7 +299,7 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
         }
         if (a.mode == 1 && a.loss_out) {
             const double s = wave_sum_rows(lacc, 4);
-            if (lane == 0) a.loss_out[b] = s / (double)N;
+            if (lane == 0) a.loss_out[b] = s / (double)Nb;
         }
         if (a.fwd_only) {
             if (lane == 0 && a.err_b) a.err_b[b] = 0;
@@ -455,9 +465,12 @@ template <typename T>
 int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch, const T *C, int64_t M,
                     int64_t B, const T *dy, int mode, int fwd_only, T *y_out, double *loss_out, T *grad_C,
                     T *grad_X, int32_t *err_b, int32_t *flags, void *workspace, int64_t workspace_bytes,
-                    hipStream_t s) {
+                    hipStream_t s, const int64_t *x_off = nullptr, const int64_t *y_off = nullptr,
+                    const int32_t *n_pts = nullptr) {
     if (M <= 0 || M > 128 || B < 0 || N <= 0 || n_x < 0 || n_x > MAX_NX_CODE + 1 || (mode != 0 && mode != 1))
         return -1;
+    const bool ragged = x_off != nullptr || y_off != nullptr || n_pts != nullptr;
+    if (ragged && (x_off == nullptr || y_off == nullptr || n_pts == nullptr || grad_X != nullptr)) return -1;
     if (dy == nullptr && (mode == 1 || !fwd_only)) return -1;  // the kernel reads dy in these cases
     if (grad_C == nullptr && !fwd_only) return -1;
     const bool want_gx = grad_X != nullptr && !fwd_only;
@@ -498,6 +511,7 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
     a.err_b = err_b; a.flags = flags;
     a.gtape = L.g3 ? (void *)(ws + L.gt_off) : nullptr;
     a.mode = mode; a.fwd_only = fwd_only; a.want_gx = want_gx ? 1 : 0;
+    a.x_off = x_off; a.y_off = y_off; a.n_pts = n_pts;
     auto bucket_args = [&](int k, int64_t row0) {
         FArgs r = a;
         r.list = lists + (size_t)k * B;
@@ -561,6 +575,29 @@ int sr_fused_root_f64(const double *X, int64_t N, int64_t n_x, const int64_t *op
                       int32_t *flags, void *workspace, int64_t workspace_bytes, void *stream) {
     return fused_root_code<double>(X, N, n_x, ops, ch, C, M, B, dy, mode, fwd_only, y_out, loss_out, grad_C, grad_X,
                                    err_b, flags, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+// Ragged: expression b has n_pts[b] points at X + x_off[b] (row-major,
+// n_x per row) and its dy / y values at offset y_off[b]; N is the largest
+// n_pts (only its bound matters).  No grad_X.
+int sr_fused_root_ragged_f32(const float *X, int64_t N, int64_t n_x, const int64_t *x_off, const int64_t *y_off,
+                             const int32_t *n_pts, const int64_t *ops, const int64_t *ch, const float *C, int64_t M,
+                             int64_t B, const float *dy, int mode, int fwd_only, float *y_out, double *loss_out,
+                             float *grad_C, int32_t *err_b, int32_t *flags, void *workspace, int64_t workspace_bytes,
+                             void *stream) {
+    if (x_off == nullptr || y_off == nullptr || n_pts == nullptr) return -1;
+    return fused_root_code<float>(X, N, n_x, ops, ch, C, M, B, dy, mode, fwd_only, y_out, loss_out, grad_C, nullptr,
+                                  err_b, flags, workspace, workspace_bytes, (hipStream_t)stream, x_off, y_off, n_pts);
+}
+int sr_fused_root_ragged_f64(const double *X, int64_t N, int64_t n_x, const int64_t *x_off, const int64_t *y_off,
+                             const int32_t *n_pts, const int64_t *ops, const int64_t *ch, const double *C, int64_t M,
+                             int64_t B, const double *dy, int mode, int fwd_only, double *y_out, double *loss_out,
+                             double *grad_C, int32_t *err_b, int32_t *flags, void *workspace, int64_t workspace_bytes,
+                             void *stream) {
+    if (x_off == nullptr || y_off == nullptr || n_pts == nullptr) return -1;
+    return fused_root_code<double>(X, N, n_x, ops, ch, C, M, B, dy, mode, fwd_only, y_out, loss_out, grad_C, nullptr,
+                                   err_b, flags, workspace, workspace_bytes, (hipStream_t)stream, x_off, y_off,
+                                   n_pts);
 }
 
 }  // extern "C"

@@ -102,3 +102,83 @@ def fused_root(
         if f[1]:
             raise RuntimeError(get_error_message(f[1], "symbolic evaluation backward pass"))
     return res
+
+
+def ragged_offsets(n_pts: torch.Tensor, n_x: int):
+    """(x_off, y_off, n32) for problems stored back to back: problem b's rows
+    start at sum_{j<b} n_pts[j]."""
+    n = n_pts.to(torch.int64)
+    start = torch.cumsum(n, 0) - n
+    return (start * n_x).contiguous(), start.contiguous(), n.to(torch.int32).contiguous()
+
+
+def fused_root_ragged(
+    X: torch.Tensor,
+    n_pts: torch.Tensor,
+    Ops: torch.Tensor,
+    Ch: torch.Tensor,
+    C: torch.Tensor,
+    dy: Optional[torch.Tensor],
+    mode: int = 0,
+    fwd_only: bool = False,
+    raise_errors: bool = True,
+) -> Dict[str, torch.Tensor]:
+    """`fused_root` for expressions that each have their own points: X is
+    (sum_b N_b, n_x) with problem b's rows after those of problems < b, and
+    dy (mode 0: upstream gradient; mode 1: targets) and the returned y are
+    (sum_b N_b,) in the same order.  loss (mode 1) is each problem's mean over
+    its own points.  No grad_X (there is no shared X)."""
+    _require_gpu(X, "X")
+    dev, dt = X.device, X.dtype
+    if dt not in (torch.float32, torch.float64) or X.dim() != 2:
+        raise RuntimeError("X must be a (sum N_b, n_x) floating-point tensor")
+    B, M = Ops.shape
+    n_x = X.shape[1]
+    n = n_pts.to(device=dev)
+    if n.shape != (B,):
+        raise RuntimeError("n_pts must have shape (B,)")
+    total = X.shape[0]
+    if B and (int(n.min()) < 1 or int(n.sum()) != total):
+        raise RuntimeError(f"n_pts must be >= 1 and sum to the {total} rows of X")
+    x_off, y_off, n32 = ragged_offsets(n, n_x)
+    Nmax = int(n.max()) if B else 1
+    X = X.contiguous()
+    Ops = Ops.to(device=dev, dtype=torch.int64).contiguous()
+    Ch = Ch.to(device=dev, dtype=torch.int64).contiguous()
+    C = C.to(device=dev, dtype=dt).contiguous()
+    if dy is None:
+        if not (fwd_only and mode == 0):
+            raise RuntimeError("dy (upstream gradient or target) is required")
+        dyc = None
+    else:
+        dyc = dy.reshape(-1).to(device=dev, dtype=dt).contiguous()
+        if dyc.numel() != total:
+            raise RuntimeError("dy must hold one value per point")
+    y = torch.empty(total, dtype=dt, device=dev)
+    loss = torch.empty(B, dtype=torch.float64, device=dev) if mode == 1 else None
+    gC = None if fwd_only else torch.empty((B, M), dtype=dt, device=dev)
+    err = torch.empty(B, dtype=torch.int32, device=dev)
+    flags = torch.empty(2, dtype=torch.int32, device=dev)
+    lib = _lib.hip()
+    is64 = dt == torch.float64
+    ws_bytes = lib.sr_fused_workspace_bytes(Nmax, n_x, B, M, 0, int(is64))
+    ws = torch.empty(max(ws_bytes, 256), dtype=torch.uint8, device=dev)
+    fn = lib.sr_fused_root_ragged_f64 if is64 else lib.sr_fused_root_ragged_f32
+    _lib.check_status(
+        fn(_lib.ptr(X), Nmax, n_x, _lib.ptr(x_off), _lib.ptr(y_off), _lib.ptr(n32), _lib.ptr(Ops), _lib.ptr(Ch),
+           _lib.ptr(C), M, B, _lib.ptr(dyc), int(mode), int(bool(fwd_only)), _lib.ptr(y), _lib.ptr(loss),
+           _lib.ptr(gC), _lib.ptr(err), _lib.ptr(flags), _lib.ptr(ws), ws.numel(), _lib.stream_of(dev)),
+        "fused_root_ragged",
+    )
+    res = dict(y=y, err=err, flags=flags)
+    if loss is not None:
+        res["loss"] = loss
+    if gC is not None:
+        res["grad_C"] = gC
+    if raise_errors:
+        f = flags.tolist()
+        if f[0]:
+            raise RuntimeError(get_error_message(f[0], "input validation"))
+        if f[1]:
+            raise RuntimeError(get_error_message(f[1], "symbolic evaluation backward pass"))
+    return res

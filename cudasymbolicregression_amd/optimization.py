@@ -15,7 +15,7 @@ import torch
 
 from . import _lib
 from ._C import _require_gpu, get_error_message
-from .fused import fused_root
+from .fused import fused_root, ragged_offsets
 
 _LINE_SEARCH = {"strong_wolfe": 0, "backtracking": 1}
 
@@ -162,10 +162,7 @@ def optimize_ragged(
     if X.shape[0] != total or Y.numel() != total:
         raise RuntimeError(f"X has {X.shape[0]} rows and Y {Y.numel()} targets; n_pts sums to {total}")
     n_x = X.shape[1]
-    start = torch.cumsum(n, 0) - n
-    x_off = (start * n_x).contiguous()
-    y_off = start.contiguous()
-    n32 = n.to(torch.int32).contiguous()
+    x_off, y_off, n32 = ragged_offsets(n, n_x)
     Xc = X.contiguous()
     Yc = Y.reshape(-1).to(torch.float32).contiguous()
     stop = None

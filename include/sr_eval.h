@@ -101,6 +101,23 @@ int sr_fused_root_f64(const double *X, int64_t N, int64_t n_x, const int64_t *op
                       double *y_out, double *loss_out, double *grad_C, double *grad_X, int32_t *err_b,
                       int32_t *flags, void *workspace, int64_t workspace_bytes, void *stream);
 
+/* Ragged form of sr_fused_root_*: expression b has its own n_pts[b] points
+ * at X + x_off[b] (rows of n_x values) and its dy (target in mode 1) and y
+ * values at offset y_off[b]; loss_out[b] is the mean over its own points.
+ * N is the largest n_pts[b] (a bound; the workspace does not depend on it
+ * without grad_X).  No grad_X.  No counterpart in the reference, whose
+ * evaluation shares one X across the batch. */
+int sr_fused_root_ragged_f32(const float *X, int64_t N, int64_t n_x, const int64_t *x_off, const int64_t *y_off,
+                             const int32_t *n_pts, const int64_t *ops, const int64_t *ch, const float *C, int64_t M,
+                             int64_t B, const float *dy, int mode, int fwd_only, float *y_out, double *loss_out,
+                             float *grad_C, int32_t *err_b, int32_t *flags, void *workspace, int64_t workspace_bytes,
+                             void *stream);
+int sr_fused_root_ragged_f64(const double *X, int64_t N, int64_t n_x, const int64_t *x_off, const int64_t *y_off,
+                             const int32_t *n_pts, const int64_t *ops, const int64_t *ch, const double *C, int64_t M,
+                             int64_t B, const double *dy, int mode, int fwd_only, double *y_out, double *loss_out,
+                             double *grad_C, int32_t *err_b, int32_t *flags, void *workspace, int64_t workspace_bytes,
+                             void *stream);
+
 /* Device-resident batched L-BFGS over the LEARNABLE_CONSTANT slots of B
  * batch-first expressions.  Replaces optimize + LBFGS_batch.step
  * (symbolic_torch/optimization.py:9-112, symbolic_torch/lbfgs_batch.py:61-326)

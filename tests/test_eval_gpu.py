@@ -301,3 +301,51 @@ def test_multiple_constants_match_oracle(K):
     assert ok, "grad_C " + msg
     ok, msg = helpers.check_vs_f64(Xt.grad.cpu().numpy(), gx32, gx64, atol=1e-5)
     assert ok, "grad_X " + msg
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_fused_root_ragged_equals_per_problem_calls(mode, dtype):
+    """Each expression on its own points (N_b in [1, 300]) in one ragged call
+    gives, bit for bit, what a dense call on that expression alone gives:
+    y, loss (mode 1), grad_C and error codes."""
+    from cudasymbolicregression_amd import fused
+
+    rng = np.random.default_rng(50 + mode)
+    B, M, n_x = 40, 96, 3
+    ops, ch = helpers.random_batch(51 + mode, B, M, n_x, unary=helpers.G2_UNARY, binary=helpers.G2_BINARY,
+                                   depth=7)
+    C = helpers.constants_like(ops, rng, dtype)
+    n = rng.integers(1, 301, size=B)
+    n[0] = 1
+    Xs = [rng.uniform(-3, 3, (int(k), n_x)).astype(dtype) for k in n]
+    ds = [rng.standard_normal(int(k)).astype(dtype) for k in n]
+    td = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(_dev())  # noqa: E731
+    r = fused.fused_root_ragged(td(np.concatenate(Xs)), td(n), td(ops), td(ch), td(C), td(np.concatenate(ds)),
+                                mode=mode, raise_errors=False)
+    bits = lambda t: t.view(torch.int32) if t.dtype == torch.float32 else t.view(torch.int64)  # noqa: E731
+    start = np.concatenate([[0], np.cumsum(n)[:-1]])
+    lengths = (ops != 0).sum(1)
+    print("program lengths:", sorted(lengths.tolist()), flush=True)
+    for b in range(B):
+        d = fused.fused_root(td(Xs[b]), td(ops[b:b + 1]), td(ch[b:b + 1]), td(C[b:b + 1]), td(ds[b][None, :]),
+                             mode=mode, raise_errors=False)
+        seg = r["y"][start[b]:start[b] + n[b]]
+        assert torch.equal(bits(seg), bits(d["y"][0])), b
+        assert torch.equal(bits(r["grad_C"][b]), bits(d["grad_C"][0])), b
+        assert int(r["err"][b]) == int(d["err"][0]), b
+        if mode == 1:
+            assert torch.equal(r["loss"][b:b + 1].view(torch.int64), d["loss"].view(torch.int64)), b
+
+
+def test_fused_root_ragged_rejects_bad_sizes():
+    from cudasymbolicregression_amd import fused
+
+    X = torch.zeros(10, 3, device=_dev())
+    ops = torch.tensor([[50, 1, 40]], device=_dev())
+    ch = torch.tensor([[[-1, -1], [-1, -1], [0, 1]]], device=_dev())
+    C = torch.zeros(1, 3, device=_dev())
+    with pytest.raises(RuntimeError, match="sum to"):
+        fused.fused_root_ragged(X, torch.tensor([9], device=_dev()), ops, ch, C, None, fwd_only=True)
+    with pytest.raises(RuntimeError, match="sum to"):
+        fused.fused_root_ragged(X[:0], torch.tensor([0], device=_dev()), ops, ch, C, None, fwd_only=True)
