@@ -90,7 +90,7 @@ def hip() -> ctypes.CDLL:
         _declare(lib, "sr_mlp_col_sum", I, [P, I64, I64, P, P, I64, P])
         _declare(lib, "sr_mlp_adam", I, [P, P, P, P, I64, F, F, F, F, I64, F, P, P])
         _declare(lib, "sr_mlp_to_bf16", I, [P, I64, P, P])
-        _declare(lib, "sr_mlp_features", I, [P, I64, I64, I64, I, I, P, I64, I, I, P, P])
+        _declare(lib, "sr_mlp_features", I, [P, I64, I64, I64, I, I, P, I64, I, I, I, P, P])
         _declare(lib, "sr_mlp_mse_head", I, [P, P, P, I64, I64, I64, F, P, P, P, P])
         _declare(lib, "sr_mlp_predict", I, [P, P, I64, I64, P, P])
         _declare(lib, "sr_mlp_select_start", I, [P, P, P, P, I64, I64, P, P, P])

@@ -190,22 +190,65 @@ __global__ void k_to_bf16(const float *__restrict__ x, int64_t n, uint16_t *__re
 
 // Input rows: one-hot over (slot, opcode) for the first `slots` nodes
 // (opcodes outside [0, vocab) encode nothing), then g(Y[b][p]) for p <
-// n_pts, zero padding up to row_len.  Rows b >= B (batch padding up to a
-// multiple of 32) are all zero.  One block per row.
-__global__ void k_features(const int64_t *__restrict__ ops, int M, int64_t B, int slots, int vocab,
-                           const float *__restrict__ Y, int64_t ldy, int n_pts, int row_len,
-                           uint16_t *__restrict__ out) {
+// n_pts with g(y) = sign(y) log1p|y| (NaN -> 0); with std_block, then the
+// standardised targets (y - mean) / std clipped to [-10, 10] (0 when std is
+// below 1e-12) and the two scalars g(mean), log1p(std), where mean and std
+// are over the n_pts targets with non-finite values read as 0.  Zero padding
+// up to row_len; rows b >= B (batch padding up to a multiple of 32) are all
+// zero.  One block of 256 threads per row; the statistics are FP64 sums
+// reduced in a fixed order.
+__global__ __launch_bounds__(256) void k_features(const int64_t *__restrict__ ops, int M, int64_t B, int slots,
+                                                 int vocab, const float *__restrict__ Y, int64_t ldy, int n_pts,
+                                                 int std_block, int row_len, uint16_t *__restrict__ out) {
+    __shared__ double red[2][256];
     const int64_t b = blockIdx.x;
     uint16_t *row = out + (size_t)b * row_len;
     const int onehot = slots * vocab;
+    const float *yr = Y + (b < B ? b : 0) * ldy;
+    auto clean = [](float y) { return (y == y && fabsf(y) <= 3.0e38f) ? y : 0.0f; };
+    double mean = 0.0, sd = 0.0;
+    if (std_block) {
+        double s1 = 0.0, s2 = 0.0;
+        if (b < B)
+            for (int p = threadIdx.x; p < n_pts; p += 256) {
+                const double y = clean(yr[p]);
+                s1 += y;
+                s2 += y * y;
+            }
+        red[0][threadIdx.x] = s1;
+        red[1][threadIdx.x] = s2;
+        __syncthreads();
+        for (int w = 128; w >= 1; w >>= 1) {
+            if ((int)threadIdx.x < w) {
+                red[0][threadIdx.x] += red[0][threadIdx.x + w];
+                red[1][threadIdx.x] += red[1][threadIdx.x + w];
+            }
+            __syncthreads();
+        }
+        const double n = n_pts > 0 ? (double)n_pts : 1.0;
+        mean = red[0][0] / n;
+        const double var = red[1][0] / n - mean * mean;
+        sd = var > 0.0 ? sqrt(var) : 0.0;
+    }
+    const float inv_sd = sd > 1e-12 ? (float)(1.0 / sd) : 0.0f;
     for (int i = threadIdx.x; i < row_len; i += blockDim.x) {
         float v = 0.0f;
-        if (b < B && i < onehot) {
-            const int k = i / vocab, c = i % vocab;
-            v = (k < M && ops[b * M + k] == c) ? 1.0f : 0.0f;
-        } else if (b < B && i < onehot + n_pts) {
-            const float y = Y[b * ldy + (i - onehot)];
-            v = (y == y) ? copysignf(log1pf(fminf(fabsf(y), 3.0e38f)), y) : 0.0f;
+        if (b < B) {
+            if (i < onehot) {
+                const int k = i / vocab, c = i % vocab;
+                v = (k < M && ops[b * M + k] == c) ? 1.0f : 0.0f;
+            } else if (i < onehot + n_pts) {
+                const float y = yr[i - onehot];
+                v = (y == y) ? copysignf(log1pf(fminf(fabsf(y), 3.0e38f)), y) : 0.0f;
+            } else if (std_block && i < onehot + 2 * n_pts) {
+                const float z = (clean(yr[i - onehot - n_pts]) - (float)mean) * inv_sd;
+                v = fminf(fmaxf(z, -10.0f), 10.0f);
+            } else if (std_block && i == onehot + 2 * n_pts) {
+                const float m = (float)mean;
+                v = copysignf(log1pf(fabsf(m)), m);
+            } else if (std_block && i == onehot + 2 * n_pts + 1) {
+                v = log1pf((float)sd);
+            }
         }
         row[i] = f2bf(v);
     }
@@ -332,12 +375,13 @@ int sr_mlp_to_bf16(const float *x, int64_t n, void *y, void *stream) {
 }
 
 int sr_mlp_features(const int64_t *ops, int64_t M, int64_t B, int64_t B_pad, int slots, int vocab, const float *Y,
-                    int64_t ldy, int n_pts, int row_len, void *out, void *stream) {
+                    int64_t ldy, int n_pts, int std_block, int row_len, void *out, void *stream) {
     if (B < 0 || B_pad < B || M <= 0 || slots < 0 || vocab <= 0 || n_pts < 0 || n_pts > ldy ||
-        (int64_t)slots * vocab + n_pts > row_len)
+        (std_block != 0 && std_block != 1) ||
+        (int64_t)slots * vocab + (int64_t)n_pts * (1 + std_block) + 2 * std_block > row_len)
         return -1;
     if (B_pad) hipLaunchKernelGGL(k_features, dim3((unsigned)B_pad), dim3(256), 0, (hipStream_t)stream, ops, (int)M,
-                                  B, slots, vocab, Y, ldy, n_pts, row_len, (uint16_t *)out);
+                                  B, slots, vocab, Y, ldy, n_pts, std_block, row_len, (uint16_t *)out);
     return last();
 }
 

@@ -7,7 +7,9 @@ from the prediction instead of randn wherever that start has the lower loss.
 
 Network (all layers `W (out, in)`, exact-erf GELU):
     x   = one-hot(Ops[:, :64] over opcodes 0..V-1)  ++  sign(y) log1p|y| at
-          the first 256 target points, zero padded to a multiple of 64
+          the first 256 target points  ++  (y - mean) / std at the same points
+          (clipped to +-10)  ++  sign(mean) log1p|mean|, log1p(std); zero
+          padded to a multiple of 64
     h_l = gelu(h_{l-1} W_l^T + b_l), l = 1..3, width 512
     out = h_3 W_4^T + b_4   (64 slots; only the constant slots are used)
 with V = VAR_START_ID + n_x opcodes (53 for n_x = 3, the reference's
@@ -52,13 +54,14 @@ class WarmStartMLP:
     """The warm-start network, its Adam state and its activation buffers."""
 
     def __init__(self, device, n_x: int = 3, slots: int = 64, n_pts: int = 256, hidden: int = 512,
-                 depth: int = 3, seed: int = 0):
+                 depth: int = 3, seed: int = 0, std_features: bool = True):
         if hidden % 32 or hidden <= 0 or depth < 1 or not 0 < slots <= N_OUT or n_pts < 0:
             raise ValueError("hidden must be a positive multiple of 32, depth >= 1, 0 < slots <= 64")
         self.device = torch.device(device)
         self.n_x, self.slots, self.n_pts = int(n_x), int(slots), int(n_pts)
         self.vocab = int(Operator.VAR_START_ID) + self.n_x
-        self.row_len = _round_up(self.slots * self.vocab + self.n_pts, 64)
+        self.std_block = 1 if std_features else 0
+        self.row_len = _round_up(self.slots * self.vocab + self.n_pts * (1 + self.std_block) + 2 * self.std_block, 64)
         self.dims = [self.row_len] + [int(hidden)] * int(depth) + [N_OUT]
         gen = torch.Generator().manual_seed(int(seed))
         dev = self.device
@@ -158,7 +161,7 @@ class WarmStartMLP:
         buf = self._buffers(Bp, train)
         lib, s = _lib.hip(), self._stream()
         _ck(lib.sr_mlp_features(_lib.ptr(ops), M, B, Bp, self.slots, self.vocab, _lib.ptr(Y), Y.shape[1],
-                                self.n_pts, self.row_len, _lib.ptr(buf["x"]), s), "sr_mlp_features")
+                                self.n_pts, self.std_block, self.row_len, _lib.ptr(buf["x"]), s), "sr_mlp_features")
         # layer 1: MFMA product over the target columns, then the one-hot
         # columns as gathered weight rows, bias and GELU
         x, H1, oh = buf["x"], self.dims[1], self.onehot
@@ -230,6 +233,17 @@ class WarmStartMLP:
             dz_f, dz_b = buf["dZ"][nxt], buf["dZb"][nxt]
         return buf["loss"][:B]
 
+    def loss_rows(self, ops: torch.Tensor, Y: torch.Tensor, label: torch.Tensor) -> torch.Tensor:
+        """Per-row loss terms sum_{k: Ops[b,k]==1} (out[b,k] - label[b,k])^2
+        (forward only)."""
+        B, M, Bp, buf = self._forward(ops, Y, False)
+        rows = buf.get("loss_eval")
+        if rows is None or rows.numel() < Bp:
+            rows = buf["loss_eval"] = torch.empty(Bp, dtype=torch.float32, device=self.device)
+        _ck(_lib.hip().sr_mlp_mse_head(_lib.ptr(buf["out"]), _lib.ptr(label), _lib.ptr(ops), M, B, Bp, 1.0,
+                                       None, None, _lib.ptr(rows), self._stream()), "sr_mlp_mse_head")
+        return rows[:B]
+
     def adam_step(self, lr: float, betas=(0.9, 0.999), eps: float = 1e-8, weight_decay: float = 0.0) -> None:
         """Apply the gradients in self.gW / self.gb (one Adam step) and refresh
         the BF16 weights."""
@@ -251,7 +265,8 @@ class WarmStartMLP:
 
     # ------------------------------------------------------------- persistence
     def state_dict(self) -> Dict[str, torch.Tensor]:
-        d = {"meta": torch.tensor([self.n_x, self.slots, self.n_pts, self.dims[1], self.layers - 1, self.step])}
+        d = {"meta": torch.tensor([self.n_x, self.slots, self.n_pts, self.dims[1], self.layers - 1, self.step,
+                                   self.std_block])}
         for l in range(self.layers):
             d[f"W{l}"] = self.W[l].cpu()
             d[f"b{l}"] = self.b[l].cpu()
@@ -265,11 +280,15 @@ class WarmStartMLP:
 
 
 def train(model: WarmStartMLP, ops: torch.Tensor, Y: torch.Tensor, label: torch.Tensor, epochs: int = 4,
-          batch: int = 4096, lr: float = 1e-3, lr_final: float = 1e-5, seed: int = 0, log=None) -> List[float]:
+          batch: int = 4096, lr: float = 1e-3, lr_final: float = 1e-5, seed: int = 0, log=None,
+          val: Optional[Tuple[torch.Tensor, torch.Tensor, torch.Tensor]] = None) -> List[float]:
     """Adam over shuffled mini-batches with a cosine learning-rate decay from
-    lr to lr_final.  Returns the mean training loss of every epoch."""
+    lr to lr_final.  Returns the mean training loss of every epoch; with
+    val = (Ops, Y, label) the held-out loss of every epoch is appended to
+    model.val_history."""
     dev = model.device
     P = ops.shape[0]
+    model.val_history = []
     if P < batch:
         batch = max(32, P // 32 * 32)
     steps_per_epoch = max(1, P // batch)
@@ -287,8 +306,14 @@ def train(model: WarmStartMLP, ops: torch.Tensor, Y: torch.Tensor, label: torch.
             acc += rows.double().sum()
             t += 1
         hist.append(float(acc) / (steps_per_epoch * batch))
+        if val is not None:
+            vl = sum(float(model.loss_rows(val[0][i:i + 16384].contiguous(), val[1][i:i + 16384].contiguous(),
+                                           val[2][i:i + 16384].contiguous()).double().sum())
+                     for i in range(0, val[0].shape[0], 16384)) / max(1, val[0].shape[0])
+            model.val_history.append(vl)
         if log:
-            log(f"epoch {ep + 1}/{epochs}: loss {hist[-1]:.4f}")
+            log(f"epoch {ep + 1}/{epochs}: loss {hist[-1]:.4f}" +
+                (f", held-out {model.val_history[-1]:.4f}" if val is not None else ""))
     return hist
 
 

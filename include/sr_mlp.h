@@ -70,10 +70,13 @@ int sr_mlp_to_bf16(const float *x, int64_t n, void *y, void *stream);
 
 /* Network input rows (BF16, B_pad x row_len): one-hot of Ops[b][k] over
  * opcodes 0..vocab-1 for k < slots, then sign(y) log1p|y| of Y[b][p] for
- * p < n_pts (NaN -> 0), zeros to row_len; rows B..B_pad-1 all zero.
- * Ops is (B, M) int64 in the reference layout, Y has row stride ldy. */
+ * p < n_pts (NaN -> 0); with std_block = 1 also the standardised targets
+ * (y - mean) / std clipped to [-10, 10] and the scalars sign(mean)
+ * log1p|mean|, log1p(std) (statistics over the n_pts targets, non-finite
+ * values read as 0); zeros to row_len; rows B..B_pad-1 all zero.  Ops is
+ * (B, M) int64 in the reference layout, Y has row stride ldy. */
 int sr_mlp_features(const int64_t *ops, int64_t M, int64_t B, int64_t B_pad, int slots, int vocab, const float *Y,
-                    int64_t ldy, int n_pts, int row_len, void *out, void *stream);
+                    int64_t ldy, int n_pts, int std_block, int row_len, void *out, void *stream);
 
 /* Masked MSE head over network outputs out (rows_p x 64, FP32): for rows
  * b < rows and slots k with Ops[b][k] == LEARNABLE_CONSTANT,

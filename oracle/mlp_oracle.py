@@ -42,9 +42,13 @@ def gelu_grad(z):
     return 0.5 * (1.0 + erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2.0 * np.pi)
 
 
-def features(ops, Y, slots: int, vocab: int, n_pts: int, row_len: int, bf16: bool = True) -> np.ndarray:
+def features(ops, Y, slots: int, vocab: int, n_pts: int, row_len: int, bf16: bool = True,
+             std_block: bool = False) -> np.ndarray:
     """Input rows: one-hot (slot, opcode) over the first `slots` nodes, then
-    sign(y) log1p(|y|) of the first n_pts targets (NaN -> 0), zero padded."""
+    sign(y) log1p(|y|) of the first n_pts targets (NaN -> 0); with std_block
+    then (y - mean) / std clipped to [-10, 10] (0 when std <= 1e-12) and
+    sign(mean) log1p|mean|, log1p(std), the statistics over the n_pts targets
+    with non-finite values read as 0; zero padded."""
     B, M = ops.shape
     out = np.zeros((B, row_len), dtype=F64)
     for k in range(min(slots, M)):
@@ -55,7 +59,16 @@ def features(ops, Y, slots: int, vocab: int, n_pts: int, row_len: int, bf16: boo
         y = np.asarray(Y, dtype=np.float32)[:, :n_pts].astype(F64)
     g = np.sign(y) * np.log1p(np.minimum(np.abs(y), 3.0e38))
     g = np.where(np.isnan(y), 0.0, g)
-    out[:, slots * vocab: slots * vocab + n_pts] = g
+    base = slots * vocab
+    out[:, base: base + n_pts] = g
+    if std_block:
+        yc = np.where(np.isfinite(y) & (np.abs(y) <= 3.0e38), y, 0.0)
+        mean = yc.mean(axis=1, keepdims=True)
+        sd = np.sqrt(np.maximum((yc * yc).mean(axis=1, keepdims=True) - mean * mean, 0.0))
+        z = np.where(sd > 1e-12, (yc - mean) / np.where(sd > 1e-12, sd, 1.0), 0.0)
+        out[:, base + n_pts: base + 2 * n_pts] = np.clip(z, -10.0, 10.0)
+        out[:, base + 2 * n_pts] = np.sign(mean[:, 0]) * np.log1p(np.abs(mean[:, 0]))
+        out[:, base + 2 * n_pts + 1] = np.log1p(sd[:, 0])
     return bf16_round(out) if bf16 else out
 
 

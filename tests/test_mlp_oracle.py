@@ -108,3 +108,25 @@ def test_features_hand_built_row():
     want[1, base:base + 3] = [0.0, np.log1p(3.0e38), -np.log1p(1.0)]  # 1e39 -> inf in FP32 -> clamped
     np.testing.assert_allclose(f, want, rtol=1e-15)
     assert f.shape == (2, row_len) and row_len % 64 == 0
+
+
+def test_features_standardised_block():
+    """The standardised block against a direct computation: constant targets
+    give zeros, a NaN reads as 0 in the statistics, values clip at +-10."""
+    ops = np.array([[1, 50, 42]], dtype=np.int64)
+    vocab, slots, n_pts = 53, 3, 4
+    row_len = 64 * ((slots * vocab + 2 * n_pts + 2 + 63) // 64)
+    base = slots * vocab
+    Y = np.array([[1.0, 2.0, np.nan, 5.0]])
+    f = mo.features(ops, Y, slots, vocab, n_pts, row_len, bf16=False, std_block=True)
+    yc = np.array([1.0, 2.0, 0.0, 5.0])
+    m, sd = yc.mean(), yc.std()
+    np.testing.assert_allclose(f[0, base + n_pts: base + 2 * n_pts], (yc - m) / sd, rtol=1e-14)
+    np.testing.assert_allclose(f[0, base + 2 * n_pts: base + 2 * n_pts + 2], [np.log1p(m), np.log1p(sd)], rtol=1e-14)
+    flat = mo.features(ops, np.full((1, 4), 3.0), slots, vocab, n_pts, row_len, bf16=False, std_block=True)
+    assert not flat[0, base + n_pts: base + 2 * n_pts].any()
+    spike = np.zeros((1, 200))
+    spike[0, 0] = 1e4
+    g = mo.features(ops, spike, slots, vocab, 200, 64 * ((slots * vocab + 402 + 63) // 64), bf16=False,
+                    std_block=True)
+    assert g[0, base + 200] == 10.0
