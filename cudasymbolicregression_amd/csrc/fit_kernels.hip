@@ -684,18 +684,24 @@ void k_fit(FitArgs a) {
 // Buckets problems by instruction count NI (operator nodes up to the root,
 // see compile_code) and live slot count K: NI <= 8, <= 16, <= 32, <= 64
 // (with K <= 64), and the rest.
+constexpr int NB_CLASSIFY = 5;  // buckets of k_classify (== NBUCKETS below)
+
+// Each wave classifies a run of 64 consecutive problems (one whole wave per
+// program) and appends the run with one atomic per bucket.
 __global__ __launch_bounds__(256) void k_classify(const int64_t *__restrict__ ops, int M, int64_t B, int n_x,
                                                   int32_t *lists, int32_t *counts) {
-    const int64_t b = (int64_t)blockIdx.x * 4 + uniform_i32((int)(threadIdx.x / WAVE));
-    if (b >= B) return;
-    int nc = 0;
-    const int ni = count_instructions_wave(ops + b * M, M, n_x, nc);
-    const int K = nc + (M > nc ? 1 : 0);
-    const int bucket = K > 64 ? 4 : (ni <= 8 ? 0 : (ni <= 16 ? 1 : (ni <= 32 ? 2 : (ni <= 64 ? 3 : 4))));
-    if (lane_id() == 0) {
-        const int slot = atomicAdd(&counts[bucket], 1);
-        lists[(int64_t)bucket * B + slot] = (int32_t)b;
+    const int64_t b0 = ((int64_t)blockIdx.x * 4 + uniform_i32((int)(threadIdx.x / WAVE))) * WAVE;
+    if (b0 >= B) return;
+    const int n = (int)(B - b0 < WAVE ? B - b0 : WAVE);
+    int mine = -1;
+    for (int j = 0; j < n; ++j) {
+        int nc = 0;
+        const int ni = count_instructions_wave(ops + (b0 + j) * M, M, n_x, nc);
+        const int K = nc + (M > nc ? 1 : 0);
+        const int bucket = K > 64 ? 4 : (ni <= 8 ? 0 : (ni <= 16 ? 1 : (ni <= 32 ? 2 : (ni <= 64 ? 3 : 4))));
+        mine = lane_id() == j ? bucket : mine;
     }
+    append_run(mine, NB_CLASSIFY, b0, B, lists, counts);
 }
 
 __global__ void k_fill_i32_f(int32_t *p, int32_t v, int64_t n) {
@@ -736,6 +742,7 @@ int grid_for(int hist) {
 // built-in default.
 constexpr int NXR = 4;
 constexpr int NBUCKETS = 5;
+static_assert(NBUCKETS == NB_CLASSIFY, "k_classify and the launch plan disagree on the buckets");
 constexpr int B4_BLOCKS_PER_CU = 2;  // x 2 waves: 1024 long problems in flight
 
 inline int b4_grid() { return num_cus() * B4_BLOCKS_PER_CU; }
@@ -836,8 +843,8 @@ int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const i
     int32_t *lists = (int32_t *)ws;
     int32_t *counters = (int32_t *)(ws + (size_t)NBUCKETS * B * 4);  // [0..4] counts, [5..9] work
     hipLaunchKernelGGL(k_fill_i32_f, dim3(1), dim3(64), 0, s, counters, 0, (int64_t)(2 * NBUCKETS));
-    hipLaunchKernelGGL(k_classify, dim3((unsigned)((B + 3) / 4)), dim3(256), 0, s, ops, (int)M, B, (int)n_x, lists,
-                       counters);
+    hipLaunchKernelGGL(k_classify, dim3((unsigned)((B + 4 * WAVE - 1) / (4 * WAVE))), dim3(256), 0, s, ops, (int)M,
+                       B, (int)n_x, lists, counters);
     FitArgs a;
     memset(&a, 0, sizeof(a));
     a.X = X; a.Y = Y; a.N = N; a.n_x = (int)n_x; a.ops = ops; a.ch = ch; a.C_init = C_init; a.M = (int)M; a.B = B;

@@ -16,7 +16,7 @@
 // Execution (gfx950): one wave per expression; the program is compiled once
 // per wave into an instruction list (sr_code.h) and evaluated over tile
 // groups of 64*PPT points with the tape and adjoints in LDS.  Expressions are
-// bucketed by instruction count (<= 8, <= 16, <= 64, the rest) so each bucket's
+// bucketed by instruction count (<= 8, <= 16, <= 32, <= 64, the rest) so each bucket's
 // kernel is sized for its tape; the buckets run concurrently on forked
 // streams.  Variable adjoints are summed per block in LDS (float atomics)
 // and written as one partial row per block; two final kernels add the rows
@@ -34,8 +34,12 @@ namespace {
 
 constexpr int F_NCREG = 8;  // constants accumulated in per-lane FP64 registers
 constexpr int F_NX = 4;     // variables held in registers per point
-constexpr int F_NBUCKETS = 4;
+constexpr int F_NBUCKETS = 5;
 constexpr size_t GX_LDS_MAX2 = 24 * 1024;  // largest per-block grad_X partial kept in LDS
+// Expressions per work-queue ticket: short programs are cheap enough that one
+// atomic per expression would serialise the queue; longer ones take one each
+// so that the last expressions spread over all waves.
+__host__ __device__ constexpr int fetch_chunk(int LT) { return LT <= 8 ? 4 : (LT <= 16 ? 2 : 1); }
 
 struct FArgs {
     const void *X;
@@ -99,8 +103,16 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
     const int64_t count = (int64_t)*a.list_count;
     const T *const X0 = X;
 
+    // expressions are fetched CH at a time: one atomic per chunk
+    constexpr int CH = fetch_chunk(LT);
+    int32_t chunk = 0, left = 0;
     for (;;) {
-        const int i = wave_fetch_add(a.work_counter);
+        if (left == 0) {
+            chunk = wave_fetch_add_n(a.work_counter, CH);
+            left = CH;
+        }
+        const int i = chunk + (CH - left);
+        --left;
         if (i >= count) break;
         const int64_t b = (int64_t)a.list[i];
         const CProg<T> p = load_cprog<T>(a.ops, a.ch, (const T *)a.C, b, a.M, n_x);
@@ -339,17 +351,21 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
 
 // Buckets by instruction count NI and constant count nc: NI <= 8, <= 16,
 // <= 64 (all with nc <= 64), the rest.  One wave per expression.
+// Each wave classifies a run of 64 consecutive expressions (one whole wave
+// per program) and appends the run with one atomic per bucket.
 __global__ __launch_bounds__(256) void k_classify_fused(const int64_t *__restrict__ ops, int M, int64_t B, int n_x,
                                                         int32_t *lists, int32_t *counts) {
-    const int64_t b = (int64_t)blockIdx.x * 4 + uniform_i32((int)(threadIdx.x / WAVE));
-    if (b >= B) return;
-    int nc = 0;
-    const int ni = count_instructions_wave(ops + b * M, M, n_x, nc);
-    const int bucket = nc > 64 ? 3 : (ni <= 8 ? 0 : (ni <= 16 ? 1 : (ni <= 64 ? 2 : 3)));
-    if (lane_id() == 0) {
-        const int slot = atomicAdd(&counts[bucket], 1);
-        lists[(int64_t)bucket * B + slot] = (int32_t)b;
+    const int64_t b0 = ((int64_t)blockIdx.x * 4 + uniform_i32((int)(threadIdx.x / WAVE))) * WAVE;
+    if (b0 >= B) return;
+    const int n = (int)(B - b0 < WAVE ? B - b0 : WAVE);
+    int mine = -1;
+    for (int j = 0; j < n; ++j) {
+        int nc = 0;
+        const int ni = count_instructions_wave(ops + (b0 + j) * M, M, n_x, nc);
+        const int bucket = nc > 64 ? 4 : (ni <= 8 ? 0 : (ni <= 16 ? 1 : (ni <= 32 ? 2 : (ni <= 64 ? 3 : 4))));
+        mine = lane_id() == j ? bucket : mine;
     }
+    append_run(mine, F_NBUCKETS, b0, B, lists, counts);
 }
 
 __global__ void k_fill_i32_e(int32_t *p, int32_t v, int64_t n) {
@@ -426,16 +442,17 @@ int fused_launch(const FArgs &a, int grid, bool gx_lds, hipStream_t s) {
 
 #define SR_F0(T) T, 8, 1, FCfg<T>::B0_PPT, 4, false
 #define SR_F1(T) T, 16, 1, FCfg<T>::B0_PPT, 4, false
-#define SR_F2(T) T, 64, 1, 1, 2, false
-#define SR_F3(T) T, 128, 2, 1, 2, true
+#define SR_F2(T) T, 32, 1, 1, 2, false
+#define SR_F3(T) T, 64, 1, 1, 2, false
+#define SR_F4(T) T, 128, 2, 1, 2, true
 
-constexpr int F3_BLOCKS_PER_CU = 2;
+constexpr int F4_BLOCKS_PER_CU = 2;
 
 // Workspace: lists [4][B] int32, counters [4]+[4] | grad_X partial rows |
 // FP64 chunk sums of the rows | global tape (bucket 3).
 struct FWs {
     size_t gx_off, g64_off, gt_off, total;
-    int g0, g1, g2, g3;
+    int g0, g1, g2, g3, g4;
     bool gx_lds;
 };
 template <typename T>
@@ -446,8 +463,9 @@ FWs fused_layout(int64_t N, int64_t n_x, int64_t B, int64_t M, bool want_gx) {
     w.g0 = fused_grid<SR_F0(T)>(w.gx_lds, gx_len, (int)n_x);
     w.g1 = M > 8 ? fused_grid<SR_F1(T)>(w.gx_lds, gx_len, (int)n_x) : 0;
     w.g2 = M > 16 ? fused_grid<SR_F2(T)>(w.gx_lds, gx_len, (int)n_x) : 0;
-    w.g3 = M > 64 ? num_cus() * F3_BLOCKS_PER_CU : 0;
-    const int64_t rows = (int64_t)w.g0 + w.g1 + w.g2 + w.g3;
+    w.g3 = M > 32 ? fused_grid<SR_F3(T)>(w.gx_lds, gx_len, (int)n_x) : 0;
+    w.g4 = M > 64 ? num_cus() * F4_BLOCKS_PER_CU : 0;
+    const int64_t rows = (int64_t)w.g0 + w.g1 + w.g2 + w.g3 + w.g4;
     size_t off = (((size_t)F_NBUCKETS * B * 4 + 64) + 255) / 256 * 256;
     w.gx_off = off;
     if (w.gx_lds) off += (size_t)rows * gx_len * sizeof(T);
@@ -456,7 +474,7 @@ FWs fused_layout(int64_t N, int64_t n_x, int64_t B, int64_t M, bool want_gx) {
     if (w.gx_lds) off += (size_t)((rows + ROW_CHUNK - 1) / ROW_CHUNK) * gx_len * sizeof(double);
     off = (off + 255) / 256 * 256;
     w.gt_off = off;
-    if (w.g3) off += (size_t)w.g3 * 2 * tape_elems(128, 1) * sizeof(T);
+    if (w.g4) off += (size_t)w.g4 * 2 * tape_elems(128, 1) * sizeof(T);
     w.total = off + 256;
     return w;
 }
@@ -498,10 +516,10 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
     }
     char *ws = (char *)workspace;
     int32_t *lists = (int32_t *)ws;
-    int32_t *counters = (int32_t *)(ws + (size_t)F_NBUCKETS * B * 4);  // [0..2] counts, [3..5] work
+    int32_t *counters = (int32_t *)(ws + (size_t)F_NBUCKETS * B * 4);  // [0..4] counts, [5..9] work
     hipLaunchKernelGGL(k_fill_i32_e, dim3(1), dim3(64), 0, s, counters, 0, (int64_t)(2 * F_NBUCKETS));
-    hipLaunchKernelGGL(k_classify_fused, dim3((unsigned)((B + 3) / 4)), dim3(256), 0, s, ops, (int)M, B, (int)n_x,
-                       lists, counters);
+    hipLaunchKernelGGL(k_classify_fused, dim3((unsigned)((B + 4 * WAVE - 1) / (4 * WAVE))), dim3(256), 0, s, ops,
+                       (int)M, B, (int)n_x, lists, counters);
     FArgs a;
     memset(&a, 0, sizeof(a));
     a.X = X; a.N = N; a.n_x = (int)n_x; a.ops = ops; a.ch = ch; a.C = C; a.M = (int)M; a.B = B; a.dy = dy;
@@ -509,7 +527,7 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
     a.gx_partial = L.gx_lds ? (void *)(ws + L.gx_off) : nullptr;
     a.grad_X_direct = (want_gx && !L.gx_lds) ? grad_X : nullptr;
     a.err_b = err_b; a.flags = flags;
-    a.gtape = L.g3 ? (void *)(ws + L.gt_off) : nullptr;
+    a.gtape = L.g4 ? (void *)(ws + L.gt_off) : nullptr;
     a.mode = mode; a.fwd_only = fwd_only; a.want_gx = want_gx ? 1 : 0;
     a.x_off = x_off; a.y_off = y_off; a.n_pts = n_pts;
     auto bucket_args = [&](int k, int64_t row0) {
@@ -525,24 +543,25 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
     if (M <= 8) {
         if ((r = fused_launch<SR_F0(T)>(bucket_args(0, 0), L.g0, L.gx_lds, s))) return r;
     } else {
-        // buckets 3+2 and 1 on side streams, bucket 0 on the caller's stream
+        // buckets 4+3, 2 and 1 on side streams, bucket 0 on the caller's stream
         Fork &f = fork_for_device();
         if (!f.ok) return (int)hipErrorOutOfMemory;
         if ((r = (int)hipEventRecord(f.ev_fork, s))) return r;
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 3; ++i)
             if ((r = (int)hipStreamWaitEvent(f.side[i], f.ev_fork, 0))) return r;
-        if (L.g3 && (r = fused_launch<SR_F3(T)>(bucket_args(3, L.g0 + L.g1 + L.g2), L.g3, L.gx_lds, f.side[0])))
-            return r;
-        if (L.g2 && (r = fused_launch<SR_F2(T)>(bucket_args(2, L.g0 + L.g1), L.g2, L.gx_lds, f.side[0]))) return r;
-        if ((r = fused_launch<SR_F1(T)>(bucket_args(1, L.g0), L.g1, L.gx_lds, f.side[1]))) return r;
+        const int64_t r3 = (int64_t)L.g0 + L.g1 + L.g2;
+        if (L.g4 && (r = fused_launch<SR_F4(T)>(bucket_args(4, r3 + L.g3), L.g4, L.gx_lds, f.side[0]))) return r;
+        if (L.g3 && (r = fused_launch<SR_F3(T)>(bucket_args(3, r3), L.g3, L.gx_lds, f.side[0]))) return r;
+        if (L.g2 && (r = fused_launch<SR_F2(T)>(bucket_args(2, L.g0 + L.g1), L.g2, L.gx_lds, f.side[1]))) return r;
+        if ((r = fused_launch<SR_F1(T)>(bucket_args(1, L.g0), L.g1, L.gx_lds, f.side[2]))) return r;
         if ((r = fused_launch<SR_F0(T)>(bucket_args(0, 0), L.g0, L.gx_lds, s))) return r;
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < 3; ++i) {
             if ((r = (int)hipEventRecord(f.ev_join[i], f.side[i]))) return r;
             if ((r = (int)hipStreamWaitEvent(s, f.ev_join[i], 0))) return r;
         }
     }
     if (L.gx_lds) {
-        const int64_t rows = (int64_t)L.g0 + L.g1 + L.g2 + L.g3;
+        const int64_t rows = (int64_t)L.g0 + L.g1 + L.g2 + L.g3 + L.g4;
         const int64_t chunks = (rows + ROW_CHUNK - 1) / ROW_CHUNK;
         const unsigned nb = (unsigned)((gx_len + 255) / 256);
         hipLaunchKernelGGL(k_sum_rows_1<T>, dim3(nb, (unsigned)chunks), dim3(256), 0, s, (const T *)(ws + L.gx_off),
