@@ -42,6 +42,7 @@
 //     its bucket; the buckets run concurrently on forked streams.
 #include "sr_code.h"
 #include "sr_device_info.h"
+#include "sr_queue.h"
 #include "sr_streams.h"
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -687,9 +688,9 @@ void k_fit(FitArgs a) {
 constexpr int NB_CLASSIFY = 5;  // buckets of k_classify (== NBUCKETS below)
 
 // Each wave classifies a run of 64 consecutive problems (one whole wave per
-// program) and appends the run with one atomic per bucket.
+// program) and records it for the deterministic bucket lists (sr_queue.h).
 __global__ __launch_bounds__(256) void k_classify(const int64_t *__restrict__ ops, int M, int64_t B, int n_x,
-                                                  int32_t *lists, int32_t *counts) {
+                                                  int8_t *bucket_of, int32_t *runcnt) {
     const int64_t b0 = ((int64_t)blockIdx.x * 4 + uniform_i32((int)(threadIdx.x / WAVE))) * WAVE;
     if (b0 >= B) return;
     const int n = (int)(B - b0 < WAVE ? B - b0 : WAVE);
@@ -701,7 +702,7 @@ __global__ __launch_bounds__(256) void k_classify(const int64_t *__restrict__ op
         const int bucket = K > 64 ? 4 : (ni <= 8 ? 0 : (ni <= 16 ? 1 : (ni <= 32 ? 2 : (ni <= 64 ? 3 : 4))));
         mine = lane_id() == j ? bucket : mine;
     }
-    append_run(mine, NB_CLASSIFY, b0, B, lists, counts);
+    queue_record_run(mine, NB_CLASSIFY, b0, n, bucket_of, runcnt);
 }
 
 __global__ void k_fill_i32_f(int32_t *p, int32_t v, int64_t n) {
@@ -788,8 +789,10 @@ inline int launch_bucket(int k, const FitArgs &a, hipStream_t s) {
     return r == -3 ? kVariants[k][0].launch(a, s) : r;
 }
 
-// Workspace: lists [5][B] int32 | counters [5] counts + [5] work | global tape (bucket 4).
-inline size_t fit_gt_off(int64_t B) { return (((size_t)NBUCKETS * B * 4 + 64) + 255) / 256 * 256; }
+// Workspace: lists [5][B] int32 | counters [5] counts + [5] work | queue
+// classification (sr_queue.h) | global tape (bucket 4).
+inline size_t fit_q_off(int64_t B) { return (((size_t)NBUCKETS * B * 4 + 64) + 255) / 256 * 256; }
+inline size_t fit_gt_off(int64_t B) { return fit_q_off(B) + (queue_ws_bytes(B, NBUCKETS) + 255) / 256 * 256; }
 inline int64_t fit_ws(int64_t B, int64_t M) {
     size_t off = fit_gt_off(B);
     if (M > 64)  // more than 64 instructions or live slots needs more than 64 nodes
@@ -843,8 +846,10 @@ int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const i
     int32_t *lists = (int32_t *)ws;
     int32_t *counters = (int32_t *)(ws + (size_t)NBUCKETS * B * 4);  // [0..4] counts, [5..9] work
     hipLaunchKernelGGL(k_fill_i32_f, dim3(1), dim3(64), 0, s, counters, 0, (int64_t)(2 * NBUCKETS));
+    void *qws = ws + fit_q_off(B);
     hipLaunchKernelGGL(k_classify, dim3((unsigned)((B + 4 * WAVE - 1) / (4 * WAVE))), dim3(256), 0, s, ops, (int)M,
-                       B, (int)n_x, lists, counters);
+                       B, (int)n_x, queue_bucket_of(qws), queue_runcnt(qws, B));
+    queue_finish(qws, B, NBUCKETS, lists, counters, s);
     FitArgs a;
     memset(&a, 0, sizeof(a));
     a.X = X; a.Y = Y; a.N = N; a.n_x = (int)n_x; a.ops = ops; a.ch = ch; a.C_init = C_init; a.M = (int)M; a.B = B;

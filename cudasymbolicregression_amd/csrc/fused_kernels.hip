@@ -23,6 +23,7 @@
 // in FP64 in a fixed order.
 #include "sr_code.h"
 #include "sr_device_info.h"
+#include "sr_queue.h"
 #include "sr_streams.h"
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -352,9 +353,10 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
 // Buckets by instruction count NI and constant count nc: NI <= 8, <= 16,
 // <= 64 (all with nc <= 64), the rest.  One wave per expression.
 // Each wave classifies a run of 64 consecutive expressions (one whole wave
-// per program) and appends the run with one atomic per bucket.
+// per program) and records it for the deterministic bucket lists
+// (sr_queue.h).
 __global__ __launch_bounds__(256) void k_classify_fused(const int64_t *__restrict__ ops, int M, int64_t B, int n_x,
-                                                        int32_t *lists, int32_t *counts) {
+                                                        int8_t *bucket_of, int32_t *runcnt) {
     const int64_t b0 = ((int64_t)blockIdx.x * 4 + uniform_i32((int)(threadIdx.x / WAVE))) * WAVE;
     if (b0 >= B) return;
     const int n = (int)(B - b0 < WAVE ? B - b0 : WAVE);
@@ -365,7 +367,7 @@ __global__ __launch_bounds__(256) void k_classify_fused(const int64_t *__restric
         const int bucket = nc > 64 ? 4 : (ni <= 8 ? 0 : (ni <= 16 ? 1 : (ni <= 32 ? 2 : (ni <= 64 ? 3 : 4))));
         mine = lane_id() == j ? bucket : mine;
     }
-    append_run(mine, F_NBUCKETS, b0, B, lists, counts);
+    queue_record_run(mine, F_NBUCKETS, b0, n, bucket_of, runcnt);
 }
 
 __global__ void k_fill_i32_e(int32_t *p, int32_t v, int64_t n) {
@@ -451,7 +453,7 @@ constexpr int F4_BLOCKS_PER_CU = 2;
 // Workspace: lists [4][B] int32, counters [4]+[4] | grad_X partial rows |
 // FP64 chunk sums of the rows | global tape (bucket 3).
 struct FWs {
-    size_t gx_off, g64_off, gt_off, total;
+    size_t q_off, gx_off, g64_off, gt_off, total;
     int g0, g1, g2, g3, g4;
     bool gx_lds;
 };
@@ -467,6 +469,8 @@ FWs fused_layout(int64_t N, int64_t n_x, int64_t B, int64_t M, bool want_gx) {
     w.g4 = M > 64 ? num_cus() * F4_BLOCKS_PER_CU : 0;
     const int64_t rows = (int64_t)w.g0 + w.g1 + w.g2 + w.g3 + w.g4;
     size_t off = (((size_t)F_NBUCKETS * B * 4 + 64) + 255) / 256 * 256;
+    w.q_off = off;
+    off += (queue_ws_bytes(B, F_NBUCKETS) + 255) / 256 * 256;
     w.gx_off = off;
     if (w.gx_lds) off += (size_t)rows * gx_len * sizeof(T);
     off = (off + 255) / 256 * 256;
@@ -518,8 +522,10 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
     int32_t *lists = (int32_t *)ws;
     int32_t *counters = (int32_t *)(ws + (size_t)F_NBUCKETS * B * 4);  // [0..4] counts, [5..9] work
     hipLaunchKernelGGL(k_fill_i32_e, dim3(1), dim3(64), 0, s, counters, 0, (int64_t)(2 * F_NBUCKETS));
+    void *qws = ws + L.q_off;
     hipLaunchKernelGGL(k_classify_fused, dim3((unsigned)((B + 4 * WAVE - 1) / (4 * WAVE))), dim3(256), 0, s, ops,
-                       (int)M, B, (int)n_x, lists, counters);
+                       (int)M, B, (int)n_x, queue_bucket_of(qws), queue_runcnt(qws, B));
+    queue_finish(qws, B, F_NBUCKETS, lists, counters, s);
     FArgs a;
     memset(&a, 0, sizeof(a));
     a.X = X; a.N = N; a.n_x = (int)n_x; a.ops = ops; a.ch = ch; a.C = C; a.M = (int)M; a.B = B; a.dy = dy;

@@ -62,22 +62,6 @@ __device__ inline int32_t wave_fetch_add_n(int32_t *counter, int32_t n) {
     return __builtin_amdgcn_readlane(got, 0);
 }
 
-// Appends a run of up to 64 consecutive items b0 + lane (lane j holds the
-// bucket of item b0 + j, or -1) to per-bucket lists (lists[k * B + ...]):
-// one atomic per bucket present in the run instead of one per item.  Within
-// a bucket, a run's items keep their order.
-__device__ inline void append_run(int my_bucket, int nbuckets, int64_t b0, int64_t B, int32_t *lists,
-                                  int32_t *counts) {
-    const int lane = (int)(threadIdx.x & (64 - 1));
-    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    for (int k = 0; k < nbuckets; ++k) {
-        const uint64_t mask = __ballot(my_bucket == k);
-        if (mask == 0ull) continue;
-        const int32_t base = wave_fetch_add_n(&counts[k], __popcll(mask));
-        if (my_bucket == k) lists[(int64_t)k * B + base + __popcll(mask & below)] = (int32_t)(b0 + lane);
-    }
-}
-
 // Butterfly sum: every lane computes the same additions in a commuted order,
 // so all lanes hold the bit-identical total; it is returned as a uniform.
 __device__ inline double wave_sum(double v) {
