@@ -253,7 +253,7 @@ def timed(fn, steps: int, warmup: int):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--problems", type=int, default=16384)
     ap.add_argument("--seed", type=int, default=5, help="test problems (training uses seed - 1)")

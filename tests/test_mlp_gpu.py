@@ -77,9 +77,21 @@ def test_gemm_nt_matches_fp64(M, N, K, epi):
 
 def test_gemm_rejects_unaligned_shapes():
     L = _lib()
+    a = torch.zeros(64 * 64 + 8, dtype=torch.bfloat16, device=_dev())
+    out = torch.empty(64 * 64, dtype=torch.float32, device=_dev())
+    ok = (L.ptr(a), L.ptr(a), None, 32, 32, 16, 16, 16, 0, L.ptr(out), None, None, _s())
+    assert L.hip().sr_mlp_gemm_nt(*ok) == 0
     for M, N, K, lda in [(33, 32, 16, 16), (32, 40, 16, 16), (32, 32, 24, 24), (0, 32, 16, 16), (32, 32, 16, 12),
                          (32, 32, 16, 8)]:
-        assert L.hip().sr_mlp_gemm_nt(None, None, None, M, N, K, lda, K, 0, None, None, None, _s()) == -1
+        assert L.hip().sr_mlp_gemm_nt(L.ptr(a), L.ptr(a), None, M, N, K, lda, K, 0, L.ptr(out), None, None,
+                                      _s()) == -1
+    # misaligned operand, no output, missing bias for a bias epilogue
+    assert L.hip().sr_mlp_gemm_nt(L.ptr(a) + 2, L.ptr(a), None, 32, 32, 16, 16, 16, 0, L.ptr(out), None, None,
+                                  _s()) == -1
+    assert L.hip().sr_mlp_gemm_nt(L.ptr(a), L.ptr(a), None, 32, 32, 16, 16, 16, 0, None, None, None, _s()) == -1
+    assert L.hip().sr_mlp_gemm_nt(L.ptr(a), L.ptr(a), None, 32, 32, 16, 16, 16, 1, L.ptr(out), None, None,
+                                  _s()) == -1
+    torch.cuda.synchronize()
 
 
 def test_gemm_strided_operands():
@@ -170,16 +182,16 @@ def _problems(P, seed, n_pts=512):
     return g, X, ops, ch, Ct, Y
 
 
-@pytest.mark.parametrize("B", [64, 50])
-def test_loss_and_grads_match_fp64_oracle(B):
+@pytest.mark.parametrize("B,std", [(64, False), (50, False), (50, True)])
+def test_loss_and_grads_match_fp64_oracle(B, std):
     from cudasymbolicregression_amd import warmstart
 
     _, _, ops, _, Ct, Y = _problems(B, seed=11)
-    model = warmstart.WarmStartMLP(_dev(), seed=3)
+    model = warmstart.WarmStartMLP(_dev(), seed=3, std_features=std)
     # larger biases than the default init so every GELU region is visited
     for b in model.b:
         b.normal_(0.0, 0.5)
-    print(f"mlp loss_and_grads B={B}", flush=True)
+    print(f"mlp loss_and_grads B={B} std_features={std}", flush=True)
     rows = model.loss_and_grads(ops, Y, Ct)
     torch.cuda.synchronize()
     params = [(_np(W), _np(b)) for W, b in zip(model.W, model.b)]
@@ -201,12 +213,13 @@ def test_loss_and_grads_match_fp64_oracle(B):
             assert np.max(np.abs(got - want)) <= 2e-2 * np.max(np.abs(want))
 
 
-def test_features_predict_and_select_start():
+@pytest.mark.parametrize("std", [False, True])
+def test_features_predict_and_select_start(std):
     from cudasymbolicregression_amd import _lib as L
     from cudasymbolicregression_amd import warmstart
 
     _, _, ops, _, _, Y = _problems(40, seed=12)
-    model = warmstart.WarmStartMLP(_dev(), seed=1)
+    model = warmstart.WarmStartMLP(_dev(), seed=1, std_features=std)
     B, M = ops.shape
     Bp = 64
     xb = torch.empty(Bp, model.row_len, dtype=torch.bfloat16, device=_dev())
