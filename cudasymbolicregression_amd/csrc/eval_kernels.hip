@@ -89,6 +89,60 @@ __global__ __launch_bounds__(64) void k_full_forward(const T *__restrict__ X, in
 }
 
 // ---------------------------------------------------------------------------
+// Full-cache forward with coalesced cache writes.  The cache (M,N,B[,K]) has
+// the expressions innermost, so one wave writing one expression's points
+// touches a different cache line per lane.  Here a block of EW waves
+// evaluates EW consecutive expressions e (= b*K + j) over one 64-point tile,
+// each into its own LDS tape, and then the whole block writes node row by
+// node row with the expression index fastest: every group of EW threads
+// stores EW contiguous values.  Blocks are XCD-remapped so that consecutive
+// expression groups of one tile run on the same XCD and fill whole L2 lines
+// together.  Tapes are EW apart by M*64 + 1 elements (conflict-free
+// transposed reads).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(512) void k_full_forward_grp(const T *__restrict__ X, int64_t N, int n_x,
+                                                         const int64_t *__restrict__ ops,
+                                                         const int64_t *__restrict__ ch, const T *__restrict__ C,
+                                                         int M, int64_t B, int64_t K, int64_t n_tiles, int EW,
+                                                         T *__restrict__ cache) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int Ls[8];
+    const int wave = uniform_i32((int)(threadIdx.x / WAVE));
+    const int lane = lane_id();
+    const int64_t E = B * K;
+    const int64_t groups = (E + EW - 1) / EW;
+    const int64_t id = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
+    const int64_t tile = id / groups, g = id % groups;
+    if (tile >= n_tiles) return;  // whole block
+    const int64_t e0 = g * EW;
+    const size_t tstride = (size_t)M * WAVE + 1;
+    T *tape = reinterpret_cast<T *>(smem) + (size_t)wave * tstride;
+    {
+        const int64_t e = e0 + wave;
+        int L = 0;
+        if (e < E) {
+            const int64_t b = e / K, j = e % K;
+            Program<T> p = load_program<T>(ops, ch, C + j, b, 1, B, K, B * K, M, n_x);
+            const int64_t n = tile * TILE + lane;
+            PointCtx<T> pc = load_point<T>(X, n, n < N, n_x);
+            forward_tile<T>(p, pc, tape, p.L);
+            L = p.L;
+        }
+        if (lane == 0) Ls[wave] = L;
+    }
+    __syncthreads();
+    const int t = (int)threadIdx.x;
+    const int el = t % EW, nl = t / EW;  // nl < 64 since blockDim = 64 * EW
+    const int64_t e = e0 + el, n = tile * TILE + nl;
+    if (e >= E || n >= N) return;
+    const T *src = reinterpret_cast<const T *>(smem) + (size_t)el * tstride + nl;
+    const int L = Ls[el];
+    const T nan = Mth<T>::qnan();
+    for (int k = 0; k < M; ++k) cache[((int64_t)k * N + n) * E + e] = k < L ? src[(size_t)k * WAVE] : nan;
+}
+
+// ---------------------------------------------------------------------------
 // Full-cache backward.  Block = 4 waves = 4 consecutive expressions, looping
 // over all point tiles together.  Children values come from the saved cache;
 // the adjoints are accumulated in place in grad_cache, the caller's copy of
@@ -163,6 +217,129 @@ __global__ __launch_bounds__(BWD_WAVES * 64) void k_full_backward(
     if (lane == 0 && err) atomicMax(err_flag, err);
 }
 
+// Full-cache backward with coalesced reads: the block's BWD_WAVES
+// consecutive expressions load each 64-point tile of their upstream
+// gradients (all M rows; rows past an expression's length must be zero, else
+// error 201) and of their cached values (rows < L) into LDS with the
+// expression index fastest, then every wave runs its reverse sweep on its
+// LDS tapes.  The adjoints stay in LDS: grad_cache is the caller's scratch
+// copy of the upstream gradient and is only read.  Outputs and reductions
+// as in k_full_backward.
+constexpr int BWD_BATCH = 16;
+
+template <typename T>
+__global__ __launch_bounds__(BWD_WAVES * 64) void k_full_backward_grp(
+    const T *__restrict__ grad_cache, const T *__restrict__ cache, const int64_t *__restrict__ ops,
+    const int64_t *__restrict__ ch, int M, int64_t B, int64_t K, int64_t N, int n_x, T *__restrict__ grad_C,
+    T *__restrict__ gx_partial, T *__restrict__ grad_X_direct, int32_t *err_flag) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int Ls[BWD_WAVES];
+    const int wave = uniform_i32((int)(threadIdx.x / WAVE));
+    const int lane = lane_id();
+    const size_t tstride = (size_t)M * WAVE + 1;
+    T *vals = reinterpret_cast<T *>(smem);
+    T *adjs = vals + (size_t)BWD_WAVES * tstride;
+    double *cextra = reinterpret_cast<double *>(adjs + (size_t)BWD_WAVES * tstride +
+                                                 ((size_t)BWD_WAVES * tstride * sizeof(T) % 8 ? 1 : 0)) +
+                     (size_t)wave * M;
+    T *gxs = reinterpret_cast<T *>(reinterpret_cast<double *>(adjs + (size_t)BWD_WAVES * tstride +
+                                                              ((size_t)BWD_WAVES * tstride * sizeof(T) % 8 ? 1 : 0)) +
+                                   (size_t)BWD_WAVES * M);  // [4][64][NXREG]
+    const int64_t E = B * K;
+    // XCD-remapped so that neighbouring expression groups share one XCD's L2
+    const int64_t blk = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
+    const int64_t e0 = blk * BWD_WAVES;
+    const int64_t e_id = e0 + wave;
+    const bool bvalid = e_id < E;
+    const int64_t b = bvalid ? e_id / K : 0, j = bvalid ? e_id % K : 0;
+    const bool want_gx = gx_partial != nullptr || grad_X_direct != nullptr;
+    Program<T> p;
+    if (bvalid) p = load_program<T>(ops, ch, (const T *)nullptr, b, 1, B, 1, B, M, n_x);
+    else { p.L = 0; p.w0 = p.w1 = 0; p.cmask0 = p.cmask1 = 0; p.root = 0; p.verr = 0; p.c0 = p.c1 = T(0); }
+    for (int k = lane; k < M; k += WAVE) cextra[k] = 0.0;
+    if (lane == 0) Ls[wave] = p.L;
+    GradAcc<T> acc;
+#pragma unroll
+    for (int q = 0; q < NCREG; ++q) acc.cacc[q] = 0.0;
+    int32_t err = 0, lerr = 0;
+    T *vt = vals + (size_t)wave * tstride;
+    T *at = adjs + (size_t)wave * tstride;
+    const int t = (int)threadIdx.x;
+    const int el = t % BWD_WAVES, nl = t / BWD_WAVES;  // nl < 64
+    const int64_t el_e = e0 + el;
+    const int64_t n_tiles = (N + TILE - 1) / TILE;
+    __syncthreads();
+    const int Lel = Ls[el];
+    for (int64_t tt = 0; tt < n_tiles; ++tt) {
+        // cooperative load, expression index fastest
+        {
+            const int64_t n = tt * TILE + nl;
+            const bool ok = el_e < E && n < N;
+            T *vd = vals + (size_t)el * tstride + nl;
+            T *ad = adjs + (size_t)el * tstride + nl;
+            // BWD_BATCH rows of loads in flight before they are used
+            for (int k0 = 0; k0 < M; k0 += BWD_BATCH) {
+                T g[BWD_BATCH], v[BWD_BATCH];
+#pragma unroll
+                for (int u = 0; u < BWD_BATCH; ++u) {
+                    const int k = k0 + u;
+                    const int64_t off = ((int64_t)k * N + n) * E + el_e;
+                    g[u] = (ok && k < M) ? grad_cache[off] : T(0);
+                    v[u] = (ok && k < Lel) ? cache[off] : T(0);
+                }
+#pragma unroll
+                for (int u = 0; u < BWD_BATCH; ++u) {
+                    const int k = k0 + u;
+                    if (k < Lel) {
+                        ad[(size_t)k * WAVE] = g[u];
+                        vd[(size_t)k * WAVE] = v[u];
+                    } else if (g[u] != T(0)) {
+                        lerr = ERR_GRAD_NO_OP;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const int64_t n = tt * TILE + lane;
+        const bool active = bvalid && n < N;
+        const int64_t nn = active ? n : 0;
+#pragma unroll
+        for (int q = 0; q < NXREG; ++q) acc.gxl[q] = T(0);
+        auto val = [&](int k) -> T { return vt[(size_t)k * WAVE + lane]; };
+        auto adj = [&](int k) -> T & { return at[(size_t)k * WAVE + lane]; };
+        int32_t e = backward_tile<T, true>(p, val, adj, p.L, active, acc, cextra, n_x,
+                                           grad_X_direct ? grad_X_direct + nn * n_x : nullptr);
+        err = e > err ? e : err;
+        if (want_gx && n_x <= NXREG) {
+#pragma unroll
+            for (int q = 0; q < NXREG; ++q) gxs[(wave * WAVE + lane) * NXREG + q] = acc.gxl[q];
+            __syncthreads();
+            if (wave == 0 && n < N) {
+                for (int v = 0; v < n_x; ++v) {
+                    T s = gxs[(0 * WAVE + lane) * NXREG + v];
+                    for (int w2 = 1; w2 < BWD_WAVES; ++w2) s += gxs[(w2 * WAVE + lane) * NXREG + v];
+                    gx_partial[blk * N * n_x + n * n_x + v] = s;
+                }
+            }
+        }
+        __syncthreads();  // LDS tapes are reloaded for the next tile
+    }
+    if (lerr) atomicMax(err_flag, lerr);
+    if (!bvalid) return;
+    double lo, hi;
+    finish_constants<T>(p, acc, cextra, lo, hi);
+    if (lane < M) grad_C[((int64_t)lane * B + b) * K + j] = (T)lo;
+    if (lane + WAVE < M) grad_C[((int64_t)(lane + WAVE) * B + b) * K + j] = (T)hi;
+    err = wave_max_i32(err);
+    if (lane == 0 && err) atomicMax(err_flag, err);
+}
+
+template <typename T>
+inline size_t backward_grp_lds(int64_t M) {
+    const size_t tapes = 2 * (size_t)BWD_WAVES * ((size_t)M * WAVE + 1) * sizeof(T);
+    return (tapes + 8) + (size_t)BWD_WAVES * M * sizeof(double) + (size_t)BWD_WAVES * WAVE * NXREG * sizeof(T);
+}
+
 // Sums P partial rows of length len in FP64, in row order.
 template <typename T>
 __global__ void k_reduce_partials(const T *__restrict__ part, int64_t P, int64_t len, T *__restrict__ out,
@@ -184,6 +361,17 @@ int forward_mb(const T *X, int64_t N, int64_t n_x, const int64_t *ops, const int
     int64_t n_tiles = (N + TILE - 1) / TILE;
     int64_t nblk = n_tiles * B * K;
     if (nblk > 0x7fffffffLL) return -1;
+    // grouped kernel: the most expressions per block whose tapes fit in LDS
+    for (int EW = 8; EW >= 2; EW /= 2) {
+        const size_t lds = (size_t)EW * ((size_t)M * WAVE + 1) * sizeof(T);
+        if (lds > lds_limit_device()) continue;
+        const int64_t groups = (B * K + EW - 1) / EW;
+        const int64_t nb = n_tiles * groups;
+        if (nb > 0x7fffffffLL) break;
+        hipLaunchKernelGGL(k_full_forward_grp<T>, dim3((unsigned)nb), dim3(64 * EW), lds, s, X, N, (int)n_x, ops, ch,
+                           C, (int)M, B, K, n_tiles, EW, cache);
+        return ret_last();
+    }
     size_t lds = (size_t)M * WAVE * sizeof(T);
     if (lds > lds_limit()) return -3;
     hipLaunchKernelGGL(k_full_forward<T>, dim3((unsigned)nblk), dim3(64), lds, s, X, N, (int)n_x, ops, ch, C,
@@ -233,11 +421,17 @@ int backward_mb(T *grad_out, const T *cache, const int64_t *ops, const int64_t *
         if (ws_bytes < backward_mb_ws_bytes<T>(B, N, n_x, K)) return -2;
         gx_partial = (T *)workspace;
     }
-    size_t lds = (size_t)BWD_WAVES * M * sizeof(double) + (size_t)BWD_WAVES * WAVE * NXREG * sizeof(T);
-    if (lds > lds_limit()) return -3;
-    hipLaunchKernelGGL(k_full_backward<T>, dim3((unsigned)nblk), dim3(BWD_WAVES * 64), lds, s, grad_out, cache, ops,
-                       ch, (int)M, B, K, N, (int)n_x, grad_C, gx_partial, (grad_X && !gx_regs) ? grad_X : nullptr,
-                       d_err);
+    if (backward_grp_lds<T>(M) <= lds_limit_device()) {
+        hipLaunchKernelGGL(k_full_backward_grp<T>, dim3((unsigned)nblk), dim3(BWD_WAVES * 64), backward_grp_lds<T>(M),
+                           s, grad_out, cache, ops, ch, (int)M, B, K, N, (int)n_x, grad_C, gx_partial,
+                           (grad_X && !gx_regs) ? grad_X : nullptr, d_err);
+    } else {
+        size_t lds = (size_t)BWD_WAVES * M * sizeof(double) + (size_t)BWD_WAVES * WAVE * NXREG * sizeof(T);
+        if (lds > lds_limit()) return -3;
+        hipLaunchKernelGGL(k_full_backward<T>, dim3((unsigned)nblk), dim3(BWD_WAVES * 64), lds, s, grad_out, cache,
+                           ops, ch, (int)M, B, K, N, (int)n_x, grad_C, gx_partial,
+                           (grad_X && !gx_regs) ? grad_X : nullptr, d_err);
+    }
     if (gx_partial) {
         int64_t len = N * n_x;
         hipLaunchKernelGGL(k_reduce_partials<T>, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s,
