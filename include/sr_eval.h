@@ -52,7 +52,8 @@ int sr_forward_mb_f64(const double *X, int64_t N, int64_t n_x, const int64_t *op
                       const double *C, int64_t M, int64_t B, double *cache, void *stream);
 
 /* Full-cache backward: grad_cache (M,N,B) holds the upstream gradient on
- * entry and the node adjoints on exit; writes grad_C (M,B) and grad_X
+ * entry; its contents on exit are unspecified (scratch: for M <= 64 in FP32
+ * it is only read, the adjoints live in LDS); writes grad_C (M,B) and grad_X
  * (N,n_x, nullable).  Replaces SymbolicEvaluation::backward + the backward
  * kernels (src/symbolic_evaluation.cpp:94-168,
  * src/evaluation_kernels_cuda.cu:218-632; CPU spec
