@@ -192,9 +192,12 @@ class _SymbolicEvaluation(torch.autograd.Function):
             )
         M, B = Ops.shape
         N, n_x = X.shape
-        # grad_cache: a private copy of the upstream gradient that the kernel
-        # turns into the adjoints of every node in place.
-        g = grad_output.to(dtype=cache.dtype).contiguous().clone()
+        # grad_cache: the upstream gradient, copied when the kernel of this
+        # size uses it as scratch (the grouped kernel only reads it).
+        lib0 = _lib.hip()
+        g = grad_output.to(dtype=cache.dtype).contiguous()
+        if not lib0.sr_backward_mb_reads_only(Ops.shape[0], int(X.dtype == torch.float64)):
+            g = g.clone()
         grad_C = torch.empty_like(C)
         want_x = ctx.needs_input_grad[0]
         grad_X = torch.empty_like(X) if want_x else None
@@ -298,7 +301,9 @@ class _SymbolicEvaluationMultiple(torch.autograd.Function):
             raise RuntimeError("Cache and grad_output must have the same shape.")
         M, B, K = C.shape
         N, n_x = X.shape
-        g = grad_output.to(dtype=cache.dtype).contiguous().clone()
+        g = grad_output.to(dtype=cache.dtype).contiguous()
+        if not _lib.hip().sr_backward_mb_reads_only(M, int(X.dtype == torch.float64)):
+            g = g.clone()  # the kernel of this size uses it as scratch
         grad_C = torch.empty_like(C)
         want_x = ctx.needs_input_grad[0]
         grad_X = torch.empty_like(X) if want_x else None

@@ -69,6 +69,7 @@ def hip() -> ctypes.CDLL:
             _declare(lib, f"sr_multi_backward_mb_{sfx}", I,
                      [P, P, P, P, I64, I64, I64, I64, I64, P, P, P, I64, P, P])
         _declare(lib, "sr_backward_mb_workspace_bytes", I64, [I64, I64, I64, I])
+        _declare(lib, "sr_backward_mb_reads_only", I, [I64, I])
         _declare(lib, "sr_multi_backward_mb_workspace_bytes", I64, [I64, I64, I64, I64, I])
         _declare(lib, "sr_fused_workspace_bytes", I64, [I64, I64, I64, I64, I, I])
         _declare(lib, "sr_lbfgs_fit", I,

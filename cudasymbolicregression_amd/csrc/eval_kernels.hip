@@ -444,6 +444,13 @@ int backward_mb(T *grad_out, const T *cache, const int64_t *ops, const int64_t *
 
 extern "C" {
 
+// 1 when sr_backward_mb_* of this size only reads grad_cache (the grouped
+// kernel), 0 when it uses grad_cache as scratch.
+int sr_backward_mb_reads_only(int64_t M, int is_f64) {
+    if (M <= 0 || M > 128) return 0;
+    return (is_f64 ? backward_grp_lds<double>(M) : backward_grp_lds<float>(M)) <= lds_limit_device() ? 1 : 0;
+}
+
 int sr_validate_mb(const int64_t *ops, const int64_t *ch, int64_t M, int64_t B, unsigned long long *d_first,
                    void *stream) {
     hipStream_t s = (hipStream_t)stream;

@@ -313,6 +313,10 @@ int sr_mlp_gemm_nt(const void *A, const void *Bt, const float *bias, int64_t M, 
                    int64_t ldb, int epi, float *out_f32, void *out_bf16, float *pre_f32, void *stream) {
     if (M <= 0 || N <= 0 || K <= 0 || M % 32 || N % 32 || K % 16 || epi < 0 || epi > 2) return -1;
     if (lda < K || ldb < K || lda % 8 || ldb % 8 || M > INT32_MAX / 2 || N > INT32_MAX / 2) return -1;
+    // fragments are read as 16-byte vectors
+    if (A == nullptr || Bt == nullptr || ((uintptr_t)A | (uintptr_t)Bt) % 16 != 0) return -1;
+    if (out_f32 == nullptr && out_bf16 == nullptr) return -1;
+    if (epi != EPI_STORE && bias == nullptr) return -1;
     hipLaunchKernelGGL(k_gemm_nt, dim3(blocks(N, 64), blocks(M, 64)), dim3(256), 0, (hipStream_t)stream,
                        (const uint16_t *)A, (const uint16_t *)Bt, bias, (int)M, (int)N, (int)K, (int)lda, (int)ldb,
                        epi, out_f32, (uint16_t *)out_bf16, pre_f32);

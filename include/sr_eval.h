@@ -60,6 +60,9 @@ int sr_forward_mb_f64(const double *X, int64_t N, int64_t n_x, const int64_t *op
  * src/evaluation_kernels_cpu.cpp:222-662).  *d_err receives the largest
  * code (0, 201, 204). */
 int64_t sr_backward_mb_workspace_bytes(int64_t B, int64_t N, int64_t n_x, int is_f64);
+/* 1 when the backward of M-node programs only reads grad_cache (so a caller
+ * may pass its upstream gradient without a scratch copy), else 0. */
+int sr_backward_mb_reads_only(int64_t M, int is_f64);
 int sr_backward_mb_f32(float *grad_cache, const float *cache, const int64_t *ops, const int64_t *ch, int64_t M,
                        int64_t B, int64_t N, int64_t n_x, float *grad_C, float *grad_X, void *workspace,
                        int64_t ws_bytes, int32_t *d_err, void *stream);

@@ -30,10 +30,10 @@ extern "C" {
 /* C[M,N] = A[M,K] * Bt[N,K]^T with BF16 operands (row-major, K contiguous,
  * rows lda / ldb elements apart) and FP32 accumulation on
  * v_mfma_f32_32x32x16_bf16; C is dense (row stride N).  M and N multiples of
- * 32, K a multiple of 16, lda and ldb multiples of 8 and >= K, base pointers
- * 16-byte aligned.  Writes whichever of out_f32 (FP32), out_bf16 (BF16) are
- * non-NULL; with EPI_BIAS_GELU, pre_f32 (optional) receives Z.  bias may be
- * NULL for EPI_STORE. */
+ * 32, K a multiple of 16, lda and ldb multiples of 8 and >= K, A and Bt
+ * 16-byte aligned (else -1).  Writes whichever of out_f32 (FP32), out_bf16
+ * (BF16) are non-NULL (at least one); with EPI_BIAS_GELU, pre_f32 (optional)
+ * receives Z.  bias is required by the bias epilogues. */
 int sr_mlp_gemm_nt(const void *A, const void *Bt, const float *bias, int64_t M, int64_t N, int64_t K, int64_t lda,
                    int64_t ldb, int epi, float *out_f32, void *out_bf16, float *pre_f32, void *stream);
 

@@ -349,3 +349,26 @@ def test_fused_root_ragged_rejects_bad_sizes():
         fused.fused_root_ragged(X, torch.tensor([9], device=_dev()), ops, ch, C, None, fwd_only=True)
     with pytest.raises(RuntimeError, match="sum to"):
         fused.fused_root_ragged(X[:0], torch.tensor([0], device=_dev()), ops, ch, C, None, fwd_only=True)
+
+
+@pytest.mark.parametrize("M", [32, 96])
+def test_backward_leaves_the_upstream_gradient_unchanged(M):
+    """The caller's dL/dcache is not modified by backward, whether the kernel
+    of this size reads it directly (M <= 64) or works on a copy (M = 96)."""
+    import cudasymbolicregression_amd as st
+
+    X, ops_bm, ch_bm, C_bm = _case(60 + M, B=24, M=M, N=90, unary=[10, 11, 14], binary=[40, 41, 42])
+    Xt = _t(X).requires_grad_(True)
+    Ct = _t(C_bm).requires_grad_(True)
+    out = st.evaluation.evaluate(Xt, _t(ops_bm), _t(ch_bm), Ct)  # (B, M, N)
+    torch.cuda.synchronize()
+    ops = ops_bm
+    g = torch.zeros_like(out)
+    root = np.maximum((ops != 0).sum(1) - 1, 0)
+    for b in range(ops.shape[0]):
+        g[b, root[b]] = torch.randn(out.shape[2], device=_dev())
+    g0 = g.clone()
+    out.backward(g)
+    torch.cuda.synchronize()
+    assert torch.equal(g, g0)
+    assert Ct.grad is not None and torch.isfinite(Ct.grad).any()
