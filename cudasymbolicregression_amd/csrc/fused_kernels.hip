@@ -493,8 +493,9 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
         return -1;
     const bool ragged = x_off != nullptr || y_off != nullptr || n_pts != nullptr;
     if (ragged && (x_off == nullptr || y_off == nullptr || n_pts == nullptr || grad_X != nullptr)) return -1;
-    if (dy == nullptr && (mode == 1 || !fwd_only)) return -1;  // the kernel reads dy in these cases
-    if (grad_C == nullptr && !fwd_only) return -1;
+    // (an empty batch may come with null arrays)
+    if (B > 0 && dy == nullptr && (mode == 1 || !fwd_only)) return -1;  // the kernel reads dy in these cases
+    if (B > 0 && grad_C == nullptr && !fwd_only) return -1;
     const bool want_gx = grad_X != nullptr && !fwd_only;
     const FWs L = fused_layout<T>(N, n_x, B, M, want_gx);
     if (workspace_bytes < (int64_t)L.total) return -2;
@@ -610,7 +611,7 @@ int sr_fused_root_ragged_f32(const float *X, int64_t N, int64_t n_x, const int64
                              int64_t B, const float *dy, int mode, int fwd_only, float *y_out, double *loss_out,
                              float *grad_C, int32_t *err_b, int32_t *flags, void *workspace, int64_t workspace_bytes,
                              void *stream) {
-    if (x_off == nullptr || y_off == nullptr || n_pts == nullptr) return -1;
+    if (B > 0 && (x_off == nullptr || y_off == nullptr || n_pts == nullptr)) return -1;
     return fused_root_code<float>(X, N, n_x, ops, ch, C, M, B, dy, mode, fwd_only, y_out, loss_out, grad_C, nullptr,
                                   err_b, flags, workspace, workspace_bytes, (hipStream_t)stream, x_off, y_off, n_pts);
 }
@@ -619,7 +620,7 @@ int sr_fused_root_ragged_f64(const double *X, int64_t N, int64_t n_x, const int6
                              int64_t B, const double *dy, int mode, int fwd_only, double *y_out, double *loss_out,
                              double *grad_C, int32_t *err_b, int32_t *flags, void *workspace, int64_t workspace_bytes,
                              void *stream) {
-    if (x_off == nullptr || y_off == nullptr || n_pts == nullptr) return -1;
+    if (B > 0 && (x_off == nullptr || y_off == nullptr || n_pts == nullptr)) return -1;
     return fused_root_code<double>(X, N, n_x, ops, ch, C, M, B, dy, mode, fwd_only, y_out, loss_out, grad_C, nullptr,
                                    err_b, flags, workspace, workspace_bytes, (hipStream_t)stream, x_off, y_off,
                                    n_pts);

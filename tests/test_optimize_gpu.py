@@ -364,3 +364,38 @@ def test_fit_replays_from_a_captured_graph():
         for a, b in zip(eager, captured):
             assert torch.equal(_bits(a), _bits(b)), rep
         del junk
+
+
+def test_empty_and_single_point_batches():
+    """B = 0 and a single point through the fit, the fused evaluator, the
+    ragged forms and the warm start: empty outputs or the right values, no
+    errors."""
+    import cudasymbolicregression_amd as st
+    from cudasymbolicregression_amd import fused, warmstart
+    from cudasymbolicregression_amd.optimization import _fit
+
+    dev = _dev()
+    X = torch.rand(1, 3, device=dev)
+    ops = torch.tensor([[1, 50, 42]], device=dev)
+    ch = torch.tensor([[[-1, -1], [-1, -1], [0, 1]]], device=dev)
+    C0 = torch.tensor([[0.5, 0.0, 0.0]], device=dev)
+    Y = (2.0 * X[:, 0])[None, :].contiguous()
+    # B = 0
+    e_ops, e_ch, e_C = ops[:0], ch[:0], C0[:0]
+    C, loss, iters, err, flags, ncl = _fit(X, Y[:0], e_ops, e_ch, e_C, 10, 10, 1e-7, "strong_wolfe")
+    assert C.shape == (0, 3) and int(flags.abs().sum()) == 0
+    r = fused.fused_root(X, e_ops, e_ch, e_C, Y[:0], mode=1)
+    assert r["y"].shape == (0, 1) and r["loss"].shape == (0,)
+    rr = fused.fused_root_ragged(X[:0], torch.zeros(0, dtype=torch.int64, device=dev), e_ops, e_ch, e_C, None,
+                                 fwd_only=True)
+    assert rr["y"].numel() == 0
+    # one point: C * X0 fits 2 * X0 exactly
+    Cf, Yf = st.optimize(X, Y, ops, ch, C0, max_iter=50)
+    assert abs(float(Cf[0, 0]) - 2.0) < 1e-4
+    Cr, info = st.optimize_ragged(X, Y.reshape(-1), torch.tensor([1], device=dev), ops, ch, C0, max_iter=50)
+    assert torch.equal(Cr, Cf)
+    # warm start of one problem (the network reads 256 target points)
+    model = warmstart.WarmStartMLP(dev, seed=0)
+    Y256 = torch.randn(1, 256, device=dev)
+    Cw = model.predict(ops, Y256)
+    assert Cw.shape == (1, 3) and Cw[0, 1] == 0 and Cw[0, 2] == 0
