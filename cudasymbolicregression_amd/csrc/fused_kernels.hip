@@ -147,21 +147,53 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
 #pragma unroll
         for (int h = 0; h < SL; ++h) gc[h] = 0.0;
         int32_t e = 0;
-        for (int64_t base = 0; base < Nb; base += (int64_t)WAVE * PPT) {
+        // The variables and upstream values of tile group base + 64 * PPT are
+        // loaded while group base is evaluated, so no group waits on its loads.
+        // Every load is issued unconditionally at an in-bounds address (points
+        // past the end read the last point; without an upstream array the
+        // loads read X) and masked afterwards: a fixed number of loads per
+        // group lets the first use wait for the current group's loads only.
+        const bool need_dy = a.mode == 1 || !a.fwd_only;
+        const T *const dsrc = need_dy ? dy + yo : X;
+        T xv[PPT][F_NX], dv[PPT], xn[PPT][F_NX], dn[PPT];
+        auto load_group = [&](int64_t gb, T (&xo)[PPT][F_NX], T (&dout)[PPT]) {
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int64_t nq = gb + q * WAVE + lane;
+                const bool aq = nq < Nb;
+                const int64_t nl = aq ? nq : Nb - 1;
+                const T *row = X + nl * (int64_t)n_x;
+#pragma unroll
+                for (int v = 0; v < F_NX; ++v) {
+                    const bool use = !GX && v < n_x;
+                    const T xval = row[use ? v : 0];
+                    xo[q][v] = (use && aq) ? xval : T(0);
+                }
+                const T dval = dsrc[need_dy ? nl : 0];
+                dout[q] = (need_dy && aq) ? dval : T(0);
+            }
+        };
+        auto take_next = [&]() {
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                dv[q] = dn[q];
+#pragma unroll
+                for (int v = 0; v < F_NX; ++v) xv[q][v] = xn[q][v];
+            }
+        };
+        if (Nb > 0) load_group(0, xv, dv);
+        for (int64_t base = 0; base < Nb; base += (int64_t)WAVE * PPT, take_next()) {
             bool act[PPT];
             int64_t n[PPT];
-            T xv[PPT][F_NX], gxr[PPT][F_NX];
+            T gxr[PPT][F_NX];
 #pragma unroll
             for (int q = 0; q < PPT; ++q) {
                 n[q] = base + q * WAVE + lane;
                 act[q] = n[q] < Nb;
-                const T *row = X + (act[q] ? n[q] : 0) * (int64_t)n_x;
 #pragma unroll
-                for (int v = 0; v < F_NX; ++v) {
-                    xv[q][v] = (!GX && act[q] && v < n_x) ? row[v] : T(0);
-                    gxr[q][v] = T(0);
-                }
+                for (int v = 0; v < F_NX; ++v) gxr[q][v] = T(0);
             }
+            load_group(base + (int64_t)WAVE * PPT, xn, dn);
             auto fetch = [&](uint32_t d, T (&av)[PPT]) {
                 const int kind = desc_kind(d), idx = desc_idx(d);
                 const int tidx = kind == K_TAPE ? idx : 0;
@@ -213,11 +245,11 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
                 if (act[q] && y_out) y_out[yo + n[q]] = yh;
                 T g0 = T(0);
                 if (a.mode == 1) {
-                    const T d = yh - (act[q] ? dy[yo + n[q]] : T(0));
+                    const T d = yh - dv[q];
                     if (act[q]) lacc += (double)d * (double)d;
                     g0 = (T(2.0) * d) * inv_n;
                 } else if (!a.fwd_only) {
-                    g0 = act[q] ? dy[yo + n[q]] : T(0);
+                    g0 = dv[q];
                 }
                 t.set_adj(NI - 1, q, act[q] ? g0 : T(0));
             }
