@@ -102,14 +102,18 @@ __device__ inline double sdot(const SVec<float, SL> &a, const SVec<float, SL> &b
 // tile groups of 64*PPT points starting at group `wslot`, every W-th group.
 // Returns the wave's FP64 loss sum (uniform), its FP64 dL/dC partials in
 // compact lanes (gc), and its error code (uniform).  Per instruction there is
-// one opcode dispatch for all PPT points; operand fetch and adjoint scatter
-// are selects on the (uniform) operand descriptors.
-template <int SL, int NX, int PPT, int LT, bool GX>
-__device__ inline void closure_code(const Code &c, int nc, const SVec<float, SL> &xt, const float *X, const float *Yb,
-                                    int64_t N, int n_x, int wslot, int W, const Tape<float, PPT> &t, double &loss,
-                                    double (&gc)[SL], int32_t &err) {
+// one opcode dispatch for all PPT points; operands come from the plan
+// (sr_code.h): a tape row, a scalar resolved once per evaluation, or a
+// variable read from X (v >= NXR_ROWS); adjoints overwrite values in place.
+template <int SL, int NX, int PPT, int LT>
+__device__ inline void closure_code(const Code &c, const Plan &pl, int nc, const SVec<float, SL> &xt, const float *X,
+                                    const float *Yb, int64_t N, int n_x, int wslot, int W, const Tape<float, PPT> &t,
+                                    double &loss, double (&gc)[SL], int32_t &err) {
     const int lane = lane_id();
     const int NI = c.NI;
+    constexpr int HW = LT > WAVE ? 2 : 1;
+    float sv0[HW], sv1[HW];
+    resolve_scalars<float, SL, LT>(c, xt, sv0, sv1);
     double lacc = 0.0;
     double cacc[FIT_NCREG];
 #pragma unroll
@@ -132,34 +136,41 @@ __device__ inline void closure_code(const Code &c, int nc, const SVec<float, SL>
             for (int v = 0; v < NX; ++v) xv[p][v] = (act[p] && v < n_x) ? row[v] : 0.0f;
             target[p] = act[p] ? Yb[n[p]] : 0.0f;
         }
-        auto fetch = [&](uint32_t d, float (&a)[PPT]) {
-            const int kind = desc_kind(d), idx = desc_idx(d);
-            const int tidx = kind == K_TAPE ? idx : 0;
-            const int vidx = kind == K_VAR ? idx : 0;
-            const float cv = slot_get<float, SL>(xt, kind == K_CONST ? idx : 0);
-            const float iv = imm_value<float>(kind == K_IMM ? idx : 0);
+        // variable rows, and zeroed extra adjoint rows
 #pragma unroll
-            for (int p = 0; p < PPT; ++p) {
-                const float tv = t.val(tidx, p);
-                float vv = 0.0f;
-                if constexpr (!GX) {
+        for (int v = 0; v < NX; ++v)
+            if (v < n_x)
 #pragma unroll
-                    for (int v = 0; v < NX; ++v) vv = (v == vidx) ? xv[p][v] : vv;
-                } else {
-                    // more variables than registers: read them from the X row
-                    if (kind == K_VAR) vv = act[p] ? X[n[p] * (int64_t)n_x + vidx] : 0.0f;
+                for (int p = 0; p < PPT; ++p) t.put(LT + v, p, xv[p][v]);
+        for (int r = NI; r < NI + pl.nextra; ++r)
+#pragma unroll
+            for (int p = 0; p < PPT; ++p) t.put(r, p, 0.0f);
+        auto fetch = [&](uint32_t f, int k, int i, float (&a)[PPT]) {
+            if (pf_xg(f, k)) {  // more variables than rows: read it from the X row
+                const uint32_t w = c.word_lt<LT>(i);
+                const int v = desc_idx(k == 0 ? instr_d0(w) : instr_d1(w));
+#pragma unroll
+                for (int p = 0; p < PPT; ++p) a[p] = act[p] ? X[n[p] * (int64_t)n_x + v] : 0.0f;
+            } else {
+                const int row = pf_row(f, k);
+                const bool sc = pf_sc(f, k);
+                const float sv = k == 0 ? (HW == 1 ? rl(sv0[0], i) : (i < WAVE ? rl(sv0[0], i) : rl(sv0[HW - 1], i - WAVE)))
+                                        : (HW == 1 ? rl(sv1[0], i) : (i < WAVE ? rl(sv1[0], i) : rl(sv1[HW - 1], i - WAVE)));
+#pragma unroll
+                for (int p = 0; p < PPT; ++p) {
+                    const float tv = t.get(row, p);
+                    a[p] = sc ? sv : tv;
                 }
-                a[p] = kind == K_TAPE ? tv : (kind == K_VAR ? vv : (kind == K_CONST ? cv : iv));
             }
         };
-        // ---- forward; adjoints are zeroed on the way
+        // ---- forward
         for (int i = 0; i < NI; ++i) {
-            const uint32_t w = c.word_lt<LT>(i);
-            const int op = instr_op(w);
+            const uint32_t f = pl.fw<LT>(i);
+            const int op = pf_op(f);
             float a0[PPT], a1[PPT], r[PPT];
-            fetch(instr_d0(w), a0);
-            if (op >= ADD && op <= POW) {
-                fetch(instr_d1(w), a1);
+            fetch(f, 0, i, a0);
+            if (pf_bin(f)) {
+                fetch(f, 1, i, a1);
             } else {
 #pragma unroll
                 for (int p = 0; p < PPT; ++p) a1[p] = 0.0f;
@@ -171,31 +182,30 @@ __device__ inline void closure_code(const Code &c, int nc, const SVec<float, SL>
                 fwd_nodes<float, PPT>(op, a0, a1, r);
             }
 #pragma unroll
-            for (int p = 0; p < PPT; ++p) {
-                t.set(i, p, r[p]);
-                t.set_adj(i, p, 0.0f);
-            }
+            for (int p = 0; p < PPT; ++p) t.put(i, p, r[p]);
         }
-        // ---- loss and the MSE upstream gradient (2 * (y - Y)) * (1 / N)
+        // ---- loss and the MSE upstream gradient (2 * (y - Y)) * (1 / N),
+        //      which replaces the root's value
 #pragma unroll
         for (int p = 0; p < PPT; ++p) {
-            const float d = t.val(NI - 1, p) - target[p];
+            const float d = t.get(NI - 1, p) - target[p];
             if (act[p]) lacc += (double)d * (double)d;
-            t.set_adj(NI - 1, p, act[p] ? (2.0f * d) * inv_n : 0.0f);
+            t.put(NI - 1, p, act[p] ? (2.0f * d) * inv_n : 0.0f);
         }
         // ---- reverse sweep
         for (int i = NI - 1; i >= 0; --i) {
-            const uint32_t w = c.word_lt<LT>(i);
-            const int op = instr_op(w);
-            const bool binary = op >= ADD && op <= POW;
+            const uint32_t f = pl.fw<LT>(i), sw = pl.sw<LT>(i);
+            const int op = pf_op(f);
+            const bool binary = pf_bin(f);
             float g[PPT], a0[PPT], a1[PPT], go0[PPT], go1[PPT];
             bool add0[PPT], add1[PPT];
             int32_t ee[PPT];
+            const int grow = ps_grow(sw);
 #pragma unroll
-            for (int p = 0; p < PPT; ++p) g[p] = t.adj(i, p);
-            fetch(instr_d0(w), a0);
+            for (int p = 0; p < PPT; ++p) g[p] = t.get(grow, p);
+            fetch(f, 0, i, a0);
             if (binary) {
-                fetch(instr_d1(w), a1);
+                fetch(f, 1, i, a1);
             } else {
 #pragma unroll
                 for (int p = 0; p < PPT; ++p) a1[p] = 0.0f;
@@ -216,33 +226,39 @@ __device__ inline void closure_code(const Code &c, int nc, const SVec<float, SL>
             for (int p = 0; p < PPT; ++p)
                 if (act[p] && ee[p] > e) e = ee[p];
             // scatter to the operands
-            auto scatter = [&](uint32_t d, const float (&go)[PPT], const bool (&add)[PPT]) {
-                const int kind = desc_kind(d), idx = desc_idx(d);
-                const int slot = kind == K_TAPE ? idx : LT;  // row LT absorbs the other kinds
+            auto scatter = [&](int k, const float (&go)[PPT], const bool (&add)[PPT]) {
+                const uint32_t mode = ps_mode(sw, k);
+                const int tg = ps_tgt(sw, k);
+                if (mode == PM_WRITE) {  // the only adjoint of a one-parent row: replaces its value
 #pragma unroll
-                for (int p = 0; p < PPT; ++p) {
-                    const float o = t.adj(slot, p);
-                    t.set_adj(slot, p, (act[p] && add[p]) ? o + go[p] : o);
-                    if (kind == K_IMM && idx == IMM_NOOP && act[p] && add[p] && go[p] != 0.0f)
-                        e = e > ERR_GRAD_NO_OP ? e : ERR_GRAD_NO_OP;
-                }
-                if (kind == K_CONST) {
+                    for (int p = 0; p < PPT; ++p) t.put(tg, p, (act[p] && add[p]) ? 0.0f + go[p] : 0.0f);
+                } else if (mode == PM_ACC) {
+#pragma unroll
+                    for (int p = 0; p < PPT; ++p) {
+                        const float o = t.get(tg, p);
+                        t.put(tg, p, (act[p] && add[p]) ? o + go[p] : o);
+                    }
+                } else if (mode == PM_CONST) {
                     double gv = 0.0;
 #pragma unroll
                     for (int p = 0; p < PPT; ++p) gv += (act[p] && add[p]) ? (double)go[p] : 0.0;
-                    if (idx < FIT_NCREG) {
+                    if (tg < FIT_NCREG) {
 #pragma unroll
                         for (int q = 0; q < FIT_NCREG; ++q)
-                            if (q == idx) cacc[q] += gv;
+                            if (q == tg) cacc[q] += gv;
                     } else {
                         const double s = wave_sum_rows(gv, 4);
-                        if (idx < WAVE) { if (lane == idx) gc[0] += s; }
-                        else if (lane == idx - WAVE) gc[SL - 1] += s;
+                        if (tg < WAVE) { if (lane == tg) gc[0] += s; }
+                        else if (lane == tg - WAVE) gc[SL - 1] += s;
                     }
+                } else if (mode == PM_NOOP) {
+#pragma unroll
+                    for (int p = 0; p < PPT; ++p)
+                        if (act[p] && add[p] && go[p] != 0.0f) e = e > ERR_GRAD_NO_OP ? e : ERR_GRAD_NO_OP;
                 }
             };
-            scatter(instr_d0(w), go0, add0);
-            if (binary) scatter(instr_d1(w), go1, add1);
+            scatter(0, go0, add0);
+            if (binary) scatter(1, go1, add1);
         }
     }
     loss = wave_sum_rows(lacc, 4);
@@ -270,8 +286,8 @@ __host__ __device__ inline size_t hist_bytes(int hist, int SL) {
 __host__ __device__ inline size_t part_bytes(int W, int SL) {
     return W > 1 ? (size_t)2 * W * (SL * WAVE + 2) * sizeof(double) : 0;
 }
-// Tape of one wave: LT value rows and LT + 1 adjoint rows per point of a
-// tile group, 64 floats each.
+// Tape of one wave: LT instruction rows and NXR_ROWS variable rows per
+// point of a tile group, 64 floats each.
 __host__ __device__ inline size_t tape_bytes(int LT, int PPT, bool mem) {
     return mem ? 0 : tape_elems(LT, PPT) * sizeof(float);
 }
@@ -287,8 +303,8 @@ constexpr int32_t ERR_BUCKET = 900;
 
 enum FitState : int { ST_INIT = 0, ST_SW = 1, ST_SW_ZERO = 2, ST_ZOOM = 3, ST_BT = 4, ST_FINAL = 5 };
 
-// LT: most instructions of a program in the bucket; NX: variables held in
-// registers per point; SL: slot registers per lane; W: waves per problem;
+// LT: most tape rows of a program in the bucket (count_rows_wave); NX:
+// variables held in tape rows per point; SL: slot registers per lane; W: waves per problem;
 // MEM: tape in the global workspace instead of LDS; WPE: waves per SIMD the
 // register budget is sized for; PPT: points per lane per dispatch.
 template <int LT, int NX, int SL, int W, bool MEM, int WPE, int PPT>
@@ -305,7 +321,6 @@ void k_fit(FitArgs a) {
         float *slot = MEM ? a.gtape + ((size_t)blockIdx.x * block_waves(W, SL) + wave) * tape_elems(LT, PPT)
                           : reinterpret_cast<float *>(smem) + (size_t)wave * tape_elems(LT, PPT);
         tape.v = slot;
-        tape.a = slot + (size_t)LT * PPT * WAVE;
     }
     unsigned char *hbase = smem + (size_t)block_waves(W, SL) * tape_bytes(LT, PPT, MEM);
     const int groups = W == 1 ? block_waves(W, SL) : 1;
@@ -349,7 +364,8 @@ void k_fit(FitArgs a) {
             continue;
         }
         const Code code = compile_code(p, a.M);
-        if (code.NI > LT) {
+        const Plan plan = compile_plan(p, a.M, code, LT, NX);
+        if (code.NI + plan.nextra > LT) {
             if (writer && lane == 0) {
                 atomicMax(&a.flags[1], ERR_BUCKET);
                 a.err_out[b] = ERR_BUCKET;
@@ -384,10 +400,7 @@ void k_fit(FitArgs a) {
             for (int h = 0; h < SL; ++h) xt.v[h] = (st == ST_INIT) ? x.v[h] : x.v[h] + a_eval * d.v[h];
             double phi, gcd[SL];
             int32_t e2;
-            if (a.n_x <= NX)
-                closure_code<SL, NX, PPT, LT, false>(code, p.nc, xt, X, Yb, N, a.n_x, wslot, W, tape, phi, gcd, e2);
-            else
-                closure_code<SL, NX, PPT, LT, true>(code, p.nc, xt, X, Yb, N, a.n_x, wslot, W, tape, phi, gcd, e2);
+            closure_code<SL, NX, PPT, LT>(code, plan, p.nc, xt, X, Yb, N, a.n_x, wslot, W, tape, phi, gcd, e2);
             if (W > 1) {
                 // partial sums of the W waves meet in LDS; every wave adds them
                 // in the same order, so all waves hold identical results
@@ -682,24 +695,24 @@ void k_fit(FitArgs a) {
     }
 }
 
-// Buckets problems by instruction count NI (operator nodes up to the root,
-// see compile_code) and live slot count K: NI <= 8, <= 16, <= 32, <= 64
-// (with K <= 64), and the rest.
+// Buckets problems by tape rows (count_rows_wave: instructions plus extra
+// adjoint rows) and live slot count K: rows <= 8, <= 16, <= 32, <= 64 (with
+// K <= 64), and the rest.
 constexpr int NB_CLASSIFY = 5;  // buckets of k_classify (== NBUCKETS below)
 
 // Each wave classifies a run of 64 consecutive problems (one whole wave per
 // program) and records it for the deterministic bucket lists (sr_queue.h).
-__global__ __launch_bounds__(256) void k_classify(const int64_t *__restrict__ ops, int M, int64_t B, int n_x,
-                                                  int8_t *bucket_of, int32_t *runcnt) {
+__global__ __launch_bounds__(256) void k_classify(const int64_t *__restrict__ ops, const int64_t *__restrict__ ch, int M,
+                                                  int64_t B, int n_x, int8_t *bucket_of, int32_t *runcnt) {
     const int64_t b0 = ((int64_t)blockIdx.x * 4 + uniform_i32((int)(threadIdx.x / WAVE))) * WAVE;
     if (b0 >= B) return;
     const int n = (int)(B - b0 < WAVE ? B - b0 : WAVE);
     int mine = -1;
     for (int j = 0; j < n; ++j) {
         int nc = 0;
-        const int ni = count_instructions_wave(ops + (b0 + j) * M, M, n_x, nc);
+        const int nr = count_rows_wave(ops + (b0 + j) * M, ch + (b0 + j) * M * 2, M, n_x, nc);
         const int K = nc + (M > nc ? 1 : 0);
-        const int bucket = K > 64 ? 4 : (ni <= 8 ? 0 : (ni <= 16 ? 1 : (ni <= 32 ? 2 : (ni <= 64 ? 3 : 4))));
+        const int bucket = K > 64 ? 4 : (nr <= 8 ? 0 : (nr <= 16 ? 1 : (nr <= 32 ? 2 : (nr <= 64 ? 3 : 4))));
         mine = lane_id() == j ? bucket : mine;
     }
     queue_record_run(mine, NB_CLASSIFY, b0, n, bucket_of, runcnt);
@@ -741,7 +754,7 @@ int grid_for(int hist) {
 // launch variants (waves per problem W x points per lane PPT); one is active
 // per bucket and process, chosen by sr_fit_set_variant (the autotuner) or the
 // built-in default.
-constexpr int NXR = 4;
+constexpr int NXR = NXR_ROWS;  // variables in tape rows (the rest are read from X)
 constexpr int NBUCKETS = 5;
 static_assert(NBUCKETS == NB_CLASSIFY, "k_classify and the launch plan disagree on the buckets");
 constexpr int B4_BLOCKS_PER_CU = 2;  // x 2 waves: 1024 long problems in flight
@@ -775,7 +788,7 @@ const FitVariant kB2[] = {{launch_variant<32, 1, 2, false, 3, 1>, 2, 1},
                           {launch_variant<32, 1, 8, false, 2, 1>, 8, 1}};
 const FitVariant kB3[] = {{launch_variant<64, 1, 4, false, 2, 1>, 4, 1},
                           {launch_variant<64, 1, 2, false, 2, 1>, 2, 1}};
-const FitVariant kB4[] = {{launch_variant<128, 2, 1, true, 2, 1>, 1, 1}};
+const FitVariant kB4[] = {{launch_variant<256, 2, 1, true, 2, 1>, 1, 1}};
 
 const FitVariant *const kVariants[NBUCKETS] = {kB0, kB1, kB2, kB3, kB4};
 constexpr int kNumVariants[NBUCKETS] = {5, 4, 4, 2, 1};
@@ -795,8 +808,8 @@ inline size_t fit_q_off(int64_t B) { return (((size_t)NBUCKETS * B * 4 + 64) + 2
 inline size_t fit_gt_off(int64_t B) { return fit_q_off(B) + (queue_ws_bytes(B, NBUCKETS) + 255) / 256 * 256; }
 inline int64_t fit_ws(int64_t B, int64_t M) {
     size_t off = fit_gt_off(B);
-    if (M > 64)  // more than 64 instructions or live slots needs more than 64 nodes
-        off += (size_t)b4_grid() * block_waves(1, 2) * tape_elems(128, 1) * sizeof(float);
+    if (M > 32)  // more than 64 tape rows (at most 2 M - 1) or live slots
+        off += (size_t)b4_grid() * block_waves(1, 2) * tape_elems(256, 1) * sizeof(float);
     return (int64_t)off + 256;
 }
 
@@ -847,8 +860,8 @@ int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const i
     int32_t *counters = (int32_t *)(ws + (size_t)NBUCKETS * B * 4);  // [0..4] counts, [5..9] work
     hipLaunchKernelGGL(k_fill_i32_f, dim3(1), dim3(64), 0, s, counters, 0, (int64_t)(2 * NBUCKETS));
     void *qws = ws + fit_q_off(B);
-    hipLaunchKernelGGL(k_classify, dim3((unsigned)((B + 4 * WAVE - 1) / (4 * WAVE))), dim3(256), 0, s, ops, (int)M,
-                       B, (int)n_x, queue_bucket_of(qws), queue_runcnt(qws, B));
+    hipLaunchKernelGGL(k_classify, dim3((unsigned)((B + 4 * WAVE - 1) / (4 * WAVE))), dim3(256), 0, s, ops, ch,
+                       (int)M, B, (int)n_x, queue_bucket_of(qws), queue_runcnt(qws, B));
     queue_finish(qws, B, NBUCKETS, lists, counters, s);
     FitArgs a;
     memset(&a, 0, sizeof(a));
@@ -859,7 +872,7 @@ int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const i
     a.max_iter = max_iter; a.history = history; a.tol_grad = tol_grad; a.line_search = line_search;
     a.ls_max_iter = ls_max_iter; a.eps = eps;
     a.stationary_exit = g_stationary_exit;
-    a.gtape = M > 64 ? (float *)(ws + fit_gt_off(B)) : nullptr;
+    a.gtape = M > 32 ? (float *)(ws + fit_gt_off(B)) : nullptr;
     auto bucket_args = [&](int k) {
         FitArgs r = a;
         r.list = lists + (size_t)k * B;
@@ -869,9 +882,9 @@ int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const i
     };
     int r = (int)hipGetLastError();
     if (r) return r;
-    // A program has at most M instructions and M live slots, so short rows
-    // only need the first buckets.
-    if (M <= 8) return launch_bucket(0, bucket_args(0), s);
+    // A program has at most 2 M - 1 tape rows and M live slots, so short
+    // rows only need the first buckets.
+    if (M <= 4) return launch_bucket(0, bucket_args(0), s);
     // Buckets 4+3, 2 and 1 on side streams, bucket 0 on the caller's stream,
     // joined back before returning.
     Fork &f = fork_for_device();
@@ -879,9 +892,9 @@ int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const i
     if ((r = (int)hipEventRecord(f.ev_fork, s))) return r;
     for (int i = 0; i < 3; ++i)
         if ((r = (int)hipStreamWaitEvent(f.side[i], f.ev_fork, 0))) return r;
-    if (M > 64 && (r = launch_bucket(4, bucket_args(4), f.side[0]))) return r;
-    if (M > 32 && (r = launch_bucket(3, bucket_args(3), f.side[0]))) return r;
-    if (M > 16 && (r = launch_bucket(2, bucket_args(2), f.side[1]))) return r;
+    if (M > 32 && (r = launch_bucket(4, bucket_args(4), f.side[0]))) return r;
+    if (M > 16 && (r = launch_bucket(3, bucket_args(3), f.side[0]))) return r;
+    if (M > 8 && (r = launch_bucket(2, bucket_args(2), f.side[1]))) return r;
     if ((r = launch_bucket(1, bucket_args(1), f.side[2]))) return r;
     if ((r = launch_bucket(0, bucket_args(0), s))) return r;
     for (int i = 0; i < 3; ++i) {

@@ -14,11 +14,11 @@
 // at offset y_off[b] (no grad_X).
 //
 // Execution (gfx950): one wave per expression; the program is compiled once
-// per wave into an instruction list (sr_code.h) and evaluated over tile
-// groups of 64*PPT points with the tape and adjoints in LDS.  Expressions are
-// bucketed by instruction count (<= 8, <= 16, <= 32, <= 64, the rest) so each bucket's
-// kernel is sized for its tape; the buckets run concurrently on forked
-// streams.  Variable adjoints are summed per block in LDS (float atomics)
+// per wave into an instruction list and an evaluation plan (sr_code.h) and
+// evaluated over tile groups of 64*PPT points with the tape in LDS (values,
+// and adjoints in place).  Expressions are bucketed by tape rows (<= 8,
+// <= 16, <= 32, <= 64, the rest) so each bucket's kernel is sized for its
+// tape; the buckets run concurrently on forked streams.  Variable adjoints are summed per block in LDS (float atomics)
 // and written as one partial row per block; two final kernels add the rows
 // in FP64 in a fixed order.
 #include "sr_code.h"
@@ -96,7 +96,6 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
         T *slot = MEM ? (T *)a.gtape + ((size_t)blockIdx.x * WPB + wave) * tape_elems(LT, PPT)
                       : reinterpret_cast<T *>(smem + gx_bytes) + (size_t)wave * tape_elems(LT, PPT);
         t.v = slot;
-        t.a = slot + (size_t)LT * PPT * WAVE;
     }
     if (gx_lds)
         for (int64_t i = threadIdx.x; i < gx_len; i += WPB * WAVE) gx_blk[i] = T(0);
@@ -125,7 +124,10 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
             continue;
         }
         const Code c = compile_code(p, a.M);
-        if (c.NI > LT) {  // the bucketing rule and the compiler disagree: never expected
+        constexpr int NVR = GX ? 0 : NXR_ROWS;  // variables held in tape rows LT + v
+        const Plan pl = compile_plan(p, a.M, c, LT, NVR);
+        const int NI = c.NI;
+        if (NI + pl.nextra > LT) {  // the bucketing rule and the compiler disagree: never expected
             if (lane == 0) {
                 atomicMax(&a.flags[1], 900);
                 if (a.err_b) a.err_b[b] = 900;
@@ -133,7 +135,9 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
             continue;
         }
         const SVec<T, SL> xc = compact_start<T, SL>(p);
-        const int NI = c.NI;
+        constexpr int HW = LT > WAVE ? 2 : 1;
+        T sv0[HW], sv1[HW];
+        resolve_scalars<T, SL, LT>(c, xc, sv0, sv1);
         // this expression's points (the shared X, or its own rows when ragged)
         const int64_t Nb = a.n_pts ? (int64_t)a.n_pts[b] : N;
         const int64_t yo = a.y_off ? a.y_off[b] : b * N;
@@ -194,33 +198,43 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
                 for (int v = 0; v < F_NX; ++v) gxr[q][v] = T(0);
             }
             load_group(base + (int64_t)WAVE * PPT, xn, dn);
-            auto fetch = [&](uint32_t d, T (&av)[PPT]) {
-                const int kind = desc_kind(d), idx = desc_idx(d);
-                const int tidx = kind == K_TAPE ? idx : 0;
-                const int vidx = kind == K_VAR ? idx : 0;
-                const T cv = slot_get<T, SL>(xc, kind == K_CONST ? idx : 0);
-                const T iv = imm_value<T>(kind == K_IMM ? idx : 0);
+            // variable rows, and zeroed extra adjoint rows
 #pragma unroll
-                for (int q = 0; q < PPT; ++q) {
-                    const T tv = t.val(tidx, q);
-                    T vv = T(0);
-                    if constexpr (!GX) {
+            for (int v = 0; v < NVR; ++v)
+                if (v < n_x)
 #pragma unroll
-                        for (int v = 0; v < F_NX; ++v) vv = (v == vidx) ? xv[q][v] : vv;
-                    } else {
-                        if (kind == K_VAR) vv = act[q] ? X[n[q] * (int64_t)n_x + vidx] : T(0);
+                    for (int q = 0; q < PPT; ++q) t.put(LT + v, q, xv[q][v]);
+            for (int r = NI; r < NI + pl.nextra; ++r)
+#pragma unroll
+                for (int q = 0; q < PPT; ++q) t.put(r, q, T(0));
+            // operand k of instruction ii: a tape row, a scalar, or (more
+            // variables than rows) a variable read from X
+            auto fetch = [&](uint32_t f, int k, int ii, T (&av)[PPT]) {
+                if (pf_xg(f, k)) {
+                    const uint32_t w = c.word_lt<LT>(ii);
+                    const int v = desc_idx(k == 0 ? instr_d0(w) : instr_d1(w));
+#pragma unroll
+                    for (int q = 0; q < PPT; ++q) av[q] = act[q] ? X[n[q] * (int64_t)n_x + v] : T(0);
+                } else {
+                    const int row = pf_row(f, k);
+                    const bool sc = pf_sc(f, k);
+                    const T sv = k == 0 ? (HW == 1 ? rl(sv0[0], ii) : (ii < WAVE ? rl(sv0[0], ii) : rl(sv0[HW - 1], ii - WAVE)))
+                                        : (HW == 1 ? rl(sv1[0], ii) : (ii < WAVE ? rl(sv1[0], ii) : rl(sv1[HW - 1], ii - WAVE)));
+#pragma unroll
+                    for (int q = 0; q < PPT; ++q) {
+                        const T tv = t.get(row, q);
+                        av[q] = sc ? sv : tv;
                     }
-                    av[q] = kind == K_TAPE ? tv : (kind == K_VAR ? vv : (kind == K_CONST ? cv : iv));
                 }
             };
-            // ---- forward; adjoints are zeroed on the way
+            // ---- forward
             for (int ii = 0; ii < NI; ++ii) {
-                const uint32_t w = c.word_lt<LT>(ii);
-                const int op = instr_op(w);
+                const uint32_t f = pl.fw<LT>(ii);
+                const int op = pf_op(f);
                 T a0[PPT], a1[PPT], r[PPT];
-                fetch(instr_d0(w), a0);
-                if (op >= ADD && op <= POW) {
-                    fetch(instr_d1(w), a1);
+                fetch(f, 0, ii, a0);
+                if (pf_bin(f)) {
+                    fetch(f, 1, ii, a1);
                 } else {
 #pragma unroll
                     for (int q = 0; q < PPT; ++q) a1[q] = T(0);
@@ -232,16 +246,14 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
                     fwd_nodes<T, PPT>(op, a0, a1, r);
                 }
 #pragma unroll
-                for (int q = 0; q < PPT; ++q) {
-                    t.set(ii, q, r[q]);
-                    t.set_adj(ii, q, T(0));
-                }
+                for (int q = 0; q < PPT; ++q) t.put(ii, q, r[q]);
             }
             // ---- root value, loss and upstream gradient (dy is read only in
-            //      MODE 1 or when the backward runs; it may be null otherwise)
+            //      MODE 1 or when the backward runs; it may be null otherwise);
+            //      the root's adjoint replaces its value
 #pragma unroll
             for (int q = 0; q < PPT; ++q) {
-                const T yh = t.val(NI - 1, q);
+                const T yh = t.get(NI - 1, q);
                 if (act[q] && y_out) y_out[yo + n[q]] = yh;
                 T g0 = T(0);
                 if (a.mode == 1) {
@@ -251,22 +263,23 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
                 } else if (!a.fwd_only) {
                     g0 = dv[q];
                 }
-                t.set_adj(NI - 1, q, act[q] ? g0 : T(0));
+                t.put(NI - 1, q, act[q] ? g0 : T(0));
             }
             if (a.fwd_only) continue;
             // ---- reverse sweep
             for (int ii = NI - 1; ii >= 0; --ii) {
-                const uint32_t w = c.word_lt<LT>(ii);
-                const int op = instr_op(w);
-                const bool binary = op >= ADD && op <= POW;
+                const uint32_t f = pl.fw<LT>(ii), sw = pl.sw<LT>(ii);
+                const int op = pf_op(f);
+                const bool binary = pf_bin(f);
                 T g[PPT], a0[PPT], a1[PPT], go0[PPT], go1[PPT];
                 bool add0[PPT], add1[PPT];
                 int32_t ee[PPT];
+                const int grow = ps_grow(sw);
 #pragma unroll
-                for (int q = 0; q < PPT; ++q) g[q] = t.adj(ii, q);
-                fetch(instr_d0(w), a0);
+                for (int q = 0; q < PPT; ++q) g[q] = t.get(grow, q);
+                fetch(f, 0, ii, a0);
                 if (binary) {
-                    fetch(instr_d1(w), a1);
+                    fetch(f, 1, ii, a1);
                 } else {
 #pragma unroll
                     for (int q = 0; q < PPT; ++q) a1[q] = T(0);
@@ -286,45 +299,59 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
 #pragma unroll
                 for (int q = 0; q < PPT; ++q)
                     if (act[q] && ee[q] > e) e = ee[q];
-                auto scatter = [&](uint32_t d, const T (&go)[PPT], const bool (&add)[PPT]) {
-                    const int kind = desc_kind(d), idx = desc_idx(d);
-                    const int slot = kind == K_TAPE ? idx : LT;  // row LT absorbs the other kinds
+                auto scatter = [&](int k, const T (&go)[PPT], const bool (&add)[PPT]) {
+                    const uint32_t mode = ps_mode(sw, k);
+                    const int tg = ps_tgt(sw, k);
+                    if (mode == PM_WRITE) {  // the only adjoint of a one-parent row: replaces its value
 #pragma unroll
-                    for (int q = 0; q < PPT; ++q) {
-                        const T o = t.adj(slot, q);
-                        t.set_adj(slot, q, (act[q] && add[q]) ? o + go[q] : o);
-                        if (kind == K_IMM && idx == IMM_NOOP && act[q] && add[q] && go[q] != T(0))
-                            e = e > ERR_GRAD_NO_OP ? e : ERR_GRAD_NO_OP;
-                    }
-                    if (kind == K_CONST) {
+                        for (int q = 0; q < PPT; ++q) t.put(tg, q, (act[q] && add[q]) ? T(0) + go[q] : T(0));
+                    } else if (mode == PM_ACC) {
+#pragma unroll
+                        for (int q = 0; q < PPT; ++q) {
+                            const T o = t.get(tg, q);
+                            t.put(tg, q, (act[q] && add[q]) ? o + go[q] : o);
+                        }
+                    } else if (mode == PM_CONST) {
                         double gv = 0.0;
 #pragma unroll
                         for (int q = 0; q < PPT; ++q) gv += (act[q] && add[q]) ? (double)go[q] : 0.0;
-                        if (idx < F_NCREG) {
+                        if (tg < F_NCREG) {
 #pragma unroll
                             for (int r = 0; r < F_NCREG; ++r)
-                                if (r == idx) cacc[r] += gv;
+                                if (r == tg) cacc[r] += gv;
                         } else {
                             const double s = wave_sum_rows(gv, 4);
-                            if (idx < WAVE) { if (lane == idx) gc[0] += s; }
-                            else if (lane == idx - WAVE) gc[SL - 1] += s;
+                            if (tg < WAVE) { if (lane == tg) gc[0] += s; }
+                            else if (lane == tg - WAVE) gc[SL - 1] += s;
                         }
-                    } else if (kind == K_VAR && want_gx) {
-#pragma unroll
-                        for (int q = 0; q < PPT; ++q) {
-                            const T gvx = (act[q] && add[q]) ? go[q] : T(0);
+                    } else if (mode == PM_VAR) {
+                        if (want_gx) {
                             if constexpr (!GX) {
 #pragma unroll
-                                for (int v = 0; v < F_NX; ++v)
-                                    if (v == idx) gxr[q][v] += gvx;
+                                for (int q = 0; q < PPT; ++q) {
+                                    const T gvx = (act[q] && add[q]) ? go[q] : T(0);
+#pragma unroll
+                                    for (int v = 0; v < F_NX; ++v)
+                                        if (v == tg) gxr[q][v] += gvx;
+                                }
                             } else {
-                                if (gvx != T(0)) atomicAdd(&((T *)a.grad_X_direct)[n[q] * n_x + idx], gvx);
+                                const uint32_t w = c.word_lt<LT>(ii);
+                                const int v = desc_idx(k == 0 ? instr_d0(w) : instr_d1(w));
+#pragma unroll
+                                for (int q = 0; q < PPT; ++q) {
+                                    const T gvx = (act[q] && add[q]) ? go[q] : T(0);
+                                    if (gvx != T(0)) atomicAdd(&((T *)a.grad_X_direct)[n[q] * n_x + v], gvx);
+                                }
                             }
                         }
+                    } else if (mode == PM_NOOP) {
+#pragma unroll
+                        for (int q = 0; q < PPT; ++q)
+                            if (act[q] && add[q] && go[q] != T(0)) e = e > ERR_GRAD_NO_OP ? e : ERR_GRAD_NO_OP;
                     }
                 };
-                scatter(instr_d0(w), go0, add0);
-                if (binary) scatter(instr_d1(w), go1, add1);
+                scatter(0, go0, add0);
+                if (binary) scatter(1, go1, add1);
             }
             // ---- variable adjoints of this tile group
             if (!GX && want_gx) {
@@ -382,21 +409,22 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
     }
 }
 
-// Buckets by instruction count NI and constant count nc: NI <= 8, <= 16,
-// <= 64 (all with nc <= 64), the rest.  One wave per expression.
+// Buckets by tape rows (count_rows_wave) and constant count nc: rows <= 8,
+// <= 16, <= 32, <= 64 (all with nc <= 64), the rest.
 // Each wave classifies a run of 64 consecutive expressions (one whole wave
 // per program) and records it for the deterministic bucket lists
 // (sr_queue.h).
-__global__ __launch_bounds__(256) void k_classify_fused(const int64_t *__restrict__ ops, int M, int64_t B, int n_x,
-                                                        int8_t *bucket_of, int32_t *runcnt) {
+__global__ __launch_bounds__(256) void k_classify_fused(const int64_t *__restrict__ ops, const int64_t *__restrict__ ch,
+                                                        int M, int64_t B, int n_x, int8_t *bucket_of,
+                                                        int32_t *runcnt) {
     const int64_t b0 = ((int64_t)blockIdx.x * 4 + uniform_i32((int)(threadIdx.x / WAVE))) * WAVE;
     if (b0 >= B) return;
     const int n = (int)(B - b0 < WAVE ? B - b0 : WAVE);
     int mine = -1;
     for (int j = 0; j < n; ++j) {
         int nc = 0;
-        const int ni = count_instructions_wave(ops + (b0 + j) * M, M, n_x, nc);
-        const int bucket = nc > 64 ? 4 : (ni <= 8 ? 0 : (ni <= 16 ? 1 : (ni <= 32 ? 2 : (ni <= 64 ? 3 : 4))));
+        const int nr = count_rows_wave(ops + (b0 + j) * M, ch + (b0 + j) * M * 2, M, n_x, nc);
+        const int bucket = nc > 64 ? 4 : (nr <= 8 ? 0 : (nr <= 16 ? 1 : (nr <= 32 ? 2 : (nr <= 64 ? 3 : 4))));
         mine = lane_id() == j ? bucket : mine;
     }
     queue_record_run(mine, F_NBUCKETS, b0, n, bucket_of, runcnt);
@@ -478,7 +506,7 @@ int fused_launch(const FArgs &a, int grid, bool gx_lds, hipStream_t s) {
 #define SR_F1(T) T, 16, 1, FCfg<T>::B0_PPT, 4, false
 #define SR_F2(T) T, 32, 1, 1, 2, false
 #define SR_F3(T) T, 64, 1, 1, 2, false
-#define SR_F4(T) T, 128, 2, 1, 2, true
+#define SR_F4(T) T, 256, 2, 1, 2, true
 
 constexpr int F4_BLOCKS_PER_CU = 2;
 
@@ -495,10 +523,11 @@ FWs fused_layout(int64_t N, int64_t n_x, int64_t B, int64_t M, bool want_gx) {
     const int64_t gx_len = N * n_x;
     w.gx_lds = want_gx && n_x <= F_NX && (size_t)gx_len * sizeof(T) <= GX_LDS_MAX2;
     w.g0 = fused_grid<SR_F0(T)>(w.gx_lds, gx_len, (int)n_x);
-    w.g1 = M > 8 ? fused_grid<SR_F1(T)>(w.gx_lds, gx_len, (int)n_x) : 0;
-    w.g2 = M > 16 ? fused_grid<SR_F2(T)>(w.gx_lds, gx_len, (int)n_x) : 0;
-    w.g3 = M > 32 ? fused_grid<SR_F3(T)>(w.gx_lds, gx_len, (int)n_x) : 0;
-    w.g4 = M > 64 ? num_cus() * F4_BLOCKS_PER_CU : 0;
+    // a program of M nodes needs at most 2 M - 1 tape rows
+    w.g1 = M > 4 ? fused_grid<SR_F1(T)>(w.gx_lds, gx_len, (int)n_x) : 0;
+    w.g2 = M > 8 ? fused_grid<SR_F2(T)>(w.gx_lds, gx_len, (int)n_x) : 0;
+    w.g3 = M > 16 ? fused_grid<SR_F3(T)>(w.gx_lds, gx_len, (int)n_x) : 0;
+    w.g4 = M > 32 ? num_cus() * F4_BLOCKS_PER_CU : 0;
     const int64_t rows = (int64_t)w.g0 + w.g1 + w.g2 + w.g3 + w.g4;
     size_t off = (((size_t)F_NBUCKETS * B * 4 + 64) + 255) / 256 * 256;
     w.q_off = off;
@@ -510,7 +539,7 @@ FWs fused_layout(int64_t N, int64_t n_x, int64_t B, int64_t M, bool want_gx) {
     if (w.gx_lds) off += (size_t)((rows + ROW_CHUNK - 1) / ROW_CHUNK) * gx_len * sizeof(double);
     off = (off + 255) / 256 * 256;
     w.gt_off = off;
-    if (w.g4) off += (size_t)w.g4 * 2 * tape_elems(128, 1) * sizeof(T);
+    if (w.g4) off += (size_t)w.g4 * 2 * tape_elems(256, 1) * sizeof(T);
     w.total = off + 256;
     return w;
 }
@@ -557,7 +586,7 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
     hipLaunchKernelGGL(k_fill_i32_e, dim3(1), dim3(64), 0, s, counters, 0, (int64_t)(2 * F_NBUCKETS));
     void *qws = ws + L.q_off;
     hipLaunchKernelGGL(k_classify_fused, dim3((unsigned)((B + 4 * WAVE - 1) / (4 * WAVE))), dim3(256), 0, s, ops,
-                       (int)M, B, (int)n_x, queue_bucket_of(qws), queue_runcnt(qws, B));
+                       ch, (int)M, B, (int)n_x, queue_bucket_of(qws), queue_runcnt(qws, B));
     queue_finish(qws, B, F_NBUCKETS, lists, counters, s);
     FArgs a;
     memset(&a, 0, sizeof(a));
@@ -579,7 +608,7 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
     };
     int r = (int)hipGetLastError();
     if (r) return r;
-    if (M <= 8) {
+    if (M <= 4) {
         if ((r = fused_launch<SR_F0(T)>(bucket_args(0, 0), L.g0, L.gx_lds, s))) return r;
     } else {
         // buckets 4+3, 2 and 1 on side streams, bucket 0 on the caller's stream
@@ -592,7 +621,7 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
         if (L.g4 && (r = fused_launch<SR_F4(T)>(bucket_args(4, r3 + L.g3), L.g4, L.gx_lds, f.side[0]))) return r;
         if (L.g3 && (r = fused_launch<SR_F3(T)>(bucket_args(3, r3), L.g3, L.gx_lds, f.side[0]))) return r;
         if (L.g2 && (r = fused_launch<SR_F2(T)>(bucket_args(2, L.g0 + L.g1), L.g2, L.gx_lds, f.side[1]))) return r;
-        if ((r = fused_launch<SR_F1(T)>(bucket_args(1, L.g0), L.g1, L.gx_lds, f.side[2]))) return r;
+        if (L.g1 && (r = fused_launch<SR_F1(T)>(bucket_args(1, L.g0), L.g1, L.gx_lds, f.side[2]))) return r;
         if ((r = fused_launch<SR_F0(T)>(bucket_args(0, 0), L.g0, L.gx_lds, s))) return r;
         for (int i = 0; i < 3; ++i) {
             if ((r = (int)hipEventRecord(f.ev_join[i], f.side[i]))) return r;

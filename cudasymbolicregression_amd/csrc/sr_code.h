@@ -120,27 +120,6 @@ __device__ inline uint32_t operand_desc(uint32_t w, uint32_t instr_index, bool i
     return (uint32_t)K_IMM;  // unknown leaf: 0
 }
 
-// Instruction count of a batch-first program row under the same rule as
-// compile_code (used to bucket problems before they are compiled): one wave
-// per row, lane k reading node k (and 64 + k), so the reads are coalesced.
-// Returns wave-uniform values.
-__device__ inline int count_instructions_wave(const int64_t *row, int M, int n_x, int &nc) {
-    const int lane = lane_id();
-    int64_t op0 = NO_OP, op1 = NO_OP;
-    if (lane < M) op0 = row[lane];
-    if (lane + WAVE < M) op1 = row[lane + WAVE];
-    const uint64_t nn0 = __ballot(op0 != NO_OP), nn1 = __ballot(op1 != NO_OP);
-    nc = __popcll(__ballot(op0 == LEARNABLE_CONSTANT)) + __popcll(__ballot(op1 == LEARNABLE_CONSTANT));
-    const int nn = __popcll(nn0) + __popcll(nn1);
-    const int root = nn > 0 ? nn - 1 : 0;
-    const bool is0 = lane < M && lane <= root && arity_of(node_op(pack_node(op0, -1, -1, n_x))) > 0;
-    const bool is1 = lane + WAVE < M && lane + WAVE <= root && arity_of(node_op(pack_node(op1, -1, -1, n_x))) > 0;
-    const uint64_t m0 = __ballot(is0), m1 = __ballot(is1);
-    const int ni = __popcll(m0) + __popcll(m1);
-    const bool root_is_op = root < WAVE ? ((m0 >> root) & 1ull) : ((m1 >> (root - WAVE)) & 1ull);
-    return ni + (root_is_op ? 0 : 1);
-}
-
 struct Code {
     uint32_t iw0, iw1;  // lane i: instruction i (and 64 + i)
     int NI;
@@ -198,21 +177,19 @@ __device__ inline Code compile_code(const CProg<T> &p, int M) {
     return c;
 }
 
-// Tape of one wave: the value of instruction i at point p of a tile group is
-// v[(i*PPT + p)*64 + lane], its adjoint a[...] likewise; a holds one extra
-// row group (index LT) that absorbs the adjoint writes of non-tape operands.
-// The arrays are in LDS (bank-conflict free: 64 lanes touch 64 consecutive
-// words) or, for the longest programs, in a global workspace slot.  Each lane
-// only touches its own column, so a write needs no synchronisation before
-// the same lane reads it back.
+// Tape of one wave: row r at point p of a tile group is
+// v[(r*PPT + p)*64 + lane].  Rows 0..LT-1 hold instruction values and
+// adjoints (below), rows LT..LT+NXR_ROWS-1 the tile group's first variables.
+// The array is in LDS (bank-conflict free: 64 lanes touch 64 consecutive
+// words) or, for the longest programs, in a global workspace slot.  Each
+// lane only touches its own column, so a write needs no synchronisation
+// before the same lane reads it back.
+constexpr int NXR_ROWS = 4;
 template <typename T, int PPT>
 struct Tape {
     T *v;
-    T *a;
-    __device__ inline T val(int i, int p) const { return v[(i * PPT + p) * WAVE + lane_id()]; }
-    __device__ inline void set(int i, int p, T x) const { v[(i * PPT + p) * WAVE + lane_id()] = x; }
-    __device__ inline T adj(int i, int p) const { return a[(i * PPT + p) * WAVE + lane_id()]; }
-    __device__ inline void set_adj(int i, int p, T x) const { a[(i * PPT + p) * WAVE + lane_id()] = x; }
+    __device__ inline T get(int r, int p) const { return v[(r * PPT + p) * WAVE + lane_id()]; }
+    __device__ inline void put(int r, int p, T x) const { v[(r * PPT + p) * WAVE + lane_id()] = x; }
 };
 
 // Values of the immediate operands: 0 (unknown leaf), CONST_1..CONST_5,
@@ -226,8 +203,216 @@ __device__ inline T imm_value(int idx) {
     return v;
 }
 
-// Tape of one wave: LT value rows and LT + 1 adjoint rows per point of a
-// tile group, 64 elements each.
-__host__ __device__ inline size_t tape_elems(int LT, int PPT) { return (size_t)(2 * LT + 1) * PPT * WAVE; }
+// Tape of one wave: LT instruction rows and NXR_ROWS variable rows per point
+// of a tile group, 64 elements each.
+__host__ __device__ inline size_t tape_elems(int LT, int PPT) { return (size_t)(LT + NXR_ROWS) * PPT * WAVE; }
+
+// ------------------------------------------------------------------ plans
+// The instruction list re-encoded for evaluation (one fetch word and one
+// scatter word per instruction, lane i and 64 + i):
+//   * an operand is a tape row (an instruction, or a variable v < NXR_ROWS
+//     in row LT + v), a wave-uniform scalar (a constant or an immediate,
+//     resolved per evaluation into sv0 / sv1, see resolve_scalars), or a
+//     variable read from X (v >= NXR_ROWS);
+//   * adjoints overwrite values in place.  In a tree every instruction below
+//     the root has one parent, and the parent's backward step is both the
+//     last use of the child's value and the only write of its adjoint.  An
+//     instruction with several parents, or with none below the root, keeps
+//     its value row and gets an extra adjoint row (NI, NI + 1, ...) that is
+//     zeroed before each tile group and accumulated into, in the same order
+//     as before.  The bucket of a program is chosen by its row count
+//     NI + extra rows (count_rows_wave), at most 2 NI - 1.
+// Fetch word: op 0-5 | binary 6 | scalar0 7 | row0 8-16 | scalar1 17 |
+// row1 18-26 | xg0 27 | xg1 28 (xg: variable read from X).
+// Scatter word: adjoint row of the instruction 0-7 | mode0 8-10 |
+// target0 11-18 | mode1 19-21 | target1 22-29.
+constexpr uint32_t PM_NONE = 0, PM_WRITE = 1, PM_ACC = 2, PM_CONST = 3, PM_VAR = 4, PM_NOOP = 5;
+
+__host__ __device__ inline int pf_op(uint32_t f) { return (int)(f & 63u); }
+__host__ __device__ inline bool pf_bin(uint32_t f) { return (f >> 6) & 1u; }
+__host__ __device__ inline bool pf_sc(uint32_t f, int k) { return (f >> (k ? 17 : 7)) & 1u; }
+__host__ __device__ inline int pf_row(uint32_t f, int k) { return (int)((f >> (k ? 18 : 8)) & 511u); }
+__host__ __device__ inline bool pf_xg(uint32_t f, int k) { return (f >> (k ? 28 : 27)) & 1u; }
+__host__ __device__ inline int ps_grow(uint32_t s) { return (int)(s & 255u); }
+__host__ __device__ inline uint32_t ps_mode(uint32_t s, int k) { return (s >> (k ? 19 : 8)) & 7u; }
+__host__ __device__ inline int ps_tgt(uint32_t s, int k) { return (int)((s >> (k ? 22 : 11)) & 255u); }
+
+struct Plan {
+    uint32_t f0, f1;  // fetch words
+    uint32_t s0, s1;  // scatter words
+    int nextra;       // extra adjoint rows (NI .. NI + nextra - 1)
+    template <int LT>
+    __device__ inline uint32_t fw(int i) const {
+        if constexpr (LT <= WAVE) return rl_u32(f0, i);
+        else return i < WAVE ? rl_u32(f0, i) : rl_u32(f1, i - WAVE);
+    }
+    template <int LT>
+    __device__ inline uint32_t sw(int i) const {
+        if constexpr (LT <= WAVE) return rl_u32(s0, i);
+        else return i < WAVE ? rl_u32(s0, i) : rl_u32(s1, i - WAVE);
+    }
+};
+
+// References of every node (lane k, 64 + k) by the operator nodes om0 / om1
+// of the node words w0 / w1, and the resulting extra-row masks (operator
+// nodes below the root with a reference count other than one).
+__device__ inline void extra_rows(uint32_t w0, uint32_t w1, uint64_t om0, uint64_t om1, int root, uint64_t &em0,
+                                  uint64_t &em1) {
+    const int lane = lane_id();
+    int rc0 = 0, rc1 = 0;
+    auto count = [&](uint32_t w) {
+        const int c0 = node_c0(w), c1 = arity_of(node_op(w)) == 2 ? node_c1(w) : -1;
+        rc0 += (c0 == lane ? 1 : 0) + (c1 == lane ? 1 : 0);
+        rc1 += (c0 == lane + WAVE ? 1 : 0) + (c1 == lane + WAVE ? 1 : 0);
+    };
+    for (uint64_t m = om0; m; m &= m - 1) count(rl_u32(w0, __ffsll((long long)m) - 1));
+    for (uint64_t m = om1; m; m &= m - 1) count(rl_u32(w1, __ffsll((long long)m) - 1));
+    em0 = __ballot(((om0 >> lane) & 1ull) && lane < root && rc0 != 1);
+    em1 = __ballot(((om1 >> lane) & 1ull) && lane + WAVE < root && rc1 != 1);
+}
+
+// Tape rows a batch-first program row needs (NI + extra rows), under the
+// same rules as compile_code and compile_plan; one wave per row, node k in
+// lane k (and 64 + k), reads coalesced.  Returns wave-uniform values.
+__device__ inline int count_rows_wave(const int64_t *ops_row, const int64_t *ch_row, int M, int n_x, int &nc) {
+    const int lane = lane_id();
+    uint32_t w[2] = {0u, 0u};
+    bool nn[2] = {false, false}, isc[2] = {false, false};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int k = lane + h * WAVE;
+        if (k < M) {
+            const int64_t op = ops_row[k];
+            w[h] = pack_node(op, ch_row[2 * k], ch_row[2 * k + 1], n_x);
+            nn[h] = op != NO_OP;
+            isc[h] = op == LEARNABLE_CONSTANT;
+        }
+    }
+    const uint64_t nn0 = __ballot(nn[0]), nn1 = __ballot(nn[1]);
+    nc = __popcll(__ballot(isc[0])) + __popcll(__ballot(isc[1]));
+    const int cnt = __popcll(nn0) + __popcll(nn1);
+    const int root = cnt > 0 ? cnt - 1 : 0;
+    const uint64_t om0 = __ballot(lane < M && lane <= root && arity_of(node_op(w[0])) > 0);
+    const uint64_t om1 = __ballot(lane + WAVE < M && lane + WAVE <= root && arity_of(node_op(w[1])) > 0);
+    const bool root_is_op = root < WAVE ? ((om0 >> root) & 1ull) : ((om1 >> (root - WAVE)) & 1ull);
+    const int ni = __popcll(om0) + __popcll(om1) + (root_is_op ? 0 : 1);
+    uint64_t em0, em1;
+    extra_rows(w[0], w[1], om0, om1, root, em0, em1);
+    return ni + __popcll(em0) + __popcll(em1);
+}
+
+// Operand record of a node (used by its parent): scalar 0 | row 1-9 |
+// xg 10 | scatter mode 11-13 | scatter target 14-21.
+__device__ inline uint32_t operand_rec(uint32_t w, bool is_op, uint32_t ix, bool extra, uint32_t erow, int var_row0,
+                                       int nvr) {
+    if (is_op) return (ix << 1) | ((extra ? PM_ACC : PM_WRITE) << 11) | ((extra ? erow : ix) << 14);
+    const uint32_t d = operand_desc(w, 0u, false);
+    const int kind = desc_kind(d);
+    const uint32_t idx = (uint32_t)desc_idx(d);
+    if (kind == K_VAR) {
+        const uint32_t t = (idx > 255u ? 255u : idx) << 14;
+        return (int)idx < nvr ? (((uint32_t)var_row0 + idx) << 1) | (PM_VAR << 11) | t : (1u << 10) | (PM_VAR << 11) | t;
+    }
+    if (kind == K_CONST) return 1u | (PM_CONST << 11) | (idx << 14);
+    return 1u | ((idx == (uint32_t)IMM_NOOP ? PM_NOOP : PM_NONE) << 11);
+}
+
+// Builds the plan of a compiled program (c = compile_code(p, M)).  Variables
+// v < nvr are tape rows var_row0 + v; the others are read from X.
+template <typename T>
+__device__ inline Plan compile_plan(const CProg<T> &p, int M, const Code &c, int var_row0, int nvr) {
+    const int lane = lane_id();
+    const int root = p.root;
+    const int NI = c.NI;
+    const uint32_t w0 = p.w0, w1 = p.w1;
+    const bool op0 = lane < M && lane <= root && arity_of(node_op(w0)) > 0;
+    const bool op1 = lane + WAVE < M && lane + WAVE <= root && arity_of(node_op(w1)) > 0;
+    const uint64_t om0 = __ballot(op0), om1 = __ballot(op1);
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
+    const uint32_t ix0 = (uint32_t)__popcll(om0 & below);
+    const uint32_t ix1 = (uint32_t)(__popcll(om0) + __popcll(om1 & below));
+    uint64_t em0, em1;
+    extra_rows(w0, w1, om0, om1, root, em0, em1);
+    const bool ex0 = (em0 >> lane) & 1ull, ex1 = (em1 >> lane) & 1ull;
+    const uint32_t er0 = (uint32_t)(NI + __popcll(em0 & below));
+    const uint32_t er1 = (uint32_t)(NI + __popcll(em0) + __popcll(em1 & below));
+    const uint32_t r0 = operand_rec(w0, op0, ix0, ex0, er0, var_row0, nvr);
+    const uint32_t r1 = operand_rec(w1, op1, ix1, ex1, er1, var_row0, nvr);
+    auto child_rec = [&](int ch) -> uint32_t {  // every lane takes part in both shuffles
+        const int cc = ch & 127;
+        const uint32_t lo = (uint32_t)__shfl((int)r0, cc & (WAVE - 1), WAVE);
+        const uint32_t hi = (uint32_t)__shfl((int)r1, cc & (WAVE - 1), WAVE);
+        return cc < WAVE ? lo : hi;
+    };
+    auto fetch_word = [](uint32_t op, bool bin, uint32_t a, uint32_t b) -> uint32_t {
+        b = bin ? b : 0u;
+        return op | ((bin ? 1u : 0u) << 6) | ((a & 1u) << 7) | (((a >> 1) & 511u) << 8) | ((b & 1u) << 17) |
+               (((b >> 1) & 511u) << 18) | (((a >> 10) & 1u) << 27) | (((b >> 10) & 1u) << 28);
+    };
+    auto scatter_word = [](uint32_t grow, bool bin, uint32_t a, uint32_t b) -> uint32_t {
+        b = bin ? b : 0u;
+        return grow | (((a >> 11) & 7u) << 8) | (((a >> 14) & 255u) << 11) | (((b >> 11) & 7u) << 19) |
+               (((b >> 14) & 255u) << 22);
+    };
+    const uint32_t a00 = child_rec(node_c0(w0)), a01 = child_rec(node_c1(w0));
+    const uint32_t a10 = child_rec(node_c0(w1)), a11 = child_rec(node_c1(w1));
+    const bool b0 = arity_of(node_op(w0)) == 2, b1 = arity_of(node_op(w1)) == 2;
+    const uint32_t F0 = fetch_word((uint32_t)node_op(w0) & 63u, b0, a00, a01);
+    const uint32_t F1 = fetch_word((uint32_t)node_op(w1) & 63u, b1, a10, a11);
+    const uint32_t S0 = scatter_word(ex0 ? er0 : ix0, b0, a00, a01);
+    const uint32_t S1 = scatter_word(ex1 ? er1 : ix1, b1, a10, a11);
+    Plan pl;
+    pl.f0 = pl.f1 = pl.s0 = pl.s1 = 0u;
+    pl.nextra = __popcll(em0) + __popcll(em1);
+    auto place = [&](int i, uint32_t f, uint32_t s) {
+        if (i < WAVE) {
+            if (lane == i) { pl.f0 = f; pl.s0 = s; }
+        } else if (lane == i - WAVE) {
+            pl.f1 = f;
+            pl.s1 = s;
+        }
+    };
+    int i = 0;
+    for (uint64_t m = om0; m; m &= m - 1, ++i) {
+        const int k = __ffsll((long long)m) - 1;
+        place(i, rl_u32(F0, k), rl_u32(S0, k));
+    }
+    for (uint64_t m = om1; m; m &= m - 1, ++i) {
+        const int k = __ffsll((long long)m) - 1;
+        place(i, rl_u32(F1, k), rl_u32(S1, k));
+    }
+    if (i < NI) {  // the root is a leaf: one copy instruction
+        const uint32_t rr = root < WAVE ? rl_u32(r0, root) : rl_u32(r1, root - WAVE);
+        place(i, fetch_word((uint32_t)OP_COPY, false, rr, 0u), scatter_word((uint32_t)i, false, rr, 0u));
+    }
+    return pl;
+}
+
+// Scalar operand values of every instruction (lane i, and 64 + i when LT >
+// 64): the constant (compact slot) or immediate each operand names, 0 for
+// row operands.  Recomputed whenever the constants change.
+template <typename T, int SL, int LT>
+__device__ inline void resolve_scalars(const Code &c, const SVec<T, SL> &xc, T (&sv0)[LT > WAVE ? 2 : 1],
+                                       T (&sv1)[LT > WAVE ? 2 : 1]) {
+    constexpr int H = LT > WAVE ? 2 : 1;
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+        const uint32_t w = h == 0 ? c.iw0 : c.iw1;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const uint32_t d = k == 0 ? instr_d0(w) : instr_d1(w);
+            const int kind = desc_kind(d), idx = desc_idx(d);
+            const int j = kind == K_CONST ? idx : 0;
+            T cv = __shfl(xc.v[0], j & (WAVE - 1), WAVE);
+            if constexpr (SL > 1) {
+                const T hv = __shfl(xc.v[SL - 1], j & (WAVE - 1), WAVE);
+                cv = j < WAVE ? cv : hv;
+            }
+            const T v = kind == K_CONST ? cv : (kind == K_IMM ? imm_value<T>(idx) : T(0));
+            if (k == 0) sv0[h] = v;
+            else sv1[h] = v;
+        }
+    }
+}
 
 }  // namespace sr

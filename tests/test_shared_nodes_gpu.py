@@ -1,0 +1,126 @@
+"""Programs whose nodes are not a tree: an operator node used by two parents
+(or twice by one), and operator nodes below the root that no parent uses.
+The evaluators keep adjoints in the value rows of one-parent instructions and
+give every other instruction its own accumulated adjoint row; these tests pin
+that second path against the oracle, which follows the reference's dense
+reverse sweep (src/evaluation_kernels_cpu.cpp:222-662) for any child lists
+that pass validation (children below their parent, arity-consistent).
+
+Fused evaluator: FP64 within 1e-10 relative, FP32 within the FP32 tolerance
+of the other evaluation tests, with identical NaN masks and error codes.
+Fit: the optimize-vs-oracle tolerance of test_optimize_gpu.py."""
+import numpy as np
+import pytest
+import torch
+
+import helpers
+import lbfgs_oracle
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+C_, X0, X1, X2 = 1, 50, 51, 52
+SIN, COS, EXP, LOG, SQUARE, SQRT = 10, 11, 12, 13, 14, 15
+ADD, SUB, MUL, DIV = 40, 41, 42, 43
+M = 10
+
+# (op, children) in postfix node order
+PROGRAMS = [
+    [(X0, ()), (SIN, (0,)), (MUL, (1, 1))],                                   # one node used twice by one parent
+    [(X0, ()), (C_, ()), (ADD, (0, 1)), (EXP, (2,)), (MUL, (3, 2))],          # node 2 used by EXP and MUL
+    [(X1, ()), (COS, (0,)), (X0, ()), (C_, ()), (MUL, (2, 3))],               # COS below the root, unused
+    [(X0, ()), (SIN, (0,)), (C_, ())],                                        # leaf root, unused SIN
+    [(X2, ()), (C_, ()), (MUL, (0, 1)), (SIN, (2,)), (COS, (2,)), (ADD, (3, 4)), (X1, ()), (DIV, (5, 6))],
+    [(X2, ()), (SQRT, (0,)), (ADD, (1, 1)), (ADD, (2, 2)), (ADD, (3, 3))],    # a chain of shared nodes
+    [(C_, ()), (X0, ()), (MUL, (0, 1)), (SQUARE, (2,)), (C_, ()), (MUL, (4, 2)), (ADD, (3, 5))],
+    [(X0, ()), (C_, ()), (MUL, (0, 1)), (SIN, (2,))],                         # a tree, for control
+    [(C_, ()), (MUL, (0, 0)), (X1, ()), (MUL, (2, 2)), (ADD, (1, 3))],        # shared leaves
+    [(X0, ()), (EXP, (0,)), (LOG, (1,)), (X1, ()), (C_, ()), (MUL, (3, 4))],  # unused chain EXP -> LOG
+    [(X0, ()), (SQRT, (0,)), (C_, ())],                                       # unused SQRT of negative points
+]
+
+
+def _dev():
+    return torch.device("cuda:0")
+
+
+def _t(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(_dev())
+
+
+def _batch(reps=1):
+    ops = np.zeros((len(PROGRAMS) * reps, M), np.int64)
+    ch = np.full((len(PROGRAMS) * reps, M, 2), -1, np.int64)
+    for r in range(reps):
+        for i, prog in enumerate(PROGRAMS):
+            b = r * len(PROGRAMS) + i
+            for k, (op, kids) in enumerate(prog):
+                ops[b, k] = op
+                for s, c in enumerate(kids):
+                    ch[b, k, s] = c
+    return ops, ch
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_fused_shared_and_unused_nodes_match_oracle(mode):
+    from cudasymbolicregression_amd import fused
+
+    print(f"case shared-node fused mode={mode}", flush=True)
+    rng = np.random.default_rng(7 + mode)
+    ops, ch = _batch()
+    B, N = ops.shape[0], 300
+    C = helpers.constants_like(ops, rng)
+    X = rng.uniform(-3, 3, (N, 3)).astype(np.float32)
+    dy = rng.standard_normal((B, N)).astype(np.float32)
+    ref64 = oracle.fused_root(X.astype(np.float64), ops, ch, C.astype(np.float64), dy.astype(np.float64), mode=mode)
+    ref32 = oracle.fused_root(X, ops, ch, C, dy, mode=mode)
+    res = {}
+    for dt in (np.float64, np.float32):
+        out = fused.fused_root(_t(X.astype(dt)), _t(ops), _t(ch), _t(C.astype(dt)), _t(dy.astype(dt)), mode=mode,
+                               want_grad_X=True, raise_errors=False)
+        res[dt] = {k: v.cpu().numpy() for k, v in out.items()}
+    for dt, ref in ((np.float64, ref64), (np.float32, ref32)):
+        err = res[dt]["err"]
+        assert (err == ref["err"]).all(), (dt, err, ref["err"])
+    ok = ref64["err"] == 0
+    # FP64 against the FP64 oracle
+    for key, sel in (("y", ok), ("grad_C", ok)) + ((("loss", ok),) if mode == 1 else ()):
+        good, msg = helpers.allclose_nan(res[np.float64][key][sel], ref64[key][sel], 1e-10, 1e-12)
+        assert good, f"{key} f64: {msg}"
+    if ok.all():
+        good, msg = helpers.allclose_nan(res[np.float64]["grad_X"], ref64["grad_X"], 1e-10, 1e-12)
+        assert good, f"grad_X f64: {msg}"
+    # FP32: values against the FP32 oracle; sums (FP64 on the GPU) at least
+    # as accurate as the oracle's FP32 sums
+    good, msg = helpers.allclose_nan(res[np.float32]["y"][ok], ref32["y"][ok], 1e-5, 1e-6)
+    assert good, f"y f32: {msg}"
+    for key in ("grad_C",) + (("loss",) if mode == 1 else ()):
+        good, msg = helpers.check_vs_f64(res[np.float32][key][ok], ref32[key][ok], ref64[key][ok], atol=1e-5)
+        assert good, f"{key} f32: {msg}"
+    if ok.all():
+        good, msg = helpers.check_vs_f64(res[np.float32]["grad_X"], ref32["grad_X"], ref64["grad_X"], atol=1e-4)
+        assert good, f"grad_X f32: {msg}"
+
+
+def test_fit_shared_and_unused_nodes_match_oracle():
+    import cudasymbolicregression_amd as st
+
+    print("case shared-node fit", flush=True)
+    rng = np.random.default_rng(11)
+    ops, ch = _batch(reps=12)
+    B, N = ops.shape[0], 64
+    X = rng.uniform(0.2, 2, (N, 3)).astype(np.float32)
+    C_true = helpers.constants_like(ops, rng, scale=1.0)
+    Y = oracle.fused_root(X, ops, ch, C_true, np.zeros((B, N), np.float32), mode=0)["y"]
+    C0 = helpers.constants_like(ops, rng, scale=1.0)
+    C_o, y_o, f_o, _ = lbfgs_oracle.optimize(X, Y, ops, ch, C0, max_iter=20)
+    Cg, Yg, info = st.optimize(_t(X), _t(Y), _t(ops), _t(ch), _t(C0), max_iter=20, return_info=True,
+                               raise_errors=False)
+    mse_g = ((Yg.cpu().numpy().astype(np.float64) - Y) ** 2).mean(1)
+    mse_o = ((y_o.astype(np.float64) - Y) ** 2).mean(1)
+    both = np.isfinite(Y).all(1) & np.isfinite(mse_g) & np.isfinite(mse_o) & (info["err"].cpu().numpy() == 0)
+    close = np.abs(mse_g - mse_o) <= 1e-3 * mse_o + 1e-7
+    frac = close[both].mean()
+    print(f"shared-node fit: {both.sum()} finite problems, {frac:.3f} within tolerance")
+    assert both.sum() > B // 2
+    assert frac >= 0.9
