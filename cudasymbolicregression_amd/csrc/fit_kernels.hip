@@ -97,6 +97,30 @@ __device__ inline double sdot(const SVec<float, SL> &a, const SVec<float, SL> &b
     return wave_sum_rows(s, K <= 16 ? 1 : 4);
 }
 
+// ------------------------------------------------------------ tile data
+// Variable rows (v < NX, in tape rows LT + v) and targets of the tile group
+// starting at point base.
+template <int PPT, int NX, int LT>
+__device__ inline void load_group(const float *X, const float *Yb, int64_t N, int n_x, int64_t base,
+                                  const Tape<float, PPT> &t, float (&target)[PPT]) {
+    const int lane = lane_id();
+    float xv[PPT][NX];
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+        const int64_t n = base + p * WAVE + lane;
+        const bool act = n < N;
+        const float *row = X + (act ? n : 0) * (int64_t)n_x;
+#pragma unroll
+        for (int v = 0; v < NX; ++v) xv[p][v] = (act && v < n_x) ? row[v] : 0.0f;
+        target[p] = act ? Yb[n] : 0.0f;
+    }
+#pragma unroll
+    for (int v = 0; v < NX; ++v)
+        if (v < n_x)
+#pragma unroll
+            for (int p = 0; p < PPT; ++p) t.put(LT + v, p, xv[p][v]);
+}
+
 // ---------------------------------------------------------------- closure
 // One wave's share of an objective evaluation at the compact constants xt:
 // tile groups of 64*PPT points starting at group `wslot`, every W-th group.
@@ -105,10 +129,15 @@ __device__ inline double sdot(const SVec<float, SL> &a, const SVec<float, SL> &b
 // one opcode dispatch for all PPT points; operands come from the plan
 // (sr_code.h): a tape row, a scalar resolved once per evaluation, or a
 // variable read from X (v >= NXR_ROWS); adjoints overwrite values in place.
+//
+// When the wave's points form one tile group (single), their variable rows
+// were written once per problem (load_single_group) and their targets are
+// in tg1; nothing is read from X or Y here.
 template <int SL, int NX, int PPT, int LT>
 __device__ inline void closure_code(const Code &c, const Plan &pl, int nc, const SVec<float, SL> &xt, const float *X,
                                     const float *Yb, int64_t N, int n_x, int wslot, int W, const Tape<float, PPT> &t,
-                                    double &loss, double (&gc)[SL], int32_t &err) {
+                                    bool single, const float (&tg1)[PPT], double &loss, double (&gc)[SL],
+                                    int32_t &err) {
     const int lane = lane_id();
     const int NI = c.NI;
     constexpr int HW = LT > WAVE ? 2 : 1;
@@ -126,22 +155,15 @@ __device__ inline void closure_code(const Code &c, const Plan &pl, int nc, const
     for (int64_t base = (int64_t)wslot * group; base < N; base += (int64_t)W * group) {
         bool act[PPT];
         int64_t n[PPT];
-        float xv[PPT][NX], target[PPT];
+        float target[PPT];
 #pragma unroll
         for (int p = 0; p < PPT; ++p) {
             n[p] = base + p * WAVE + lane;
             act[p] = n[p] < N;
-            const float *row = X + (act[p] ? n[p] : 0) * (int64_t)n_x;
-#pragma unroll
-            for (int v = 0; v < NX; ++v) xv[p][v] = (act[p] && v < n_x) ? row[v] : 0.0f;
-            target[p] = act[p] ? Yb[n[p]] : 0.0f;
+            target[p] = tg1[p];
         }
-        // variable rows, and zeroed extra adjoint rows
-#pragma unroll
-        for (int v = 0; v < NX; ++v)
-            if (v < n_x)
-#pragma unroll
-                for (int p = 0; p < PPT; ++p) t.put(LT + v, p, xv[p][v]);
+        if (!single) load_group<PPT, NX, LT>(X, Yb, N, n_x, base, t, target);
+        // zeroed extra adjoint rows
         for (int r = NI; r < NI + pl.nextra; ++r)
 #pragma unroll
             for (int p = 0; p < PPT; ++p) t.put(r, p, 0.0f);
@@ -380,6 +402,12 @@ void k_fit(FitArgs a) {
         const int64_t N = a.n_pts ? (int64_t)a.n_pts[b] : a.N;
         const double stop = a.stop_loss ? a.stop_loss[b] : -1.0;
         const int K = p.K;
+        // the wave's points form one tile group: load them once per problem
+        const bool single = N <= (int64_t)W * WAVE * PPT;
+        float tg1[PPT];
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) tg1[q] = 0.0f;
+        if (single) load_group<PPT, NX, LT>(X, Yb, N, a.n_x, (int64_t)wslot * WAVE * PPT, tape, tg1);
 
         SVec<float, SL> x = compact_start<float, SL>(p), g, d, g_prev, g_lo, g_new, ga;
 #pragma unroll
@@ -400,7 +428,8 @@ void k_fit(FitArgs a) {
             for (int h = 0; h < SL; ++h) xt.v[h] = (st == ST_INIT) ? x.v[h] : x.v[h] + a_eval * d.v[h];
             double phi, gcd[SL];
             int32_t e2;
-            closure_code<SL, NX, PPT, LT>(code, plan, p.nc, xt, X, Yb, N, a.n_x, wslot, W, tape, phi, gcd, e2);
+            closure_code<SL, NX, PPT, LT>(code, plan, p.nc, xt, X, Yb, N, a.n_x, wslot, W, tape, single, tg1, phi,
+                                          gcd, e2);
             if (W > 1) {
                 // partial sums of the W waves meet in LDS; every wave adds them
                 // in the same order, so all waves hold identical results

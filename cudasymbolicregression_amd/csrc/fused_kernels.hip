@@ -210,7 +210,7 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
             // operand k of instruction ii: a tape row, a scalar, or (more
             // variables than rows) a variable read from X
             auto fetch = [&](uint32_t f, int k, int ii, T (&av)[PPT]) {
-                if (pf_xg(f, k)) {
+                if (GX && pf_xg(f, k)) {  // (without GX every variable has a row)
                     const uint32_t w = c.word_lt<LT>(ii);
                     const int v = desc_idx(k == 0 ? instr_d0(w) : instr_d1(w));
 #pragma unroll
