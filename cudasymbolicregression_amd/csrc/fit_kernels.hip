@@ -816,11 +816,13 @@ const FitVariant kB2[] = {{launch_variant<32, 1, 2, false, 3, 1>, 2, 1},
                           {launch_variant<32, 1, 4, false, 3, 2>, 4, 2},
                           {launch_variant<32, 1, 8, false, 2, 1>, 8, 1}};
 const FitVariant kB3[] = {{launch_variant<64, 1, 4, false, 2, 1>, 4, 1},
-                          {launch_variant<64, 1, 2, false, 2, 1>, 2, 1}};
+                          {launch_variant<64, 1, 2, false, 2, 1>, 2, 1},
+                          {launch_variant<64, 1, 4, false, 2, 2>, 4, 2},
+                          {launch_variant<64, 1, 8, false, 2, 1>, 8, 1}};
 const FitVariant kB4[] = {{launch_variant<256, 2, 1, true, 2, 1>, 1, 1}};
 
 const FitVariant *const kVariants[NBUCKETS] = {kB0, kB1, kB2, kB3, kB4};
-constexpr int kNumVariants[NBUCKETS] = {5, 4, 4, 2, 1};
+constexpr int kNumVariants[NBUCKETS] = {5, 4, 4, 4, 1};
 int g_variant[NBUCKETS] = {2, 2, 2, 0, 0};  // fastest on the bench workload (autotune.tune_fit)
 int g_stationary_exit = 1;
 
@@ -921,8 +923,13 @@ int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const i
     if ((r = (int)hipEventRecord(f.ev_fork, s))) return r;
     for (int i = 0; i < 3; ++i)
         if ((r = (int)hipStreamWaitEvent(f.side[i], f.ev_fork, 0))) return r;
-    if (M > 32 && (r = launch_bucket(4, bucket_args(4), f.side[0]))) return r;
+    // Buckets 4 and 3 share a stream.  With M <= 64, bucket 4 only takes
+    // programs with extra adjoint rows (shared or unused nodes, rare), so it
+    // goes second: a persistent grid that is not yet resident holds back the
+    // next kernel on its stream.
+    if (M > 64 && (r = launch_bucket(4, bucket_args(4), f.side[0]))) return r;
     if (M > 16 && (r = launch_bucket(3, bucket_args(3), f.side[0]))) return r;
+    if (M > 32 && M <= 64 && (r = launch_bucket(4, bucket_args(4), f.side[0]))) return r;
     if (M > 8 && (r = launch_bucket(2, bucket_args(2), f.side[1]))) return r;
     if ((r = launch_bucket(1, bucket_args(1), f.side[2]))) return r;
     if ((r = launch_bucket(0, bucket_args(0), s))) return r;

@@ -618,8 +618,12 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
         for (int i = 0; i < 3; ++i)
             if ((r = (int)hipStreamWaitEvent(f.side[i], f.ev_fork, 0))) return r;
         const int64_t r3 = (int64_t)L.g0 + L.g1 + L.g2;
-        if (L.g4 && (r = fused_launch<SR_F4(T)>(bucket_args(4, r3 + L.g3), L.g4, L.gx_lds, f.side[0]))) return r;
+        // buckets 4 and 3 share a stream; with M <= 64 bucket 4 only takes
+        // programs with extra adjoint rows (rare) and goes second
+        if (M > 64 && (r = fused_launch<SR_F4(T)>(bucket_args(4, r3 + L.g3), L.g4, L.gx_lds, f.side[0]))) return r;
         if (L.g3 && (r = fused_launch<SR_F3(T)>(bucket_args(3, r3), L.g3, L.gx_lds, f.side[0]))) return r;
+        if (L.g4 && M <= 64 && (r = fused_launch<SR_F4(T)>(bucket_args(4, r3 + L.g3), L.g4, L.gx_lds, f.side[0])))
+            return r;
         if (L.g2 && (r = fused_launch<SR_F2(T)>(bucket_args(2, L.g0 + L.g1), L.g2, L.gx_lds, f.side[1]))) return r;
         if (L.g1 && (r = fused_launch<SR_F1(T)>(bucket_args(1, L.g0), L.g1, L.gx_lds, f.side[2]))) return r;
         if ((r = fused_launch<SR_F0(T)>(bucket_args(0, 0), L.g0, L.gx_lds, s))) return r;
