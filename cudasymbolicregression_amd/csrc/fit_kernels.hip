@@ -167,21 +167,39 @@ __device__ inline void closure_code(const Code &c, const Plan &pl, int nc, const
         for (int r = NI; r < NI + pl.nextra; ++r)
 #pragma unroll
             for (int p = 0; p < PPT; ++p) t.put(r, p, 0.0f);
-        auto fetch = [&](uint32_t f, int k, int i, float (&a)[PPT]) {
-            if (pf_xg(f, k)) {  // more variables than rows: read it from the X row
+        // operands of instruction i: tape rows or scalars, both rows read
+        // before either is used (a unary instruction reads row 0 and discards
+        // it); variables v >= NX are read from X
+        auto fetch = [&](uint32_t f, int i, float (&a0)[PPT], float (&a1)[PPT]) {
+            const bool bin = pf_bin(f);
+            const float s0 = HW == 1 ? rl(sv0[0], i) : (i < WAVE ? rl(sv0[0], i) : rl(sv0[HW - 1], i - WAVE));
+            const float s1 = HW == 1 ? rl(sv1[0], i) : (i < WAVE ? rl(sv1[0], i) : rl(sv1[HW - 1], i - WAVE));
+            if (pf_xg(f, 0) || pf_xg(f, 1)) {
                 const uint32_t w = c.word_lt<LT>(i);
-                const int v = desc_idx(k == 0 ? instr_d0(w) : instr_d1(w));
 #pragma unroll
-                for (int p = 0; p < PPT; ++p) a[p] = act[p] ? X[n[p] * (int64_t)n_x + v] : 0.0f;
+                for (int k = 0; k < 2; ++k) {
+                    float *av = k == 0 ? a0 : a1;
+                    const int v = desc_idx(k == 0 ? instr_d0(w) : instr_d1(w));
+                    const bool xg = pf_xg(f, k), sc = pf_sc(f, k);
+                    const int row = pf_row(f, k);
+#pragma unroll
+                    for (int p = 0; p < PPT; ++p) {
+                        const float tv = xg ? (act[p] ? X[n[p] * (int64_t)n_x + v] : 0.0f) : t.get(row, p);
+                        av[p] = (k == 1 && !bin) ? 0.0f : (sc ? (k == 0 ? s0 : s1) : tv);
+                    }
+                }
             } else {
-                const int row = pf_row(f, k);
-                const bool sc = pf_sc(f, k);
-                const float sv = k == 0 ? (HW == 1 ? rl(sv0[0], i) : (i < WAVE ? rl(sv0[0], i) : rl(sv0[HW - 1], i - WAVE)))
-                                        : (HW == 1 ? rl(sv1[0], i) : (i < WAVE ? rl(sv1[0], i) : rl(sv1[HW - 1], i - WAVE)));
+                const int r0 = pf_row(f, 0), r1 = pf_row(f, 1);
+                const bool c0 = pf_sc(f, 0), c1 = pf_sc(f, 1);
+                float t0[PPT], t1[PPT];
+#pragma unroll
+                for (int p = 0; p < PPT; ++p) t0[p] = t.get(r0, p);
+#pragma unroll
+                for (int p = 0; p < PPT; ++p) t1[p] = t.get(r1, p);
 #pragma unroll
                 for (int p = 0; p < PPT; ++p) {
-                    const float tv = t.get(row, p);
-                    a[p] = sc ? sv : tv;
+                    a0[p] = c0 ? s0 : t0[p];
+                    a1[p] = bin ? (c1 ? s1 : t1[p]) : 0.0f;
                 }
             }
         };
@@ -190,13 +208,7 @@ __device__ inline void closure_code(const Code &c, const Plan &pl, int nc, const
             const uint32_t f = pl.fw<LT>(i);
             const int op = pf_op(f);
             float a0[PPT], a1[PPT], r[PPT];
-            fetch(f, 0, i, a0);
-            if (pf_bin(f)) {
-                fetch(f, 1, i, a1);
-            } else {
-#pragma unroll
-                for (int p = 0; p < PPT; ++p) a1[p] = 0.0f;
-            }
+            fetch(f, i, a0, a1);
             if (op == OP_COPY) {
 #pragma unroll
                 for (int p = 0; p < PPT; ++p) r[p] = a0[p];
@@ -225,13 +237,7 @@ __device__ inline void closure_code(const Code &c, const Plan &pl, int nc, const
             const int grow = ps_grow(sw);
 #pragma unroll
             for (int p = 0; p < PPT; ++p) g[p] = t.get(grow, p);
-            fetch(f, 0, i, a0);
-            if (binary) {
-                fetch(f, 1, i, a1);
-            } else {
-#pragma unroll
-                for (int p = 0; p < PPT; ++p) a1[p] = 0.0f;
-            }
+            fetch(f, i, a0, a1);
             if (op == OP_COPY) {
 #pragma unroll
                 for (int p = 0; p < PPT; ++p) {

@@ -207,23 +207,41 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
             for (int r = NI; r < NI + pl.nextra; ++r)
 #pragma unroll
                 for (int q = 0; q < PPT; ++q) t.put(r, q, T(0));
-            // operand k of instruction ii: a tape row, a scalar, or (more
-            // variables than rows) a variable read from X
-            auto fetch = [&](uint32_t f, int k, int ii, T (&av)[PPT]) {
-                if (GX && pf_xg(f, k)) {  // (without GX every variable has a row)
+            // operands of instruction ii: tape rows or scalars, both rows read
+            // before either is used (a unary instruction reads row 0 and
+            // discards it); with GX, variables read from X
+            auto fetch = [&](uint32_t f, int ii, T (&a0)[PPT], T (&a1)[PPT]) {
+                const bool bin = pf_bin(f);
+                if (GX && (pf_xg(f, 0) || pf_xg(f, 1))) {
                     const uint32_t w = c.word_lt<LT>(ii);
-                    const int v = desc_idx(k == 0 ? instr_d0(w) : instr_d1(w));
 #pragma unroll
-                    for (int q = 0; q < PPT; ++q) av[q] = act[q] ? X[n[q] * (int64_t)n_x + v] : T(0);
+                    for (int k = 0; k < 2; ++k) {
+                        T *av = k == 0 ? a0 : a1;
+                        const T sv = k == 0 ? (HW == 1 ? rl(sv0[0], ii) : (ii < WAVE ? rl(sv0[0], ii) : rl(sv0[HW - 1], ii - WAVE)))
+                                            : (HW == 1 ? rl(sv1[0], ii) : (ii < WAVE ? rl(sv1[0], ii) : rl(sv1[HW - 1], ii - WAVE)));
+                        const int v = desc_idx(k == 0 ? instr_d0(w) : instr_d1(w));
+                        const bool xg = pf_xg(f, k), sc = pf_sc(f, k);
+                        const int row = pf_row(f, k);
+#pragma unroll
+                        for (int q = 0; q < PPT; ++q) {
+                            const T tv = xg ? (act[q] ? X[n[q] * (int64_t)n_x + v] : T(0)) : t.get(row, q);
+                            av[q] = (k == 1 && !bin) ? T(0) : (sc ? sv : tv);
+                        }
+                    }
                 } else {
-                    const int row = pf_row(f, k);
-                    const bool sc = pf_sc(f, k);
-                    const T sv = k == 0 ? (HW == 1 ? rl(sv0[0], ii) : (ii < WAVE ? rl(sv0[0], ii) : rl(sv0[HW - 1], ii - WAVE)))
-                                        : (HW == 1 ? rl(sv1[0], ii) : (ii < WAVE ? rl(sv1[0], ii) : rl(sv1[HW - 1], ii - WAVE)));
+                    const int r0 = pf_row(f, 0), r1 = pf_row(f, 1);
+                    const bool c0 = pf_sc(f, 0), c1 = pf_sc(f, 1);
+                    const T s0 = HW == 1 ? rl(sv0[0], ii) : (ii < WAVE ? rl(sv0[0], ii) : rl(sv0[HW - 1], ii - WAVE));
+                    const T s1 = HW == 1 ? rl(sv1[0], ii) : (ii < WAVE ? rl(sv1[0], ii) : rl(sv1[HW - 1], ii - WAVE));
+                    T t0[PPT], t1[PPT];
+#pragma unroll
+                    for (int q = 0; q < PPT; ++q) t0[q] = t.get(r0, q);
+#pragma unroll
+                    for (int q = 0; q < PPT; ++q) t1[q] = t.get(r1, q);
 #pragma unroll
                     for (int q = 0; q < PPT; ++q) {
-                        const T tv = t.get(row, q);
-                        av[q] = sc ? sv : tv;
+                        a0[q] = c0 ? s0 : t0[q];
+                        a1[q] = bin ? (c1 ? s1 : t1[q]) : T(0);
                     }
                 }
             };
@@ -232,13 +250,7 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
                 const uint32_t f = pl.fw<LT>(ii);
                 const int op = pf_op(f);
                 T a0[PPT], a1[PPT], r[PPT];
-                fetch(f, 0, ii, a0);
-                if (pf_bin(f)) {
-                    fetch(f, 1, ii, a1);
-                } else {
-#pragma unroll
-                    for (int q = 0; q < PPT; ++q) a1[q] = T(0);
-                }
+                fetch(f, ii, a0, a1);
                 if (op == OP_COPY) {
 #pragma unroll
                     for (int q = 0; q < PPT; ++q) r[q] = a0[q];
@@ -277,13 +289,7 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
                 const int grow = ps_grow(sw);
 #pragma unroll
                 for (int q = 0; q < PPT; ++q) g[q] = t.get(grow, q);
-                fetch(f, 0, ii, a0);
-                if (binary) {
-                    fetch(f, 1, ii, a1);
-                } else {
-#pragma unroll
-                    for (int q = 0; q < PPT; ++q) a1[q] = T(0);
-                }
+                fetch(f, ii, a0, a1);
                 if (op == OP_COPY) {
 #pragma unroll
                     for (int q = 0; q < PPT; ++q) {
