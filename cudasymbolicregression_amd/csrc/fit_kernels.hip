@@ -213,7 +213,7 @@ __device__ inline void closure_code(const Code &c, const Plan &pl, int nc, const
 #pragma unroll
                 for (int p = 0; p < PPT; ++p) r[p] = a0[p];
             } else {
-                fwd_nodes<float, PPT>(op, a0, a1, r);
+                fwd_ops<float, PPT>(op, a0, a1, r);
             }
 #pragma unroll
             for (int p = 0; p < PPT; ++p) t.put(i, p, r[p]);
@@ -248,7 +248,7 @@ __device__ inline void closure_code(const Code &c, const Plan &pl, int nc, const
                     ee[p] = 0;
                 }
             } else {
-                bwd_nodes<float, PPT>(op, g, a0, a1, go0, add0, go1, add1, ee);
+                bwd_ops<float, PPT>(op, g, a0, a1, go0, add0, go1, add1, ee);
             }
 #pragma unroll
             for (int p = 0; p < PPT; ++p)

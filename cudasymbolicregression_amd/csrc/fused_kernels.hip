@@ -255,7 +255,7 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
 #pragma unroll
                     for (int q = 0; q < PPT; ++q) r[q] = a0[q];
                 } else {
-                    fwd_nodes<T, PPT>(op, a0, a1, r);
+                    fwd_ops<T, PPT>(op, a0, a1, r);
                 }
 #pragma unroll
                 for (int q = 0; q < PPT; ++q) t.put(ii, q, r[q]);
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
                         ee[q] = 0;
                     }
                 } else {
-                    bwd_nodes<T, PPT>(op, g, a0, a1, go0, add0, go1, add1, ee);
+                    bwd_ops<T, PPT>(op, g, a0, a1, go0, add0, go1, add1, ee);
                 }
 #pragma unroll
                 for (int q = 0; q < PPT; ++q)

@@ -241,6 +241,39 @@ __device__ inline void bwd_nodes(int op, const T *g, const T *a0, const T *a1, T
 #undef SR_EACH
 }
 
+// ADD, SUB and MUL (the most frequent operators of typical grammars) are
+// evaluated branch-free, ahead of the dispatch tree: one range test instead
+// of a chain of opcode comparisons.  Same values as fwd_nodes / bwd_nodes.
+template <typename T, int P>
+__device__ inline void fwd_ops(int op, const T *a0, const T *a1, T *r) {
+    if (op >= ADD && op <= MUL) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const T s = a0[p] + a1[p], d = a0[p] - a1[p], m = a0[p] * a1[p];
+            r[p] = op == MUL ? m : (op == SUB ? d : s);
+        }
+    } else {
+        fwd_nodes<T, P>(op, a0, a1, r);
+    }
+}
+template <typename T, int P>
+__device__ inline void bwd_ops(int op, const T *g, const T *a0, const T *a1, T *go0, bool *add0, T *go1, bool *add1,
+                               int32_t *e) {
+    if (op >= ADD && op <= MUL) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const bool mul = op == MUL;
+            const T m0 = g[p] * a1[p], m1 = g[p] * a0[p];
+            go0[p] = mul ? m0 : g[p];
+            go1[p] = mul ? m1 : (op == SUB ? -g[p] : g[p]);
+            add0[p] = add1[p] = !(g[p] == T(0));
+            e[p] = 0;
+        }
+    } else {
+        bwd_nodes<T, P>(op, g, a0, a1, go0, add0, go1, add1, e);
+    }
+}
+
 template <typename T>
 __device__ inline int32_t bwd_node(int op, T g, T a0, T a1, T &go0, bool &add0, T &go1, bool &add1) {
     int32_t e;
