@@ -266,19 +266,16 @@ __device__ inline void closure_code(const Code &c, const Plan &pl, int nc, const
                         const float o = t.get(tg, p);
                         t.put(tg, p, (act[p] && add[p]) ? o + go[p] : o);
                     }
-                } else if (mode == PM_CONST) {
+                } else if (mode == PM_CROW) {  // a constant used once: summed after the tile group
+#pragma unroll
+                    for (int p = 0; p < PPT; ++p) t.put(tg, p, (act[p] && add[p]) ? go[p] : 0.0f);
+                } else if (mode == PM_CONST) {  // a constant used several times: wave sum per use
                     double gv = 0.0;
 #pragma unroll
                     for (int p = 0; p < PPT; ++p) gv += (act[p] && add[p]) ? (double)go[p] : 0.0;
-                    if (tg < FIT_NCREG) {
-#pragma unroll
-                        for (int q = 0; q < FIT_NCREG; ++q)
-                            if (q == tg) cacc[q] += gv;
-                    } else {
-                        const double s = wave_sum_rows(gv, 4);
-                        if (tg < WAVE) { if (lane == tg) gc[0] += s; }
-                        else if (lane == tg - WAVE) gc[SL - 1] += s;
-                    }
+                    const double s = wave_sum_rows(gv, 4);
+                    if (tg < WAVE) { if (lane == tg) gc[0] += s; }
+                    else if (lane == tg - WAVE) gc[SL - 1] += s;
                 } else if (mode == PM_NOOP) {
 #pragma unroll
                     for (int p = 0; p < PPT; ++p)
@@ -288,6 +285,29 @@ __device__ inline void closure_code(const Code &c, const Plan &pl, int nc, const
             scatter(0, go0, add0);
             if (binary) scatter(1, go1, add1);
         }
+        // ---- adjoints of the constants used once, from their rows, in FP64
+#pragma unroll
+        for (int q = 0; q < FIT_NCREG; ++q) {
+            if (q < nc) {
+                const int row = pl.crow(q);
+                if (row != (int)NO_CROW) {
+                    double gv = 0.0;
+#pragma unroll
+                    for (int p = 0; p < PPT; ++p) gv += (double)t.get(row, p);
+                    cacc[q] += gv;
+                }
+            }
+        }
+        for (int j = FIT_NCREG; j < nc; ++j) {
+            const int row = pl.crow(j);
+            if (row == (int)NO_CROW) continue;
+            double gv = 0.0;
+#pragma unroll
+            for (int p = 0; p < PPT; ++p) gv += (double)t.get(row, p);
+            const double s = wave_sum_rows(gv, 4);
+            if (j < WAVE) { if (lane == j) gc[0] += s; }
+            else if (lane == j - WAVE) gc[SL - 1] += s;
+        }
     }
     loss = wave_sum_rows(lacc, 4);
     const int ncr = nc < FIT_NCREG ? nc : FIT_NCREG;
@@ -295,7 +315,7 @@ __device__ inline void closure_code(const Code &c, const Plan &pl, int nc, const
     for (int q = 0; q < FIT_NCREG; ++q) {
         if (q < ncr) {
             const double s = wave_sum_rows(cacc[q], 4);
-            if (lane == q) gc[0] = s;
+            if (lane == q) gc[0] += s;
         }
     }
     err = wave_max_err(e);

@@ -317,19 +317,16 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
                             const T o = t.get(tg, q);
                             t.put(tg, q, (act[q] && add[q]) ? o + go[q] : o);
                         }
-                    } else if (mode == PM_CONST) {
+                    } else if (mode == PM_CROW) {  // a constant used once: summed after the tile group
+#pragma unroll
+                        for (int q = 0; q < PPT; ++q) t.put(tg, q, (act[q] && add[q]) ? go[q] : T(0));
+                    } else if (mode == PM_CONST) {  // a constant used several times: wave sum per use
                         double gv = 0.0;
 #pragma unroll
                         for (int q = 0; q < PPT; ++q) gv += (act[q] && add[q]) ? (double)go[q] : 0.0;
-                        if (tg < F_NCREG) {
-#pragma unroll
-                            for (int r = 0; r < F_NCREG; ++r)
-                                if (r == tg) cacc[r] += gv;
-                        } else {
-                            const double s = wave_sum_rows(gv, 4);
-                            if (tg < WAVE) { if (lane == tg) gc[0] += s; }
-                            else if (lane == tg - WAVE) gc[SL - 1] += s;
-                        }
+                        const double s = wave_sum_rows(gv, 4);
+                        if (tg < WAVE) { if (lane == tg) gc[0] += s; }
+                        else if (lane == tg - WAVE) gc[SL - 1] += s;
                     } else if (mode == PM_VAR) {
                         if (want_gx) {
                             if constexpr (!GX) {
@@ -358,6 +355,29 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
                 };
                 scatter(0, go0, add0);
                 if (binary) scatter(1, go1, add1);
+            }
+            // ---- adjoints of the constants used once, from their rows, in FP64
+#pragma unroll
+            for (int r = 0; r < F_NCREG; ++r) {
+                if (r < p.nc) {
+                    const int row = pl.crow(r);
+                    if (row != (int)NO_CROW) {
+                        double gv = 0.0;
+#pragma unroll
+                        for (int q = 0; q < PPT; ++q) gv += (double)t.get(row, q);
+                        cacc[r] += gv;
+                    }
+                }
+            }
+            for (int j = F_NCREG; j < p.nc; ++j) {
+                const int row = pl.crow(j);
+                if (row == (int)NO_CROW) continue;
+                double gv = 0.0;
+#pragma unroll
+                for (int q = 0; q < PPT; ++q) gv += (double)t.get(row, q);
+                const double s = wave_sum_rows(gv, 4);
+                if (j < WAVE) { if (lane == j) gc[0] += s; }
+                else if (lane == j - WAVE) gc[SL - 1] += s;
             }
             // ---- variable adjoints of this tile group
             if (!GX && want_gx) {
@@ -388,7 +408,7 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
         for (int r = 0; r < F_NCREG; ++r) {
             if (r < ncr) {
                 const double s = wave_sum_rows(cacc[r], 4);
-                if (lane == r) gc[0] = s;
+                if (lane == r) gc[0] += s;
             }
         }
         // dL/dC back to node slots: constant node k takes its compact lane

@@ -220,13 +220,21 @@ __host__ __device__ inline size_t tape_elems(int LT, int PPT) { return (size_t)(
 //     instruction with several parents, or with none below the root, keeps
 //     its value row and gets an extra adjoint row (NI, NI + 1, ...) that is
 //     zeroed before each tile group and accumulated into, in the same order
-//     as before.  The bucket of a program is chosen by its row count
-//     NI + extra rows (count_rows_wave), at most 2 NI - 1.
+//     as before;
+//   * a constant used once has its adjoint written into a row (the row of
+//     the instruction that uses it, dead by then, or an extra row for the
+//     second constant operand of one instruction; crow0 / crow1 name it per
+//     constant), and the evaluators add those rows into their FP64
+//     per-constant sums after each tile group.  A constant used more than
+//     once is reduced over the wave at every use (PM_CONST).
+// The bucket of a program is chosen by its row count NI + extra rows
+// (count_rows_wave), at most 2 NI.
 // Fetch word: op 0-5 | binary 6 | scalar0 7 | row0 8-16 | scalar1 17 |
 // row1 18-26 | xg0 27 | xg1 28 (xg: variable read from X).
 // Scatter word: adjoint row of the instruction 0-7 | mode0 8-10 |
 // target0 11-18 | mode1 19-21 | target1 22-29.
-constexpr uint32_t PM_NONE = 0, PM_WRITE = 1, PM_ACC = 2, PM_CONST = 3, PM_VAR = 4, PM_NOOP = 5;
+constexpr uint32_t PM_NONE = 0, PM_WRITE = 1, PM_ACC = 2, PM_CONST = 3, PM_VAR = 4, PM_NOOP = 5, PM_CROW = 6;
+constexpr uint32_t NO_CROW = 255;
 
 __host__ __device__ inline int pf_op(uint32_t f) { return (int)(f & 63u); }
 __host__ __device__ inline bool pf_bin(uint32_t f) { return (f >> 6) & 1u; }
@@ -238,9 +246,10 @@ __host__ __device__ inline uint32_t ps_mode(uint32_t s, int k) { return (s >> (k
 __host__ __device__ inline int ps_tgt(uint32_t s, int k) { return (int)((s >> (k ? 22 : 11)) & 255u); }
 
 struct Plan {
-    uint32_t f0, f1;  // fetch words
-    uint32_t s0, s1;  // scatter words
-    int nextra;       // extra adjoint rows (NI .. NI + nextra - 1)
+    uint32_t f0, f1;      // fetch words
+    uint32_t s0, s1;      // scatter words
+    uint32_t crow0, crow1;  // lane j: adjoint row of constant j (and 64 + j), or NO_CROW
+    int nextra;           // extra rows (NI .. NI + nextra - 1)
     template <int LT>
     __device__ inline uint32_t fw(int i) const {
         if constexpr (LT <= WAVE) return rl_u32(f0, i);
@@ -251,15 +260,17 @@ struct Plan {
         if constexpr (LT <= WAVE) return rl_u32(s0, i);
         else return i < WAVE ? rl_u32(s0, i) : rl_u32(s1, i - WAVE);
     }
+    __device__ inline int crow(int j) const { return (int)(j < WAVE ? rl_u32(crow0, j) : rl_u32(crow1, j - WAVE)); }
 };
 
-// References of every node (lane k, 64 + k) by the operator nodes om0 / om1
-// of the node words w0 / w1, and the resulting extra-row masks (operator
-// nodes below the root with a reference count other than one).
-__device__ inline void extra_rows(uint32_t w0, uint32_t w1, uint64_t om0, uint64_t om1, int root, uint64_t &em0,
-                                  uint64_t &em1) {
+// Uses of every node (lane k, 64 + k) by the operator nodes om0 / om1 of the
+// node words w0 / w1, plus the copy instruction of a leaf root; the extra
+// instruction rows (operator nodes below the root used other than once), and
+// the constants used exactly once (cs0 / cs1).
+__device__ inline void node_uses(uint32_t w0, uint32_t w1, uint64_t om0, uint64_t om1, int root, int M, uint64_t &em0,
+                                 uint64_t &em1, uint64_t &cs0, uint64_t &cs1) {
     const int lane = lane_id();
-    int rc0 = 0, rc1 = 0;
+    int rc0 = lane == root ? 1 : 0, rc1 = lane + WAVE == root ? 1 : 0;  // the copy of a leaf root
     auto count = [&](uint32_t w) {
         const int c0 = node_c0(w), c1 = arity_of(node_op(w)) == 2 ? node_c1(w) : -1;
         rc0 += (c0 == lane ? 1 : 0) + (c1 == lane ? 1 : 0);
@@ -267,8 +278,25 @@ __device__ inline void extra_rows(uint32_t w0, uint32_t w1, uint64_t om0, uint64
     };
     for (uint64_t m = om0; m; m &= m - 1) count(rl_u32(w0, __ffsll((long long)m) - 1));
     for (uint64_t m = om1; m; m &= m - 1) count(rl_u32(w1, __ffsll((long long)m) - 1));
-    em0 = __ballot(((om0 >> lane) & 1ull) && lane < root && rc0 != 1);
-    em1 = __ballot(((om1 >> lane) & 1ull) && lane + WAVE < root && rc1 != 1);
+    const bool op0 = (om0 >> lane) & 1ull, op1 = (om1 >> lane) & 1ull;
+    // an operator root is not a copy's operand: its count is its parents'
+    em0 = __ballot(op0 && lane < root && rc0 != 1);
+    em1 = __ballot(op1 && lane + WAVE < root && rc1 != 1);
+    cs0 = __ballot(lane < M && node_op(w0) == LEARNABLE_CONSTANT && rc0 == 1);
+    cs1 = __ballot(lane + WAVE < M && node_op(w1) == LEARNABLE_CONSTANT && rc1 == 1);
+}
+
+// Binary operator nodes whose two operands are constants used once: each
+// needs one extra row for its second constant's adjoint.
+__device__ inline void second_const_rows(uint32_t w0, uint32_t w1, uint64_t om0, uint64_t om1, uint64_t cs0,
+                                         uint64_t cs1, uint64_t &dm0, uint64_t &dm1) {
+    const int lane = lane_id();
+    auto is_cs = [&](int c) -> bool { return c < WAVE ? ((cs0 >> c) & 1ull) : (c < 2 * WAVE && ((cs1 >> (c - WAVE)) & 1ull)); };
+    auto both = [&](uint32_t w) -> bool {
+        return arity_of(node_op(w)) == 2 && is_cs(node_c0(w)) && is_cs(node_c1(w));
+    };
+    dm0 = __ballot(((om0 >> lane) & 1ull) && both(w0));
+    dm1 = __ballot(((om1 >> lane) & 1ull) && both(w1));
 }
 
 // Tape rows a batch-first program row needs (NI + extra rows), under the
@@ -296,15 +324,16 @@ __device__ inline int count_rows_wave(const int64_t *ops_row, const int64_t *ch_
     const uint64_t om1 = __ballot(lane + WAVE < M && lane + WAVE <= root && arity_of(node_op(w[1])) > 0);
     const bool root_is_op = root < WAVE ? ((om0 >> root) & 1ull) : ((om1 >> (root - WAVE)) & 1ull);
     const int ni = __popcll(om0) + __popcll(om1) + (root_is_op ? 0 : 1);
-    uint64_t em0, em1;
-    extra_rows(w[0], w[1], om0, om1, root, em0, em1);
-    return ni + __popcll(em0) + __popcll(em1);
+    uint64_t em0, em1, cs0, cs1, dm0, dm1;
+    node_uses(w[0], w[1], om0, om1, root, M, em0, em1, cs0, cs1);
+    second_const_rows(w[0], w[1], om0, om1, cs0, cs1, dm0, dm1);
+    return ni + __popcll(em0) + __popcll(em1) + __popcll(dm0) + __popcll(dm1);
 }
 
 // Operand record of a node (used by its parent): scalar 0 | row 1-9 |
-// xg 10 | scatter mode 11-13 | scatter target 14-21.
-__device__ inline uint32_t operand_rec(uint32_t w, bool is_op, uint32_t ix, bool extra, uint32_t erow, int var_row0,
-                                       int nvr) {
+// xg 10 | scatter mode 11-13 | scatter target 14-21 | constant used once 22.
+__device__ inline uint32_t operand_rec(uint32_t w, bool is_op, uint32_t ix, bool extra, uint32_t erow, bool once,
+                                       int var_row0, int nvr) {
     if (is_op) return (ix << 1) | ((extra ? PM_ACC : PM_WRITE) << 11) | ((extra ? erow : ix) << 14);
     const uint32_t d = operand_desc(w, 0u, false);
     const int kind = desc_kind(d);
@@ -313,7 +342,7 @@ __device__ inline uint32_t operand_rec(uint32_t w, bool is_op, uint32_t ix, bool
         const uint32_t t = (idx > 255u ? 255u : idx) << 14;
         return (int)idx < nvr ? (((uint32_t)var_row0 + idx) << 1) | (PM_VAR << 11) | t : (1u << 10) | (PM_VAR << 11) | t;
     }
-    if (kind == K_CONST) return 1u | (PM_CONST << 11) | (idx << 14);
+    if (kind == K_CONST) return 1u | (PM_CONST << 11) | (idx << 14) | ((once ? 1u : 0u) << 22);
     return 1u | ((idx == (uint32_t)IMM_NOOP ? PM_NOOP : PM_NONE) << 11);
 }
 
@@ -331,13 +360,18 @@ __device__ inline Plan compile_plan(const CProg<T> &p, int M, const Code &c, int
     const uint64_t below = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
     const uint32_t ix0 = (uint32_t)__popcll(om0 & below);
     const uint32_t ix1 = (uint32_t)(__popcll(om0) + __popcll(om1 & below));
-    uint64_t em0, em1;
-    extra_rows(w0, w1, om0, om1, root, em0, em1);
+    uint64_t em0, em1, cs0, cs1, dm0, dm1;
+    node_uses(w0, w1, om0, om1, root, M, em0, em1, cs0, cs1);
+    second_const_rows(w0, w1, om0, om1, cs0, cs1, dm0, dm1);
     const bool ex0 = (em0 >> lane) & 1ull, ex1 = (em1 >> lane) & 1ull;
+    const int ne = __popcll(em0) + __popcll(em1);
     const uint32_t er0 = (uint32_t)(NI + __popcll(em0 & below));
     const uint32_t er1 = (uint32_t)(NI + __popcll(em0) + __popcll(em1 & below));
-    const uint32_t r0 = operand_rec(w0, op0, ix0, ex0, er0, var_row0, nvr);
-    const uint32_t r1 = operand_rec(w1, op1, ix1, ex1, er1, var_row0, nvr);
+    // second-constant rows follow the extra instruction rows
+    const uint32_t dr0 = (uint32_t)(NI + ne + __popcll(dm0 & below));
+    const uint32_t dr1 = (uint32_t)(NI + ne + __popcll(dm0) + __popcll(dm1 & below));
+    const uint32_t r0 = operand_rec(w0, op0, ix0, ex0, er0, (cs0 >> lane) & 1ull, var_row0, nvr);
+    const uint32_t r1 = operand_rec(w1, op1, ix1, ex1, er1, (cs1 >> lane) & 1ull, var_row0, nvr);
     auto child_rec = [&](int ch) -> uint32_t {  // every lane takes part in both shuffles
         const int cc = ch & 127;
         const uint32_t lo = (uint32_t)__shfl((int)r0, cc & (WAVE - 1), WAVE);
@@ -349,21 +383,31 @@ __device__ inline Plan compile_plan(const CProg<T> &p, int M, const Code &c, int
         return op | ((bin ? 1u : 0u) << 6) | ((a & 1u) << 7) | (((a >> 1) & 511u) << 8) | ((b & 1u) << 17) |
                (((b >> 1) & 511u) << 18) | (((a >> 10) & 1u) << 27) | (((b >> 10) & 1u) << 28);
     };
-    auto scatter_word = [](uint32_t grow, bool bin, uint32_t a, uint32_t b) -> uint32_t {
-        b = bin ? b : 0u;
-        return grow | (((a >> 11) & 7u) << 8) | (((a >> 14) & 255u) << 11) | (((b >> 11) & 7u) << 19) |
-               (((b >> 14) & 255u) << 22);
+    // a constant used once writes its adjoint to crow_a (operand 0) or crow_b
+    auto scatter_mode = [](uint32_t a, uint32_t crow) -> uint32_t {
+        const bool once = (a >> 22) & 1u;
+        return once ? (PM_CROW | (crow << 3)) : (((a >> 11) & 7u) | (((a >> 14) & 255u) << 3));
+    };
+    auto scatter_word = [&](uint32_t grow, bool bin, uint32_t a, uint32_t b, uint32_t crow_a,
+                            uint32_t crow_b) -> uint32_t {
+        const uint32_t ma = scatter_mode(a, crow_a), mb = bin ? scatter_mode(b, crow_b) : 0u;
+        return grow | ((ma & 7u) << 8) | ((ma >> 3) << 11) | ((mb & 7u) << 19) | ((mb >> 3) << 22);
     };
     const uint32_t a00 = child_rec(node_c0(w0)), a01 = child_rec(node_c1(w0));
     const uint32_t a10 = child_rec(node_c0(w1)), a11 = child_rec(node_c1(w1));
     const bool b0 = arity_of(node_op(w0)) == 2, b1 = arity_of(node_op(w1)) == 2;
+    const bool d0 = (dm0 >> lane) & 1ull, d1 = (dm1 >> lane) & 1ull;
+    // rows of the constant operands: the instruction's own row, and for the
+    // second of two constants its second-constant row
+    const uint32_t ca0 = ix0, cb0 = d0 ? dr0 : ix0, ca1 = ix1, cb1 = d1 ? dr1 : ix1;
     const uint32_t F0 = fetch_word((uint32_t)node_op(w0) & 63u, b0, a00, a01);
     const uint32_t F1 = fetch_word((uint32_t)node_op(w1) & 63u, b1, a10, a11);
-    const uint32_t S0 = scatter_word(ex0 ? er0 : ix0, b0, a00, a01);
-    const uint32_t S1 = scatter_word(ex1 ? er1 : ix1, b1, a10, a11);
+    const uint32_t S0 = scatter_word(ex0 ? er0 : ix0, b0, a00, a01, ca0, cb0);
+    const uint32_t S1 = scatter_word(ex1 ? er1 : ix1, b1, a10, a11, ca1, cb1);
     Plan pl;
     pl.f0 = pl.f1 = pl.s0 = pl.s1 = 0u;
-    pl.nextra = __popcll(em0) + __popcll(em1);
+    pl.crow0 = pl.crow1 = NO_CROW;
+    pl.nextra = ne + __popcll(dm0) + __popcll(dm1);
     auto place = [&](int i, uint32_t f, uint32_t s) {
         if (i < WAVE) {
             if (lane == i) { pl.f0 = f; pl.s0 = s; }
@@ -372,18 +416,31 @@ __device__ inline Plan compile_plan(const CProg<T> &p, int M, const Code &c, int
             pl.s1 = s;
         }
     };
+    // the constant ordinal of an operand record, and its row, into lane o
+    auto place_crow = [&](uint32_t rec, uint32_t row) {
+        if ((rec >> 22) & 1u) {
+            const int o = (int)((rec >> 14) & 255u);
+            if (o < WAVE) { if (lane == o) pl.crow0 = row; }
+            else if (lane == o - WAVE) pl.crow1 = row;
+        }
+    };
     int i = 0;
     for (uint64_t m = om0; m; m &= m - 1, ++i) {
         const int k = __ffsll((long long)m) - 1;
         place(i, rl_u32(F0, k), rl_u32(S0, k));
+        place_crow(rl_u32(a00, k), rl_u32(ca0, k));
+        if (rl_u32((uint32_t)b0, k)) place_crow(rl_u32(a01, k), rl_u32(cb0, k));
     }
     for (uint64_t m = om1; m; m &= m - 1, ++i) {
         const int k = __ffsll((long long)m) - 1;
         place(i, rl_u32(F1, k), rl_u32(S1, k));
+        place_crow(rl_u32(a10, k), rl_u32(ca1, k));
+        if (rl_u32((uint32_t)b1, k)) place_crow(rl_u32(a11, k), rl_u32(cb1, k));
     }
     if (i < NI) {  // the root is a leaf: one copy instruction
         const uint32_t rr = root < WAVE ? rl_u32(r0, root) : rl_u32(r1, root - WAVE);
-        place(i, fetch_word((uint32_t)OP_COPY, false, rr, 0u), scatter_word((uint32_t)i, false, rr, 0u));
+        place(i, fetch_word((uint32_t)OP_COPY, false, rr, 0u), scatter_word((uint32_t)i, false, rr, 0u, (uint32_t)i, 0u));
+        place_crow(rr, (uint32_t)i);
     }
     return pl;
 }
