@@ -1,10 +1,15 @@
-"""Programs whose nodes are not a tree: an operator node used by two parents
+"""Evaluation-plan paths beyond trees of at most four variables.
+
+Programs whose nodes are not a tree: an operator node used by two parents
 (or twice by one), and operator nodes below the root that no parent uses.
 The evaluators keep adjoints in the value rows of one-parent instructions and
 give every other instruction its own accumulated adjoint row; these tests pin
 that second path against the oracle, which follows the reference's dense
 reverse sweep (src/evaluation_kernels_cpu.cpp:222-662) for any child lists
 that pass validation (children below their parent, arity-consistent).
+
+Programs with more than four variables (the variables beyond the four tape
+rows are read from X at every use).
 
 Fused evaluator: FP64 within 1e-10 relative, FP32 within the FP32 tolerance
 of the other evaluation tests, with identical NaN masks and error codes.
@@ -122,5 +127,59 @@ def test_fit_shared_and_unused_nodes_match_oracle():
     close = np.abs(mse_g - mse_o) <= 1e-3 * mse_o + 1e-7
     frac = close[both].mean()
     print(f"shared-node fit: {both.sum()} finite problems, {frac:.3f} within tolerance")
+    assert both.sum() > B // 2
+    assert frac >= 0.9
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_fused_many_variables_match_oracle(mode):
+    """n_x = 7: the variables beyond the 4 tape rows are read from X (and
+    their adjoints go straight to grad_X).  Operators without exp / division
+    keep the programs well conditioned, so FP64 agrees to 1e-10 (with exp of
+    exp, one-ulp differences of the device exp reach the sums)."""
+    from cudasymbolicregression_amd import fused
+
+    print(f"case fused n_x=7 mode={mode}", flush=True)
+    n_x, B, M, N = 7, 200, 24, 150
+    rng = np.random.default_rng(21 + mode)
+    ops, ch = helpers.random_batch(21 + mode, B, M, n_x, unary=[10, 11, 14], binary=[40, 41, 42], depth=5)
+    C = helpers.constants_like(ops, rng)
+    X = rng.uniform(-2, 2, (N, n_x)).astype(np.float64)
+    dy = rng.standard_normal((B, N))
+    ref = oracle.fused_root(X, ops, ch, C.astype(np.float64), dy, mode=mode)
+    out = {k: v.cpu().numpy() for k, v in fused.fused_root(_t(X), _t(ops), _t(ch), _t(C.astype(np.float64)), _t(dy),
+                                                           mode=mode, want_grad_X=True, raise_errors=False).items()}
+    assert (out["err"] == ref["err"]).all()
+    ok = ref["err"] == 0
+    for key in ("y", "grad_C") + (("loss",) if mode == 1 else ()):
+        good, msg = helpers.allclose_nan(out[key][ok], ref[key][ok], 1e-10, 1e-12)
+        assert good, f"{key}: {msg}"
+    if ok.all():
+        good, msg = helpers.allclose_nan(out["grad_X"], ref["grad_X"], 1e-10, 1e-10)
+        assert good, f"grad_X: {msg}"
+
+
+def test_fit_many_variables_match_oracle():
+    """n_x = 6 in the fit: variables 4 and 5 are read from X at every use."""
+    import cudasymbolicregression_amd as st
+
+    print("case fit n_x=6", flush=True)
+    n_x, B, M, N = 6, 96, 16, 64
+    rng = np.random.default_rng(31)
+    ops, ch = helpers.random_batch(31, B, M, n_x, unary=[10, 11, 12, 14], binary=[40, 41, 42], leaf_const_p=0.4,
+                                   depth=5)
+    X = rng.uniform(-2, 2, (N, n_x)).astype(np.float32)
+    C_true = helpers.constants_like(ops, rng, scale=1.0)
+    Y = oracle.fused_root(X, ops, ch, C_true, np.zeros((B, N), np.float32), mode=0)["y"]
+    C0 = helpers.constants_like(ops, rng, scale=1.0)
+    C_o, y_o, f_o, _ = lbfgs_oracle.optimize(X, Y, ops, ch, C0, max_iter=30)
+    Cg, Yg, info = st.optimize(_t(X), _t(Y), _t(ops), _t(ch), _t(C0), max_iter=30, return_info=True,
+                               raise_errors=False)
+    mse_g = ((Yg.cpu().numpy().astype(np.float64) - Y) ** 2).mean(1)
+    mse_o = ((y_o.astype(np.float64) - Y) ** 2).mean(1)
+    both = np.isfinite(Y).all(1) & np.isfinite(mse_g) & np.isfinite(mse_o) & (info["err"].cpu().numpy() == 0)
+    close = np.abs(mse_g - mse_o) <= 1e-3 * mse_o + 1e-7
+    frac = close[both].mean()
+    print(f"n_x=6 fit: {both.sum()} finite problems, {frac:.3f} within tolerance")
     assert both.sum() > B // 2
     assert frac >= 0.9
