@@ -1,6 +1,6 @@
 """Autotuner for the fit kernels' launch variants.
 
-The device L-BFGS (`optimize`) buckets problems by instruction count and runs
+The device L-BFGS (`optimize`) buckets problems by tape rows and runs
 one kernel per bucket; each bucket has a few launch variants that trade
 per-problem latency against throughput: W waves cooperating on one problem
 (points split between them) and PPT points per lane per instruction dispatch.

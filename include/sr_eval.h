@@ -142,7 +142,7 @@ int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const i
                  int line_search, int ls_max_iter, double eps, void *workspace, int64_t ws_bytes, void *stream);
 
 /* Launch variants of the fit kernels (no counterpart in the reference; used
- * by the autotuner).  Problems are bucketed by instruction count; each
+ * by the autotuner).  Problems are bucketed by tape rows; each
  * bucket has variants (waves per problem x points per lane per dispatch)
  * of which one is active per process.  Host-only calls; return -1 on an
  * out-of-range bucket or variant. */
