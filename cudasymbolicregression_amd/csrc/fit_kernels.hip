@@ -257,18 +257,18 @@ __device__ inline void closure_code(const Code &c, const Plan &pl, int nc, const
             auto scatter = [&](int k, const float (&go)[PPT], const bool (&add)[PPT]) {
                 const uint32_t mode = ps_mode(sw, k);
                 const int tg = ps_tgt(sw, k);
-                if (mode == PM_WRITE) {  // the only adjoint of a one-parent row: replaces its value
+                if (mode - 1u < 3u) {
+                    // PM_WRITE: the only adjoint of a one-parent row, replacing
+                    // its value; PM_CROW: a constant used once, summed after the
+                    // tile group; PM_ACC: an extra adjoint row, accumulated
+                    float o[PPT];
 #pragma unroll
-                    for (int p = 0; p < PPT; ++p) t.put(tg, p, (act[p] && add[p]) ? 0.0f + go[p] : 0.0f);
-                } else if (mode == PM_ACC) {
+                    for (int p = 0; p < PPT; ++p) o[p] = 0.0f;
+                    if (mode == PM_ACC)
 #pragma unroll
-                    for (int p = 0; p < PPT; ++p) {
-                        const float o = t.get(tg, p);
-                        t.put(tg, p, (act[p] && add[p]) ? o + go[p] : o);
-                    }
-                } else if (mode == PM_CROW) {  // a constant used once: summed after the tile group
+                        for (int p = 0; p < PPT; ++p) o[p] = t.get(tg, p);
 #pragma unroll
-                    for (int p = 0; p < PPT; ++p) t.put(tg, p, (act[p] && add[p]) ? go[p] : 0.0f);
+                    for (int p = 0; p < PPT; ++p) t.put(tg, p, o[p] + ((act[p] && add[p]) ? go[p] : 0.0f));
                 } else if (mode == PM_CONST) {  // a constant used several times: wave sum per use
                     double gv = 0.0;
 #pragma unroll

@@ -308,18 +308,19 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
                 auto scatter = [&](int k, const T (&go)[PPT], const bool (&add)[PPT]) {
                     const uint32_t mode = ps_mode(sw, k);
                     const int tg = ps_tgt(sw, k);
-                    if (mode == PM_WRITE) {  // the only adjoint of a one-parent row: replaces its value
+                    if (mode - 1u < 3u) {
+                        // PM_WRITE: the only adjoint of a one-parent row,
+                        // replacing its value; PM_CROW: a constant used once,
+                        // summed after the tile group; PM_ACC: an extra adjoint
+                        // row, accumulated
+                        T o[PPT];
 #pragma unroll
-                        for (int q = 0; q < PPT; ++q) t.put(tg, q, (act[q] && add[q]) ? T(0) + go[q] : T(0));
-                    } else if (mode == PM_ACC) {
+                        for (int q = 0; q < PPT; ++q) o[q] = T(0);
+                        if (mode == PM_ACC)
 #pragma unroll
-                        for (int q = 0; q < PPT; ++q) {
-                            const T o = t.get(tg, q);
-                            t.put(tg, q, (act[q] && add[q]) ? o + go[q] : o);
-                        }
-                    } else if (mode == PM_CROW) {  // a constant used once: summed after the tile group
+                            for (int q = 0; q < PPT; ++q) o[q] = t.get(tg, q);
 #pragma unroll
-                        for (int q = 0; q < PPT; ++q) t.put(tg, q, (act[q] && add[q]) ? go[q] : T(0));
+                        for (int q = 0; q < PPT; ++q) t.put(tg, q, o[q] + ((act[q] && add[q]) ? go[q] : T(0)));
                     } else if (mode == PM_CONST) {  // a constant used several times: wave sum per use
                         double gv = 0.0;
 #pragma unroll

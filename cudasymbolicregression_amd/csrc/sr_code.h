@@ -233,7 +233,9 @@ __host__ __device__ inline size_t tape_elems(int LT, int PPT) { return (size_t)(
 // row1 18-26 | xg0 27 | xg1 28 (xg: variable read from X).
 // Scatter word: adjoint row of the instruction 0-7 | mode0 8-10 |
 // target0 11-18 | mode1 19-21 | target1 22-29.
-constexpr uint32_t PM_NONE = 0, PM_WRITE = 1, PM_ACC = 2, PM_CONST = 3, PM_VAR = 4, PM_NOOP = 5, PM_CROW = 6;
+// Modes 1-3 write a row (the value 0 + g, or the row's previous value + g
+// for PM_ACC), so one range test selects them.
+constexpr uint32_t PM_NONE = 0, PM_WRITE = 1, PM_CROW = 2, PM_ACC = 3, PM_VAR = 4, PM_CONST = 5, PM_NOOP = 6;
 constexpr uint32_t NO_CROW = 255;
 
 __host__ __device__ inline int pf_op(uint32_t f) { return (int)(f & 63u); }
