@@ -243,7 +243,8 @@ __device__ inline void bwd_nodes(int op, const T *g, const T *a0, const T *a1, T
 
 // ADD, SUB and MUL (the most frequent operators of typical grammars) are
 // evaluated branch-free, ahead of the dispatch tree: one range test instead
-// of a chain of opcode comparisons.  Same values as fwd_nodes / bwd_nodes.
+// of a chain of opcode comparisons; DIV has its own test next.  Same values
+// as fwd_nodes / bwd_nodes.
 template <typename T, int P>
 __device__ inline void fwd_ops(int op, const T *a0, const T *a1, T *r) {
     if (op >= ADD && op <= MUL) {
@@ -252,6 +253,8 @@ __device__ inline void fwd_ops(int op, const T *a0, const T *a1, T *r) {
             const T s = a0[p] + a1[p], d = a0[p] - a1[p], m = a0[p] * a1[p];
             r[p] = op == MUL ? m : (op == SUB ? d : s);
         }
+    } else if (op == DIV) {  // the next most frequent: its own case, not the tree
+        fwd_nodes<T, P>(DIV, a0, a1, r);
     } else {
         fwd_nodes<T, P>(op, a0, a1, r);
     }
@@ -269,6 +272,8 @@ __device__ inline void bwd_ops(int op, const T *g, const T *a0, const T *a1, T *
             add0[p] = add1[p] = !(g[p] == T(0));
             e[p] = 0;
         }
+    } else if (op == DIV) {
+        bwd_nodes<T, P>(DIV, g, a0, a1, go0, add0, go1, add1, e);
     } else {
         bwd_nodes<T, P>(op, g, a0, a1, go0, add0, go1, add1, e);
     }
