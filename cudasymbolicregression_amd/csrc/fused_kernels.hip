@@ -437,24 +437,17 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
 }
 
 // Buckets by tape rows (count_rows_wave) and constant count nc: rows <= 8,
-// <= 16, <= 32, <= 64 (all with nc <= 64), the rest.
-// Each wave classifies a run of 64 consecutive expressions (one whole wave
-// per program) and records it for the deterministic bucket lists
-// (sr_queue.h).
+// <= 16, <= 32, <= 64 (all with nc <= 64), the rest.  One wave per
+// expression; the run counts of the deterministic bucket lists follow in
+// k_queue_count (sr_queue.h).
 __global__ __launch_bounds__(256) void k_classify_fused(const int64_t *__restrict__ ops, const int64_t *__restrict__ ch,
-                                                        int M, int64_t B, int n_x, int8_t *bucket_of,
-                                                        int32_t *runcnt) {
-    const int64_t b0 = ((int64_t)blockIdx.x * 4 + uniform_i32((int)(threadIdx.x / WAVE))) * WAVE;
-    if (b0 >= B) return;
-    const int n = (int)(B - b0 < WAVE ? B - b0 : WAVE);
-    int mine = -1;
-    for (int j = 0; j < n; ++j) {
-        int nc = 0;
-        const int nr = count_rows_wave(ops + (b0 + j) * M, ch + (b0 + j) * M * 2, M, n_x, nc);
-        const int bucket = nc > 64 ? 4 : (nr <= 8 ? 0 : (nr <= 16 ? 1 : (nr <= 32 ? 2 : (nr <= 64 ? 3 : 4))));
-        mine = lane_id() == j ? bucket : mine;
-    }
-    queue_record_run(mine, F_NBUCKETS, b0, n, bucket_of, runcnt);
+                                                        int M, int64_t B, int n_x, int8_t *bucket_of) {
+    const int64_t b = (int64_t)blockIdx.x * 4 + uniform_i32((int)(threadIdx.x / WAVE));
+    if (b >= B) return;
+    int nc = 0;
+    const int nr = count_rows_wave(ops + b * M, ch + b * M * 2, M, n_x, nc);
+    const int bucket = nc > 64 ? 4 : (nr <= 8 ? 0 : (nr <= 16 ? 1 : (nr <= 32 ? 2 : (nr <= 64 ? 3 : 4))));
+    if (lane_id() == 0) bucket_of[b] = (int8_t)bucket;
 }
 
 __global__ void k_fill_i32_e(int32_t *p, int32_t v, int64_t n) {
@@ -612,8 +605,8 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
     int32_t *counters = (int32_t *)(ws + (size_t)F_NBUCKETS * B * 4);  // [0..4] counts, [5..9] work
     hipLaunchKernelGGL(k_fill_i32_e, dim3(1), dim3(64), 0, s, counters, 0, (int64_t)(2 * F_NBUCKETS));
     void *qws = ws + L.q_off;
-    hipLaunchKernelGGL(k_classify_fused, dim3((unsigned)((B + 4 * WAVE - 1) / (4 * WAVE))), dim3(256), 0, s, ops,
-                       ch, (int)M, B, (int)n_x, queue_bucket_of(qws), queue_runcnt(qws, B));
+    hipLaunchKernelGGL(k_classify_fused, dim3((unsigned)((B + 3) / 4)), dim3(256), 0, s, ops, ch, (int)M, B,
+                       (int)n_x, queue_bucket_of(qws));
     queue_finish(qws, B, F_NBUCKETS, lists, counters, s);
     FArgs a;
     memset(&a, 0, sizeof(a));

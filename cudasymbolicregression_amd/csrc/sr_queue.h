@@ -1,8 +1,10 @@
 // Deterministic bucket lists for the persistent work queues.
 //
-// Items (expressions or fitting problems) are classified into NB buckets in
-// runs of 64 consecutive items, one wave per run:
-//   1. the wave writes each item's bucket and its run's per-bucket counts;
+// Items (expressions or fitting problems) are classified into NB buckets
+// (the caller's kernel writes each item's bucket, one wave per item, since
+// a program row is read by a whole wave), then in runs of 64 consecutive
+// items, one wave per run:
+//   1. the wave counts its run's items per bucket;
 //   2. one block turns the counts into per-run offsets (exclusive prefix
 //      sums over runs, bucket by bucket) and the bucket totals;
 //   3. each wave scatters its run into the bucket lists at those offsets.
@@ -22,19 +24,6 @@ constexpr int QRUN = 64;  // items per classification run (one wave)
 inline int64_t queue_runs(int64_t B) { return (B + QRUN - 1) / QRUN; }
 inline size_t queue_ws_bytes(int64_t B, int NB) {
     return (((size_t)B + 255) / 256 * 256) + 2 * (size_t)queue_runs(B) * NB * sizeof(int32_t) + 256;
-}
-
-// Pass 1, called by the wave that classified run r = b0 / 64 (lane j holds
-// the bucket of item b0 + j, n items in the run).
-__device__ inline void queue_record_run(int my_bucket, int NB, int64_t b0, int n, int8_t *bucket_of,
-                                        int32_t *runcnt) {
-    const int lane = (int)(threadIdx.x & (64 - 1));
-    if (lane < n) bucket_of[b0 + lane] = (int8_t)my_bucket;
-    const int64_t r = b0 / QRUN;
-    for (int k = 0; k < NB; ++k) {
-        const int cnt = __popcll(__ballot(lane < n && my_bucket == k));
-        if (lane == 0) runcnt[r * NB + k] = cnt;
-    }
 }
 
 namespace {
@@ -70,6 +59,21 @@ __global__ __launch_bounds__(256) void k_queue_scan(const int32_t *__restrict__ 
     }
 }
 
+// Pass 1: one wave per run, from the buckets in bucket_of.
+__global__ __launch_bounds__(256) void k_queue_count(const int8_t *__restrict__ bucket_of, int64_t B, int NB,
+                                                    int32_t *__restrict__ runcnt) {
+    const int64_t b0 = ((int64_t)blockIdx.x * 4 + uniform_i32((int)(threadIdx.x / WAVE))) * QRUN;
+    if (b0 >= B) return;
+    const int lane = lane_id();
+    const int n = (int)(B - b0 < QRUN ? B - b0 : QRUN);
+    const int my = lane < n ? (int)bucket_of[b0 + lane] : -1;
+    const int64_t r = b0 / QRUN;
+    for (int k = 0; k < NB; ++k) {
+        const int cnt = __popcll(__ballot(my == k));
+        if (lane == 0) runcnt[r * NB + k] = cnt;
+    }
+}
+
 // Pass 3: one wave per run.
 __global__ __launch_bounds__(256) void k_queue_scatter(const int8_t *__restrict__ bucket_of,
                                                       const int32_t *__restrict__ runoff, int64_t B, int NB,
@@ -89,20 +93,21 @@ __global__ __launch_bounds__(256) void k_queue_scatter(const int8_t *__restrict_
 
 }  // namespace
 
-// Passes 2 and 3 (pass 1 is the caller's classification kernel).
+// Passes 1-3, after the caller's classification kernel filled bucket_of.
 inline void queue_finish(void *ws, int64_t B, int NB, int32_t *lists, int32_t *counts, hipStream_t s) {
     char *p = (char *)ws;
     const int8_t *bucket_of = (const int8_t *)p;
     const int64_t R = queue_runs(B);
     int32_t *runcnt = (int32_t *)(p + ((size_t)B + 255) / 256 * 256);
     int32_t *runoff = runcnt + (size_t)R * NB;
+    hipLaunchKernelGGL(k_queue_count, dim3((unsigned)((B + 4 * QRUN - 1) / (4 * QRUN))), dim3(256), 0, s, bucket_of,
+                       B, NB, runcnt);
     hipLaunchKernelGGL(k_queue_scan, dim3(1), dim3(256), 0, s, runcnt, R, NB, runoff, counts);
     hipLaunchKernelGGL(k_queue_scatter, dim3((unsigned)((B + 4 * QRUN - 1) / (4 * QRUN))), dim3(256), 0, s,
                        bucket_of, runoff, B, NB, lists);
 }
 
-// Pointers of pass 1 inside the queue workspace.
+// The per-item buckets inside the queue workspace.
 inline int8_t *queue_bucket_of(void *ws) { return (int8_t *)ws; }
-inline int32_t *queue_runcnt(void *ws, int64_t B) { return (int32_t *)((char *)ws + ((size_t)B + 255) / 256 * 256); }
 
 }  // namespace sr
