@@ -84,7 +84,8 @@ def hip() -> ctypes.CDLL:
         _declare(lib, "sr_fit_set_stationary_exit", I, [I])
         F = c_float
         _declare(lib, "sr_mlp_gemm_nt", I, [P, P, P, I64, I64, I64, I64, I64, I, P, P, P, P])
-        _declare(lib, "sr_mlp_onehot_gelu", I, [P, P, P, P, I64, I64, I64, I, I, I64, P, P, P])
+        _declare(lib, "sr_mlp_onehot_gelu", I, [P, P, P, P, I64, I64, I64, I, I, I64, P, P, P, P])
+        _declare(lib, "sr_mlp_noop_suffix", I, [P, I, I, I64, I64, P, P])
         _declare(lib, "sr_mlp_transpose", I, [P, P, I64, I64, P])
         _declare(lib, "sr_mlp_gelu_bwd", I, [P, P, I64, P, P, P])
         _declare(lib, "sr_mlp_col_sum_workspace_bytes", I64, [I64, I64])

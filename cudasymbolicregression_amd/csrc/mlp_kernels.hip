@@ -97,27 +97,49 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t *__restrict__ A,
 __global__ __launch_bounds__(256) void k_onehot_gelu(const float *__restrict__ zy, const uint16_t *__restrict__ WT,
                                                     const float *__restrict__ bias, const int64_t *__restrict__ ops,
                                                     int M, int64_t B, int slots, int vocab, int H,
-                                                    float *__restrict__ pre, uint16_t *__restrict__ h_out) {
+                                                    const float *__restrict__ noop_suffix, float *__restrict__ pre,
+                                                    uint16_t *__restrict__ h_out) {
     __shared__ int rows[64];
+    __shared__ int tail;  // first slot of the trailing NO_OP run (noop_suffix)
     const int64_t b = blockIdx.x;
-    if (threadIdx.x < 64) {
+    if (threadIdx.x < 64) {  // wave 0
         const int k = threadIdx.x;
         int r = -1;
+        bool op = false;
         if (b < B && k < slots && k < M) {
             const int64_t c = ops[b * M + k];
             if (c >= 0 && c < vocab) r = k * vocab + (int)c;
+            op = c != 0;
         }
         rows[k] = r;
+        const uint64_t m = __ballot(op);
+        if (k == 0) tail = m ? 64 - __clzll(m) : 0;
     }
     __syncthreads();
+    const bool sfx = noop_suffix != nullptr && b < B;
+    const int kend = sfx ? tail : 64;
     for (int j = threadIdx.x; j < H; j += blockDim.x) {
         float z = zy[(size_t)b * H + j] + bias[j];
-        for (int k = 0; k < 64; ++k) {
+        for (int k = 0; k < kend; ++k) {
             const int r = rows[k];
             if (r >= 0) z += bf2f(WT[(size_t)r * H + j]);
         }
+        if (sfx) z += noop_suffix[(size_t)tail * H + j];
         pre[(size_t)b * H + j] = z;
         h_out[(size_t)b * H + j] = f2bf(gelu(z));
+    }
+}
+
+// S[L][j] = sum over slots k = lim-1 down to L of the NO_OP row WT[k*vocab][j]
+// (L = 0 .. lim; S[lim][j] = 0): the one-hot term of a trailing NO_OP run.
+__global__ void k_noop_suffix(const uint16_t *__restrict__ WT, int lim, int vocab, int H, float *__restrict__ S) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= H) return;
+    float acc = 0.0f;
+    S[(size_t)lim * H + j] = 0.0f;
+    for (int k = lim - 1; k >= 0; --k) {
+        acc += bf2f(WT[(size_t)k * vocab * H + j]);
+        S[(size_t)k * H + j] = acc;
     }
 }
 
@@ -324,11 +346,20 @@ int sr_mlp_gemm_nt(const void *A, const void *Bt, const float *bias, int64_t M, 
 }
 
 int sr_mlp_onehot_gelu(const float *zy, const void *WT, const float *bias, const int64_t *ops, int64_t M, int64_t B,
-                       int64_t B_pad, int slots, int vocab, int64_t H, float *pre, void *h_out, void *stream) {
+                       int64_t B_pad, int slots, int vocab, int64_t H, const float *noop_suffix, float *pre,
+                       void *h_out, void *stream) {
     if (M <= 0 || B < 0 || B_pad < B || slots < 0 || slots > 64 || vocab <= 0 || H <= 0) return -1;
     if (B_pad) hipLaunchKernelGGL(k_onehot_gelu, dim3((unsigned)B_pad), dim3(256), 0, (hipStream_t)stream, zy,
-                                  (const uint16_t *)WT, bias, ops, (int)M, B, slots, vocab, (int)H, pre,
+                                  (const uint16_t *)WT, bias, ops, (int)M, B, slots, vocab, (int)H, noop_suffix, pre,
                                   (uint16_t *)h_out);
+    return last();
+}
+
+int sr_mlp_noop_suffix(const void *WT, int slots, int vocab, int64_t M, int64_t H, float *S, void *stream) {
+    if (slots < 0 || slots > 64 || vocab <= 0 || M <= 0 || H <= 0 || H > INT32_MAX) return -1;
+    const int lim = slots < M ? slots : (int)M;
+    hipLaunchKernelGGL(k_noop_suffix, dim3(blocks(H, 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const uint16_t *)WT, lim, vocab, (int)H, S);
     return last();
 }
 

@@ -78,6 +78,7 @@ class WarmStartMLP:
         # first layer's BF16 weights transposed (one row per input column): the
         # one-hot block of the input is applied as a sum of gathered rows
         self.WT0 = torch.empty(self.dims[0], self.dims[1], dtype=torch.bfloat16, device=dev)
+        self.noop_suffix = torch.empty(self.slots + 1, self.dims[1], dtype=torch.float32, device=dev)
         self.onehot = self.slots * self.vocab
         self.mW = [torch.zeros_like(w) for w in self.W]
         self.vW = [torch.zeros_like(w) for w in self.W]
@@ -172,8 +173,12 @@ class WarmStartMLP:
         _ck(lib.sr_mlp_gemm_nt(x.data_ptr() + 2 * oh, self.Wb[0].data_ptr() + 2 * oh, None, Bp, H1, ky,
                                self.row_len, self.row_len, EPI_STORE, _lib.ptr(buf["z"][0]), None, None, s),
             "sr_mlp_gemm_nt")
+        # the trailing NO_OP slots (program padding) as one precomputed sum
+        _ck(lib.sr_mlp_noop_suffix(_lib.ptr(self.WT0), self.slots, self.vocab, M, H1, _lib.ptr(self.noop_suffix), s),
+            "sr_mlp_noop_suffix")
         _ck(lib.sr_mlp_onehot_gelu(_lib.ptr(buf["z"][0]), _lib.ptr(self.WT0), _lib.ptr(self.b[0]), _lib.ptr(ops), M, B,
-                                   Bp, self.slots, self.vocab, H1, _lib.ptr(buf["z"][0]), _lib.ptr(buf["h"][0]), s),
+                                   Bp, self.slots, self.vocab, H1, _lib.ptr(self.noop_suffix), _lib.ptr(buf["z"][0]),
+                                   _lib.ptr(buf["h"][0]), s),
             "sr_mlp_onehot_gelu")
         h = buf["h"][0]
         for l in range(1, self.layers):

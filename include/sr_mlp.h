@@ -43,9 +43,17 @@ int sr_mlp_gemm_nt(const void *A, const void *Bt, const float *bias, int64_t M, 
  * weights transposed, one row per input column); pre = z (FP32) and
  * h_out = gelu(z) (BF16), both B_pad x H.  zy (B_pad x H, FP32, may alias
  * pre) is the product of the remaining input columns; rows b >= B take no
- * one-hot term. */
+ * one-hot term.  With noop_suffix (from sr_mlp_noop_suffix for the same WT,
+ * slots, vocab, M and H; nullable) the trailing run of NO_OP slots of a row
+ * adds one precomputed sum instead of a gathered row per slot (the padding
+ * of short programs); the terms are the same, summed in another order. */
 int sr_mlp_onehot_gelu(const float *zy, const void *WT, const float *bias, const int64_t *ops, int64_t M, int64_t B,
-                       int64_t B_pad, int slots, int vocab, int64_t H, float *pre, void *h_out, void *stream);
+                       int64_t B_pad, int slots, int vocab, int64_t H, const float *noop_suffix, float *pre,
+                       void *h_out, void *stream);
+
+/* S[L][j] = sum of WT[k*vocab + NO_OP][j] over slots L <= k < min(slots, M),
+ * for L = 0 .. min(slots, M) (S: (min(slots, M) + 1) x H FP32). */
+int sr_mlp_noop_suffix(const void *WT, int slots, int vocab, int64_t M, int64_t H, float *S, void *stream);
 
 /* out[c][r] = in[r][c] for a rows x cols BF16 matrix. */
 int sr_mlp_transpose(const void *in, void *out, int64_t rows, int64_t cols, void *stream);
