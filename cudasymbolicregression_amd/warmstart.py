@@ -330,8 +330,10 @@ def select_start(X, Y, Ops, Ch, C_warm, C_rand) -> Tuple[torch.Tensor, torch.Ten
     is NaN or above the random start's).  Returns (C_start, used_warm (B,)
     int32, loss_warm, loss_rand); losses are FP64 means over the points."""
     B, M = Ops.shape
-    lw = fused_root(X, Ops, Ch, C_warm, Y, mode=1, fwd_only=True, raise_errors=False)["loss"]
-    lr = fused_root(X, Ops, Ch, C_rand, Y, mode=1, fwd_only=True, raise_errors=False)["loss"]
+    # both starts in one evaluation call (2B expressions)
+    both = fused_root(X, torch.cat([Ops, Ops]), torch.cat([Ch, Ch]), torch.cat([C_warm, C_rand]), torch.cat([Y, Y]),
+                      mode=1, fwd_only=True, raise_errors=False)["loss"]
+    lw, lr = both[:B], both[B:]
     C = torch.empty((B, M), dtype=torch.float32, device=Ops.device)
     used = torch.empty(B, dtype=torch.int32, device=Ops.device)
     _ck(_lib.hip().sr_mlp_select_start(_lib.ptr(C_warm), _lib.ptr(C_rand), _lib.ptr(lw), _lib.ptr(lr), M, B,
