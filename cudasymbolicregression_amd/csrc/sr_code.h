@@ -228,7 +228,8 @@ __host__ __device__ inline size_t tape_elems(int LT, int PPT) { return (size_t)(
 //     per-constant sums after each tile group.  A constant used more than
 //     once is reduced over the wave at every use (PM_CONST).
 // The bucket of a program is chosen by its row count NI + extra rows
-// (count_rows_wave), at most 2 NI.
+// (count_rows_wave).  With n operator nodes and l leaves (n + l <= M), the
+// rows are at most (n + 1) + n + l / 2 <= 2 M - 1.
 // Fetch word: op 0-5 | binary 6 | scalar0 7 | row0 8-16 | scalar1 17 |
 // row1 18-26 | xg0 27 | xg1 28 (xg: variable read from X).
 // Scatter word: adjoint row of the instruction 0-7 | mode0 8-10 |
