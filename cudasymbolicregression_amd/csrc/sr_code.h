@@ -103,7 +103,6 @@ constexpr int IMM_NOOP = 8;  // a NO_OP node used as an operand: NaN, and 201 if
 constexpr int OP_COPY = 63;
 constexpr int MAX_NX_CODE = 2047;
 
-__host__ __device__ inline int instr_op(uint32_t w) { return (int)(w & 63u); }
 __host__ __device__ inline uint32_t instr_d0(uint32_t w) { return (w >> 6) & 0x1FFFu; }
 __host__ __device__ inline uint32_t instr_d1(uint32_t w) { return w >> 19; }
 __host__ __device__ inline int desc_kind(uint32_t d) { return (int)(d & 3u); }
