@@ -10,13 +10,18 @@
 // Every matrix product is C[M,N] = A[M,K] * Bt[N,K]^T ("NT", both operands
 // K-contiguous), which is the natural layout of a linear layer (W stored
 // out x in); the backward products use it after explicit transposes.
-// One wave computes a 32x32 tile with v_mfma_f32_32x32x16_bf16; a block of
-// four waves covers 64x64.  Operand fragments are read straight from global
-// memory (16 contiguous bytes per lane and k-step), which is enough for this
-// network's size (the fitting kernels dominate the time).
+// Products large enough to fill the GPU with 128x128 blocks use k_gemm_nt:
+// one wave computes a 64x64 tile as 2x2 v_mfma_f32_32x32x16_bf16 tiles (each
+// fragment feeds two MFMAs) with the next K step's fragments in flight; the
+// others use k_gemm_nt_small (one 32x32 tile per wave, a block covering
+// 64x64).  Operand fragments are read straight from global memory (16
+// contiguous bytes per lane and k-step).
+#include "sr_device_info.h"
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <math.h>
+
+using sr::num_cus;
 
 namespace {
 
@@ -49,27 +54,15 @@ enum Epi : int { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2 };
 // MFMA 32x32x16 lane layout (per wave): lane l holds A[i = l % 32][k0 + 8*(l/32) + e]
 // and Bt[j = l % 32][k0 + 8*(l/32) + e] for e = 0..7; accumulator register r of
 // lane l is C[8*(r/4) + 4*(l/32) + r%4][l % 32].
-__global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t *__restrict__ A, const uint16_t *__restrict__ Bt,
-                                                const float *__restrict__ bias, int M, int N, int K, int lda, int ldb,
-                                                int epi,
-                                                float *__restrict__ out_f32, uint16_t *__restrict__ out_bf16,
-                                                float *__restrict__ pre_f32) {
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const int i0 = blockIdx.y * 64 + (wave / 2) * 32;
-    const int j0 = blockIdx.x * 64 + (wave % 2) * 32;
-    if (i0 >= M || j0 >= N) return;  // whole wave leaves together
-    const int r = lane % 32, kh = (lane / 32) * 8;
-    const uint16_t *pa = A + (size_t)(i0 + r) * lda + kh;
-    const uint16_t *pb = Bt + (size_t)(j0 + r) * ldb + kh;
-    f32x16 acc;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) acc[q] = 0.0f;
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(pa + k0);
-        const bf16x8 b = *reinterpret_cast<const bf16x8 *>(pb + k0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-    }
-    const int j = j0 + r;
+// A block of 4 waves computes 128 x 128 outputs, each wave 64 x 64 as 2 x 2
+// MFMA tiles, so every fragment loaded feeds two MFMAs; the fragments of the
+// next K step are loaded before this step's MFMAs are issued.  Fragment rows
+// past M or N (the edges of the last block) are clamped to row M-1 / N-1:
+// loaded and multiplied, never stored.
+__device__ inline void gemm_store_tile(const f32x16 &acc, int i0, int j0, int lane, int N, int epi,
+                                       const float *__restrict__ bias, float *__restrict__ out_f32,
+                                       uint16_t *__restrict__ out_bf16, float *__restrict__ pre_f32) {
+    const int j = j0 + lane % 32;
     const float bj = (epi != EPI_STORE && bias) ? bias[j] : 0.0f;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
@@ -86,6 +79,78 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t *__restrict__ A,
             if (out_bf16) out_bf16[o] = f2bf(z);
         }
     }
+}
+
+// The same product with one 32 x 32 tile per wave (a block covers 64 x 64):
+// four times the blocks of k_gemm_nt, for products too small to fill the
+// GPU with 128 x 128 blocks (the weight gradients, M = N = 512, K = 4096).
+__global__ __launch_bounds__(256) void k_gemm_nt_small(const uint16_t *__restrict__ A,
+                                                      const uint16_t *__restrict__ Bt, const float *__restrict__ bias,
+                                                      int M, int N, int K, int lda, int ldb, int epi,
+                                                      float *__restrict__ out_f32, uint16_t *__restrict__ out_bf16,
+                                                      float *__restrict__ pre_f32) {
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int i0 = blockIdx.y * 64 + (wave / 2) * 32;
+    const int j0 = blockIdx.x * 64 + (wave % 2) * 32;
+    if (i0 >= M || j0 >= N) return;  // whole wave leaves together
+    const int r = lane % 32, kh = (lane / 32) * 8;
+    const uint16_t *pa = A + (size_t)(i0 + r) * lda + kh;
+    const uint16_t *pb = Bt + (size_t)(j0 + r) * ldb + kh;
+    f32x16 acc;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.0f;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(pa + k0);
+        const bf16x8 b = *reinterpret_cast<const bf16x8 *>(pb + k0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    gemm_store_tile(acc, i0, j0, lane, N, epi, bias, out_f32, out_bf16, pre_f32);
+}
+
+__global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t *__restrict__ A, const uint16_t *__restrict__ Bt,
+                                                const float *__restrict__ bias, int M, int N, int K, int lda, int ldb,
+                                                int epi,
+                                                float *__restrict__ out_f32, uint16_t *__restrict__ out_bf16,
+                                                float *__restrict__ pre_f32) {
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int i0 = blockIdx.y * 128 + (wave / 2) * 64;
+    const int j0 = blockIdx.x * 128 + (wave % 2) * 64;
+    if (i0 >= M || j0 >= N) return;  // whole wave leaves together
+    const int r = lane % 32, kh = (lane / 32) * 8;
+    const int ra = i0 + r, rb = i0 + 32 + r < M ? i0 + 32 + r : M - 1;
+    const int ca = j0 + r, cb = j0 + 32 + r < N ? j0 + 32 + r : N - 1;
+    const uint16_t *pa0 = A + (size_t)ra * lda + kh, *pa1 = A + (size_t)rb * lda + kh;
+    const uint16_t *pb0 = Bt + (size_t)ca * ldb + kh, *pb1 = Bt + (size_t)cb * ldb + kh;
+    auto ld = [](const uint16_t *p) { return *reinterpret_cast<const bf16x8 *>(p); };
+    f32x16 c00, c01, c10, c11;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) c00[q] = c01[q] = c10[q] = c11[q] = 0.0f;
+    auto step = [&](const bf16x8 &x0, const bf16x8 &x1, const bf16x8 &y0, const bf16x8 &y1) {
+        c00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0, y0, c00, 0, 0, 0);
+        c01 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0, y1, c01, 0, 0, 0);
+        c10 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x1, y0, c10, 0, 0, 0);
+        c11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x1, y1, c11, 0, 0, 0);
+    };
+    // two register sets: the loads of one K step are in flight while the
+    // other set's MFMAs run
+    bf16x8 a0 = ld(pa0), a1 = ld(pa1), b0 = ld(pb0), b1 = ld(pb1);
+    int k0 = 0;
+    for (; k0 + 32 <= K; k0 += 32) {
+        const bf16x8 e0 = ld(pa0 + k0 + 16), e1 = ld(pa1 + k0 + 16), f0 = ld(pb0 + k0 + 16), f1 = ld(pb1 + k0 + 16);
+        step(a0, a1, b0, b1);
+        const int kn = k0 + 32 < K ? k0 + 32 : k0;  // past the end: reload (in bounds, unused)
+        a0 = ld(pa0 + kn);
+        a1 = ld(pa1 + kn);
+        b0 = ld(pb0 + kn);
+        b1 = ld(pb1 + kn);
+        step(e0, e1, f0, f1);
+    }
+    if (k0 < K) step(a0, a1, b0, b1);  // K an odd multiple of 16
+    const bool i1 = i0 + 32 < M, j1 = j0 + 32 < N;
+    gemm_store_tile(c00, i0, j0, lane, N, epi, bias, out_f32, out_bf16, pre_f32);
+    if (j1) gemm_store_tile(c01, i0, j0 + 32, lane, N, epi, bias, out_f32, out_bf16, pre_f32);
+    if (i1) gemm_store_tile(c10, i0 + 32, j0, lane, N, epi, bias, out_f32, out_bf16, pre_f32);
+    if (i1 && j1) gemm_store_tile(c11, i0 + 32, j0 + 32, lane, N, epi, bias, out_f32, out_bf16, pre_f32);
 }
 
 // First layer with a one-hot block in its input: z = zy + b + sum_k WT[k V +
@@ -339,9 +404,14 @@ int sr_mlp_gemm_nt(const void *A, const void *Bt, const float *bias, int64_t M, 
     if (A == nullptr || Bt == nullptr || ((uintptr_t)A | (uintptr_t)Bt) % 16 != 0) return -1;
     if (out_f32 == nullptr && out_bf16 == nullptr) return -1;
     if (epi != EPI_STORE && bias == nullptr) return -1;
-    hipLaunchKernelGGL(k_gemm_nt, dim3(blocks(N, 64), blocks(M, 64)), dim3(256), 0, (hipStream_t)stream,
-                       (const uint16_t *)A, (const uint16_t *)Bt, bias, (int)M, (int)N, (int)K, (int)lda, (int)ldb,
-                       epi, out_f32, (uint16_t *)out_bf16, pre_f32);
+    if (blocks(N, 128) * blocks(M, 128) >= 2 * (int64_t)num_cus())
+        hipLaunchKernelGGL(k_gemm_nt, dim3(blocks(N, 128), blocks(M, 128)), dim3(256), 0, (hipStream_t)stream,
+                           (const uint16_t *)A, (const uint16_t *)Bt, bias, (int)M, (int)N, (int)K, (int)lda,
+                           (int)ldb, epi, out_f32, (uint16_t *)out_bf16, pre_f32);
+    else
+        hipLaunchKernelGGL(k_gemm_nt_small, dim3(blocks(N, 64), blocks(M, 64)), dim3(256), 0, (hipStream_t)stream,
+                           (const uint16_t *)A, (const uint16_t *)Bt, bias, (int)M, (int)N, (int)K, (int)lda,
+                           (int)ldb, epi, out_f32, (uint16_t *)out_bf16, pre_f32);
     return last();
 }
 
