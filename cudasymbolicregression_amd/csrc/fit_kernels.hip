@@ -143,6 +143,10 @@ __device__ inline void closure_code(const Code &c, const Plan &pl, int nc, const
     constexpr int HW = LT > WAVE ? 2 : 1;
     float sv0[HW], sv1[HW];
     resolve_scalars<float, SL, LT>(c, xt, sv0, sv1);
+    auto scalar = [&](const float (&sv)[HW], int i) -> float {  // instruction i's scalar operand
+        if constexpr (HW == 1) return rl(sv[0], i);
+        else return i < WAVE ? rl(sv[0], i) : rl(sv[HW - 1], i - WAVE);
+    };
     double lacc = 0.0;
     double cacc[FIT_NCREG];
 #pragma unroll
@@ -172,8 +176,7 @@ __device__ inline void closure_code(const Code &c, const Plan &pl, int nc, const
         // it); variables v >= NX are read from X
         auto fetch = [&](uint32_t f, int i, float (&a0)[PPT], float (&a1)[PPT]) {
             const bool bin = pf_bin(f);
-            const float s0 = HW == 1 ? rl(sv0[0], i) : (i < WAVE ? rl(sv0[0], i) : rl(sv0[HW - 1], i - WAVE));
-            const float s1 = HW == 1 ? rl(sv1[0], i) : (i < WAVE ? rl(sv1[0], i) : rl(sv1[HW - 1], i - WAVE));
+            const float s0 = scalar(sv0, i), s1 = scalar(sv1, i);
             if (pf_xg(f, 0) || pf_xg(f, 1)) {
                 const uint32_t w = c.word_lt<LT>(i);
 #pragma unroll

@@ -138,6 +138,10 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
         constexpr int HW = LT > WAVE ? 2 : 1;
         T sv0[HW], sv1[HW];
         resolve_scalars<T, SL, LT>(c, xc, sv0, sv1);
+        auto scalar = [&](const T (&sv)[HW], int ii) -> T {  // instruction ii's scalar operand
+            if constexpr (HW == 1) return rl(sv[0], ii);
+            else return ii < WAVE ? rl(sv[0], ii) : rl(sv[HW - 1], ii - WAVE);
+        };
         // this expression's points (the shared X, or its own rows when ragged)
         const int64_t Nb = a.n_pts ? (int64_t)a.n_pts[b] : N;
         const int64_t yo = a.y_off ? a.y_off[b] : b * N;
@@ -217,8 +221,7 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
 #pragma unroll
                     for (int k = 0; k < 2; ++k) {
                         T *av = k == 0 ? a0 : a1;
-                        const T sv = k == 0 ? (HW == 1 ? rl(sv0[0], ii) : (ii < WAVE ? rl(sv0[0], ii) : rl(sv0[HW - 1], ii - WAVE)))
-                                            : (HW == 1 ? rl(sv1[0], ii) : (ii < WAVE ? rl(sv1[0], ii) : rl(sv1[HW - 1], ii - WAVE)));
+                        const T sv = k == 0 ? scalar(sv0, ii) : scalar(sv1, ii);
                         const int v = desc_idx(k == 0 ? instr_d0(w) : instr_d1(w));
                         const bool xg = pf_xg(f, k), sc = pf_sc(f, k);
                         const int row = pf_row(f, k);
@@ -231,8 +234,7 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
                 } else {
                     const int r0 = pf_row(f, 0), r1 = pf_row(f, 1);
                     const bool c0 = pf_sc(f, 0), c1 = pf_sc(f, 1);
-                    const T s0 = HW == 1 ? rl(sv0[0], ii) : (ii < WAVE ? rl(sv0[0], ii) : rl(sv0[HW - 1], ii - WAVE));
-                    const T s1 = HW == 1 ? rl(sv1[0], ii) : (ii < WAVE ? rl(sv1[0], ii) : rl(sv1[HW - 1], ii - WAVE));
+                    const T s0 = scalar(sv0, ii), s1 = scalar(sv1, ii);
                     T t0[PPT], t1[PPT];
 #pragma unroll
                     for (int q = 0; q < PPT; ++q) t0[q] = t.get(r0, q);
