@@ -76,7 +76,8 @@ LOSS_SLOTS = 5  # d = y - Y, d*d, accumulate, 2*d, *1/N
 FP32_PEAK_TFLOPS = 157.3
 VALU_LANE_PEAK = 256 * 4 * 16 * 2.4e9  # unpacked FP32 lane-instructions/s (3.93e13)
 HBM_PEAK_GBS = 8000.0
-FIT_TRAFFIC_PROFILED = 81.2e6  # HBM bytes per fit call, rocprofv3 FETCH_SIZE/WRITE_SIZE (calibrated)
+FIT_TRAFFIC_PROFILED = 101.3e6  # HBM bytes per fit call, rocprofv3 FETCH_SIZE/WRITE_SIZE (calibrated)
+C2_TRAFFIC_PROFILED = 887.6e6  # HBM bytes per C2 evaluate call, same counters
 
 
 def log(msg: str) -> None:
@@ -124,7 +125,8 @@ def bench_evaluate(dev, steps: int, seed: int = 1, B: int = 65536, N: int = 1024
     t = ms / 1e3
     return dict(workload=f"G2 B={B}, padded 64, N={N}, n_x=3, fused fwd+bwd with grad_X", ms=round(ms, 4),
                 hbm=dict(achieved_GBps=round(nbytes / t / 1e9, 1), peak_GBps=HBM_PEAK_GBS,
-                         frac=round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4), bytes=nbytes),
+                         frac=round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4), bytes=nbytes,
+                         traffic=C2_TRAFFIC_PROFILED, traffic_source="profiles/evaluate_c2_hbm_traffic.md"),
                 valu=dict(achieved_TFLOPs=round(slots / t / 1e12, 4), peak_TFLOPs=FP32_PEAK_TFLOPS,
                           frac=round(slots / t / 1e12 / FP32_PEAK_TFLOPS, 5),
                           frac_unpacked_lane_peak=round(slots / t / VALU_LANE_PEAK, 4), slots=slots),
@@ -316,6 +318,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-evaluate", action="store_true")
     ap.add_argument("--no-ragged", action="store_true")
+    ap.add_argument("--no-warm", action="store_true",
+                    help="plain fits only: no MLP training and no warm-start step (value is the plain rate)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer (calibrates the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -351,44 +355,57 @@ def main() -> int:
     log(f"test set ready: {args.problems} problems in {time.time() - t0:.1f} s")
 
     # ---- training (untimed): labels from the plain L-BFGS on a disjoint sample
-    t0 = time.time()
-    tops, tch, tY, label, tstats = warmstart.make_training_set(g, X, args.train_tuples, args.seed - 1)
-    model = warmstart.WarmStartMLP(dev, seed=0)
-    torch.cuda.synchronize()
-    t1 = time.time()
-    hist = warmstart.train(model, tops, tY, label, epochs=args.epochs, batch=4096, lr=1e-3, lr_final=1e-5)
-    torch.cuda.synchronize()
-    tstats.update(train_s=round(time.time() - t1, 2), epochs=args.epochs, batch=4096,
-                  loss_first_epoch=round(hist[0], 4), loss_last_epoch=round(hist[-1], 4),
-                  zero_predictor_loss=round(float((label.double() ** 2).sum()) / max(1, label.shape[0]), 4))
-    log(f"training: {tstats} ({time.time() - t0:.1f} s)")
-    del tops, tch, tY, label
+    tstats = None
+    if not args.no_warm:
+        t0 = time.time()
+        tops, tch, tY, label, tstats = warmstart.make_training_set(g, X, args.train_tuples, args.seed - 1)
+        model = warmstart.WarmStartMLP(dev, seed=0)
+        torch.cuda.synchronize()
+        t1 = time.time()
+        hist = warmstart.train(model, tops, tY, label, epochs=args.epochs, batch=4096, lr=1e-3, lr_final=1e-5)
+        torch.cuda.synchronize()
+        tstats.update(train_s=round(time.time() - t1, 2), epochs=args.epochs, batch=4096,
+                      loss_first_epoch=round(hist[0], 4), loss_last_epoch=round(hist[-1], 4),
+                      zero_predictor_loss=round(float((label.double() ** 2).sum()) / max(1, label.shape[0]), 4))
+        log(f"training: {tstats} ({time.time() - t0:.1f} s)")
+        del tops, tch, tY, label
 
     # ---- plain: randn starts
     ms_p, (C_p, loss_p, iters_p, err_p, _, ncl_p) = timed(lambda: fit(w, w["C0"]), args.steps, args.warmup)
     fit_p = int((warmstart.rel_mse_fp64(X, Y, ops, ch, C_p) <= 1e-6).sum())
     log(f"plain: {ms_p:.2f} ms/step, {fit_p} fitted")
 
-    # ---- warm: network prediction, start selection, fit
-    def warm_step():
-        C_start, used, _, _ = warmstart.warm_start(model, X, Y, ops, ch, w["C0"])
-        return fit(w, C_start) + (used,)
-
-    ms_w, (C_w, loss_w, iters_w, err_w, _, ncl_w, used) = timed(warm_step, args.steps, args.warmup)
-    fit_w = int((warmstart.rel_mse_fp64(X, Y, ops, ch, C_w) <= 1e-6).sum())
-    ms_ws, _ = timed(lambda: warmstart.warm_start(model, X, Y, ops, ch, w["C0"]), args.steps, 1)
-    log(f"warm: {ms_w:.2f} ms/step ({ms_ws:.2f} ms start selection), {fit_w} fitted")
-    value = fit_w / (ms_w / 1000.0)
     plain_value = fit_p / (ms_p / 1000.0)
+    warm = None
+    if not args.no_warm:
+        # ---- warm: network prediction, start selection, fit
+        def warm_step():
+            C_start, used, _, _ = warmstart.warm_start(model, X, Y, ops, ch, w["C0"])
+            return fit(w, C_start) + (used,)
 
-    # roofline of the fitting kernels (warm step): algorithmic slots / fit time
-    fit_ms = ms_w - ms_ws
+        ms_w, (C_w, loss_w, iters_w, err_w, _, ncl_w, used) = timed(warm_step, args.steps, args.warmup)
+        fit_w = int((warmstart.rel_mse_fp64(X, Y, ops, ch, C_w) <= 1e-6).sum())
+        ms_ws, _ = timed(lambda: warmstart.warm_start(model, X, Y, ops, ch, w["C0"]), args.steps, 1)
+        log(f"warm: {ms_w:.2f} ms/step ({ms_ws:.2f} ms start selection), {fit_w} fitted")
+        value = fit_w / (ms_w / 1000.0)
+        warm = {"ms": round(ms_w, 3), "start_selection_ms": round(ms_ws, 3), "fitted": fit_w,
+                "fitted_per_s": round(value, 1), "warm_start_taken": round(float(used.double().mean()), 4),
+                "mean_iters": round(float(iters_w.double().mean()), 2),
+                "closures": int(ncl_w.sum()), "problems_without_error": int((err_w == 0).sum())}
+        fit_ms, ncl_r, ms_step = ms_w - ms_ws, ncl_w, ms_w
+    else:
+        value = plain_value
+        fit_ms, ncl_r, ms_step = ms_p, ncl_p, ms_p
+
+    # roofline of the fitting kernels (warm step, or the plain step without
+    # one): algorithmic slots / fit time
     per_point = torch.tensor([expression_cost(row) + LOSS_SLOTS for row in ops.cpu().tolist()], dtype=torch.float64)
-    flops = float((per_point * ncl_w.cpu().double() * X.shape[0]).sum())
+    flops = float((per_point * ncl_r.cpu().double() * X.shape[0]).sum())
     achieved = flops / (fit_ms / 1000.0) / 1e12
     roofline = dict(bound="mfma", achieved=round(achieved, 4), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=round(achieved / FP32_PEAK_TFLOPS, 5), traffic=None,
-                    traffic_profiled_bytes=FIT_TRAFFIC_PROFILED, traffic_source="profiles/fit_hbm_traffic.md",
+                    frac=round(achieved / FP32_PEAK_TFLOPS, 5), traffic=FIT_TRAFFIC_PROFILED,
+                    traffic_source="profiles/fit_hbm_traffic.md: rocprofv3 FETCH_SIZE/WRITE_SIZE of the plain-start "
+                                   "fit call (the same kernels on the same problems), not measured live",
                     kernel="sr_lbfgs_fit: the k_fit bucket kernels of one call (concurrent)",
                     flop_per_launch=flops, frac_unpacked_lane_peak=round(flops / (fit_ms / 1e3) / VALU_LANE_PEAK, 5))
 
@@ -414,13 +431,14 @@ def main() -> int:
         log(f"cpu baseline: {cpu['value']:.2f} fitted/s ({cpu['sample']})")
 
     out = {
-        "metric": "expressions fitted/sec (rel-MSE<=1e-6), warm-start vs plain L-BFGS, 1 MI355X",
+        "metric": ("expressions fitted/sec (rel-MSE<=1e-6), plain L-BFGS only, 1 MI355X" if args.no_warm else
+                   "expressions fitted/sec (rel-MSE<=1e-6), warm-start vs plain L-BFGS, 1 MI355X"),
         "value": round(value, 2),
         "unit": "expressions fitted/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_w, 3),
+        "ms_per_step": round(ms_step, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -431,14 +449,11 @@ def main() -> int:
                         "seed-fixed points X~U(-3,3), n_x=3, C*~N(0,2^2); MLP warm start with randn fallback",
             "problems": args.problems, "n_points": 512, "n_x": 3, "max_iter": 100, "history": 10,
             "line_search": "strong_wolfe", "tol_grad": 1e-7, "stop_rel_loss": 5e-7,
-            "warm": {"ms": round(ms_w, 3), "start_selection_ms": round(ms_ws, 3), "fitted": fit_w,
-                     "fitted_per_s": round(value, 1), "warm_start_taken": round(float(used.double().mean()), 4),
-                     "mean_iters": round(float(iters_w.double().mean()), 2),
-                     "closures": int(ncl_w.sum()), "problems_without_error": int((err_w == 0).sum())},
+            "warm": warm,
             "plain": {"ms": round(ms_p, 3), "fitted": fit_p, "fitted_per_s": round(plain_value, 1),
                       "mean_iters": round(float(iters_p.double().mean()), 2), "closures": int(ncl_p.sum()),
                       "problems_without_error": int((err_p == 0).sum())},
-            "warm_over_plain": round(value / plain_value, 4) if plain_value > 0 else None,
+            "warm_over_plain": round(value / plain_value, 4) if warm is not None and plain_value > 0 else None,
             "training": tstats,
         },
         "roofline": roofline,

@@ -132,10 +132,15 @@ struct Code {
     }
 };
 
+// Selects rather than stores under a branch: a store to either field under
+// a condition becomes a store through a computed address, which puts the
+// struct in private (scratch) memory.
 __device__ inline void place_word(Code &c, int i, uint32_t v) {
     const int lane = lane_id();
-    if (i < WAVE) { if (lane == i) c.iw0 = v; }
-    else if (lane == i - WAVE) c.iw1 = v;
+    const bool hi = i >= WAVE;
+    const bool mine = lane == (hi ? i - WAVE : i);
+    c.iw0 = (mine && !hi) ? v : c.iw0;
+    c.iw1 = (mine && hi) ? v : c.iw1;
 }
 
 template <typename T>
@@ -410,21 +415,23 @@ __device__ inline Plan compile_plan(const CProg<T> &p, int M, const Code &c, int
     pl.f0 = pl.f1 = pl.s0 = pl.s1 = 0u;
     pl.crow0 = pl.crow1 = NO_CROW;
     pl.nextra = ne + __popcll(dm0) + __popcll(dm1);
+    // (selects, as in place_word, keep the plan in registers)
     auto place = [&](int i, uint32_t f, uint32_t s) {
-        if (i < WAVE) {
-            if (lane == i) { pl.f0 = f; pl.s0 = s; }
-        } else if (lane == i - WAVE) {
-            pl.f1 = f;
-            pl.s1 = s;
-        }
+        const bool hi = i >= WAVE;
+        const bool mine = lane == (hi ? i - WAVE : i);
+        pl.f0 = (mine && !hi) ? f : pl.f0;
+        pl.s0 = (mine && !hi) ? s : pl.s0;
+        pl.f1 = (mine && hi) ? f : pl.f1;
+        pl.s1 = (mine && hi) ? s : pl.s1;
     };
     // the constant ordinal of an operand record, and its row, into lane o
     auto place_crow = [&](uint32_t rec, uint32_t row) {
-        if ((rec >> 22) & 1u) {
-            const int o = (int)((rec >> 14) & 255u);
-            if (o < WAVE) { if (lane == o) pl.crow0 = row; }
-            else if (lane == o - WAVE) pl.crow1 = row;
-        }
+        const bool once = (rec >> 22) & 1u;
+        const int o = (int)((rec >> 14) & 255u);
+        const bool hi = o >= WAVE;
+        const bool mine = once && lane == (hi ? o - WAVE : o);
+        pl.crow0 = (mine && !hi) ? row : pl.crow0;
+        pl.crow1 = (mine && hi) ? row : pl.crow1;
     };
     int i = 0;
     for (uint64_t m = om0; m; m &= m - 1, ++i) {
