@@ -170,15 +170,18 @@ class _SymbolicEvaluation(torch.autograd.Function):
         M, B = Ops.shape
         N, n_x = X.shape
         cache = torch.empty((M, N, B), dtype=X.dtype, device=X.device)
-        sfx = "f64" if X.dtype == torch.float64 else "f32"
+        is64 = X.dtype == torch.float64
+        sfx = "f64" if is64 else "f32"
         lib = _lib.hip()
+        ws = _workspace(lib.sr_forward_mb_workspace_bytes(M, N, B, 1, int(is64)), X.device)
         _lib.check_status(
-            getattr(lib, f"sr_forward_mb_{sfx}")(
+            getattr(lib, f"sr_forward_mb_ws_{sfx}")(
                 _lib.ptr(X), N, n_x, _lib.ptr(Ops), _lib.ptr(Ch), _lib.ptr(C), M, B, _lib.ptr(cache),
-                _lib.stream_of(X.device),
+                _lib.ptr(ws), ws.numel(), _lib.stream_of(X.device),
             ),
             "symbolic forward",
         )
+        del ws
         ctx.save_for_backward(X, Ops, Ch, C, cache)
         return cache
 
@@ -282,15 +285,18 @@ class _SymbolicEvaluationMultiple(torch.autograd.Function):
         M, B, K = C.shape
         N, n_x = X.shape
         cache = torch.empty((M, N, B, K), dtype=X.dtype, device=X.device)
-        sfx = "f64" if X.dtype == torch.float64 else "f32"
+        is64 = X.dtype == torch.float64
+        sfx = "f64" if is64 else "f32"
         lib = _lib.hip()
+        ws = _workspace(lib.sr_forward_mb_workspace_bytes(M, N, B, K, int(is64)), X.device)
         _lib.check_status(
-            getattr(lib, f"sr_multi_forward_mb_{sfx}")(
+            getattr(lib, f"sr_multi_forward_mb_ws_{sfx}")(
                 _lib.ptr(X), N, n_x, _lib.ptr(Ops), _lib.ptr(Ch), _lib.ptr(C), M, B, K, _lib.ptr(cache),
-                _lib.stream_of(X.device),
+                _lib.ptr(ws), ws.numel(), _lib.stream_of(X.device),
             ),
             "symbolic multiple forward",
         )
+        del ws
         ctx.save_for_backward(X, Ops, Ch, C, cache)
         return cache
 

@@ -89,6 +89,92 @@ __global__ __launch_bounds__(64) void k_full_forward(const T *__restrict__ X, in
 }
 
 // ---------------------------------------------------------------------------
+// Row-buffer forward (with a workspace).  Most of the cache is NO_OP padding,
+// and the grouped kernel's stores cover EW * 4 contiguous bytes per point,
+// which holds HBM writes to about a sixth of their peak.  With a workspace the
+// forward runs in two passes over groups of FG = 64 consecutive expressions:
+//   pass 1 (k_fwd_rows) evaluates every expression with its rows k < L_e in
+//     the row buffer as the tape (N contiguous values per row);
+//   pass 2 (k_fwd_transpose) writes the cache one (k, n) row of 64
+//     expressions at a time, 64 * sizeof(T) contiguous bytes, NaN for k >= L_e.
+// k_fwd_plan / k_fwd_scan place the groups in the row buffer.  A group whose
+// rows do not fit in the buffer is evaluated with the cache itself as the
+// tape and skips pass 2, so any workspace size gives the same cache.
+// Workspace: goff int64[G + 1] | eoff int64[E] | Lw uint8[E] | rows T[cap].
+// ---------------------------------------------------------------------------
+constexpr int FG = WAVE;  // expressions per row-buffer group (one per lane)
+constexpr int64_t FWD_ROWS_SHARE = 4;  // recommended row buffer: 1 / FWD_ROWS_SHARE of the cache
+
+struct FwdRows {
+    const int64_t *goff;  // group g's rows start at goff[g] and end at goff[g + 1]
+    const int64_t *eoff;  // expression e's rows start at goff[e / FG] + eoff[e]
+    const uint8_t *Lw;    // rows of expression e (its highest non-NO_OP node + 1)
+    void *rows;
+    int64_t cap;          // elements of rows; -1: no row buffer
+};
+
+inline int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
+inline int64_t fwd_index_bytes(int64_t E) {
+    const int64_t G = (E + FG - 1) / FG;
+    return align256(8 * (G + 1)) + align256(8 * E) + align256(E);
+}
+
+// One wave per group: L of every expression, offsets inside the group, and
+// the group's size in goff[g] (made exclusive offsets by k_fwd_scan).
+__global__ __launch_bounds__(64) void k_fwd_plan(const int64_t *__restrict__ ops, int M, int64_t B, int64_t K,
+                                                 int64_t N, int64_t *__restrict__ goff, int64_t *__restrict__ eoff,
+                                                 uint8_t *__restrict__ Lw) {
+    const int lane = lane_id();
+    const int64_t E = B * K;
+    const int64_t e = (int64_t)blockIdx.x * FG + lane;
+    int L = 0;
+    if (e < E) {
+        const int64_t b = e / K;
+        for (int k = 0; k < M; ++k) L = ops[(int64_t)k * B + b] != NO_OP ? k + 1 : L;
+    }
+    const int64_t size = (int64_t)L * N;
+    int64_t incl = size;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const int64_t up = __shfl_up(incl, d, WAVE);
+        if (lane >= d) incl += up;
+    }
+    if (e < E) {
+        eoff[e] = incl - size;
+        Lw[e] = (uint8_t)L;
+    }
+    if (lane == WAVE - 1) goff[blockIdx.x] = incl;
+}
+
+// One block: goff[0..G) sizes -> exclusive offsets, goff[G] = total.
+__global__ __launch_bounds__(1024) void k_fwd_scan(int64_t *__restrict__ goff, int64_t G) {
+    __shared__ int64_t wsum[16];
+    __shared__ int64_t carry_s;
+    const int t = (int)threadIdx.x, lane = lane_id(), w = t / WAVE;
+    if (t == 0) carry_s = 0;
+    __syncthreads();
+    for (int64_t base = 0; base < G; base += 1024) {
+        const int64_t i = base + t;
+        const int64_t v = i < G ? goff[i] : 0;
+        int64_t incl = v;
+#pragma unroll
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const int64_t up = __shfl_up(incl, d, WAVE);
+            if (lane >= d) incl += up;
+        }
+        if (lane == WAVE - 1) wsum[w] = incl;
+        __syncthreads();
+        int64_t before = carry_s;
+        for (int q = 0; q < w; ++q) before += wsum[q];
+        if (i < G) goff[i] = before + incl - v;
+        __syncthreads();
+        if (t == 1023) carry_s = before + incl;
+        __syncthreads();
+    }
+    if (t == 0) goff[G] = carry_s;
+}
+
+// ---------------------------------------------------------------------------
 // Full-cache forward with coalesced cache writes.  The cache (M,N,B[,K]) has
 // the expressions innermost, so one wave writing one expression's points
 // touches a different cache line per lane.  Here a block of EW waves
@@ -140,6 +226,126 @@ __global__ __launch_bounds__(512) void k_full_forward_grp(const T *__restrict__ 
     const int L = Ls[el];
     const T nan = Mth<T>::qnan();
     for (int k = 0; k < M; ++k) cache[((int64_t)k * N + n) * E + e] = k < L ? src[(size_t)k * WAVE] : nan;
+}
+
+// Pass 1: one wave per (expression e_begin + i, i < e_count, and FR_TILES
+// consecutive 64-point tiles), FR_WAVES per block.  A program of at most FR_LROWS nodes runs on an
+// LDS tape and then writes its rows to the row buffer; a longer one uses its
+// rows in the row buffer as the tape (lanes = points), so the LDS per wave
+// stays small and the pass runs at full occupancy.  A group with no room in the
+// buffer is evaluated with the cache itself as the tape (node k of expression
+// e at point n in its final place, then the NaN padding) and skips pass 2.
+constexpr int FR_WAVES = 4;
+constexpr int FR_LROWS = 16;  // programs of at most this many nodes use an LDS tape
+constexpr int FR_TILES = 4;   // consecutive point tiles per wave (one program load)
+template <typename T>
+__global__ __launch_bounds__(FR_WAVES * 64) void k_fwd_rows(const T *__restrict__ X, int64_t N, int n_x,
+                                                           const int64_t *__restrict__ ops,
+                                                           const int64_t *__restrict__ ch, const T *__restrict__ C,
+                                                           int M, int64_t B, int64_t K, int64_t n_tiles, int64_t e_begin,
+                                                           int64_t e_count, FwdRows fr, T *__restrict__ cache) {
+    __shared__ T ltape[FR_WAVES][FR_LROWS * WAVE];
+    const int wave = uniform_i32((int)(threadIdx.x / WAVE));
+    const int lane = lane_id();
+    const int64_t E = B * K;
+    const int64_t id = (int64_t)blockIdx.x * FR_WAVES + wave;
+    const int64_t e = e_begin + id % e_count, tile0 = id / e_count * FR_TILES;
+    if (tile0 >= n_tiles) return;  // whole wave
+    const int64_t b = e / K, j = e % K, grp = e / FG;
+    const Program<T> p = load_program<T>(ops, ch, C + j, b, 1, B, K, B * K, M, n_x);
+    const bool fits = fr.goff[grp + 1] <= fr.cap;
+    const int L = fits ? (p.L < (int)fr.Lw[e] ? p.L : (int)fr.Lw[e]) : p.L;
+    // the program is loaded once for FR_TILES consecutive tiles
+    for (int64_t tile = tile0; tile < tile0 + FR_TILES && tile < n_tiles; ++tile) {
+        const int64_t n = tile * TILE + lane;
+        if (n >= N) break;  // no wave-wide operation follows
+        const PointCtx<T> pc = load_point<T>(X, n, true, n_x);
+        // Rows are clamped below L: a valid program only names children
+        // below their parent, and an unvalidated one stays in bounds.
+        if (fits) {
+            T *t = reinterpret_cast<T *>(fr.rows) + fr.goff[grp] + fr.eoff[e] + n;
+            if (L <= FR_LROWS) {  // short program: LDS tape, then its rows out
+                T *lt = ltape[wave];
+                forward_tile_at<T>(p, pc, [&](int k) -> T & { return lt[(k < L ? k : L - 1) * WAVE + lane]; }, L);
+                for (int k = 0; k < L; ++k) t[(int64_t)k * N] = lt[k * WAVE + lane];
+            } else {
+                forward_tile_at<T>(p, pc, [&](int k) -> T & { return t[(int64_t)(k < L ? k : L - 1) * N]; }, L);
+            }
+        } else {
+            T *t = cache + n * E + e;
+            const int64_t rs = N * E;
+            forward_tile_at<T>(p, pc, [&](int k) -> T & { return t[(int64_t)(k < L ? k : L - 1) * rs]; }, L);
+            const T nan = Mth<T>::qnan();
+            for (int k = L; k < M; ++k) t[(int64_t)k * rs] = nan;
+        }
+    }
+}
+
+// Pass 2: block (group g_begin + i, i < g_count, and 64-point tile),
+// TR_WAVES waves.  Per node row k:
+// the group's 64 rows (lanes = points) into an LDS tile, then the cache rows
+// (k, n) with lanes = expressions, NaN for k >= L_e.  The loads of row k + 1
+// are issued before row k's stores, so they are in flight while those drain.
+// Rows past every L_e are NaN without the tile.  The tile is 64 x 65 (odd
+// stride: the transposed reads are conflict-free).
+constexpr int TR_WAVES = 8;
+template <typename T>
+__global__ __launch_bounds__(TR_WAVES * 64) void k_fwd_transpose(int M, int64_t N, int64_t E, int64_t g_begin,
+                                                                int64_t g_count, FwdRows fr, T *__restrict__ cache) {
+    __shared__ T tile[FG][WAVE + 1];
+    __shared__ int64_t base_s[FG];
+    __shared__ int L_s[FG];
+    const int wave = uniform_i32((int)(threadIdx.x / WAVE));
+    const int lane = lane_id();
+    const int64_t g = g_begin + (int64_t)blockIdx.x % g_count, tl = (int64_t)blockIdx.x / g_count;
+    if (fr.goff[g + 1] > fr.cap) return;  // written by pass 1
+    const int64_t e = g * FG + lane, n0 = tl * WAVE;
+    if (wave == 0) {
+        L_s[lane] = e < E ? (int)fr.Lw[e] : 0;
+        base_s[lane] = e < E ? fr.goff[g] + fr.eoff[e] : 0;
+    }
+    __syncthreads();
+    int Lmax = 0;
+    for (int q = 0; q < FG; ++q) Lmax = L_s[q] > Lmax ? L_s[q] : Lmax;
+    const T *rows = reinterpret_cast<const T *>(fr.rows);
+    const int64_t gbase = fr.goff[g];  // a valid element whenever Lmax > 0
+    const T nan = Mth<T>::qnan();
+    const int myL = L_s[lane];
+    const int64_t n = n0 + lane;
+    constexpr int QW = FG / TR_WAVES;
+    T v[QW];
+    // expression q's row k over the tile's points (every load at an
+    // in-bounds address of the group; unused values are dropped)
+    auto load_row = [&](int k) {
+#pragma unroll
+        for (int i = 0; i < QW; ++i) {
+            const int q = wave + i * TR_WAVES;
+            const bool ok = k < L_s[q] && n < N;
+            v[i] = rows[ok ? base_s[q] + (int64_t)k * N + n : gbase];
+        }
+    };
+    if (Lmax > 0) load_row(0);
+    for (int k = 0; k < M; ++k) {
+        if (k < Lmax) {
+#pragma unroll
+            for (int i = 0; i < QW; ++i) {
+                const int q = wave + i * TR_WAVES;
+                if (k < L_s[q] && n < N) tile[q][lane] = v[i];
+            }
+            __syncthreads();
+            if (k + 1 < Lmax) load_row(k + 1);
+            for (int pp = wave; pp < WAVE; pp += TR_WAVES) {
+                const int64_t nn = n0 + pp;
+                if (nn < N && e < E) cache[((int64_t)k * N + nn) * E + e] = k < myL ? tile[lane][pp] : nan;
+            }
+            __syncthreads();
+        } else {
+            for (int pp = wave; pp < WAVE; pp += TR_WAVES) {
+                const int64_t nn = n0 + pp;
+                if (nn < N && e < E) cache[((int64_t)k * N + nn) * E + e] = nan;
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -355,12 +561,24 @@ inline int ret_last() { return (int)hipGetLastError(); }
 
 template <typename T>
 int forward_mb(const T *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch, const T *C, int64_t M,
-               int64_t B, int64_t K, T *cache, hipStream_t s) {
-    if (M <= 0 || M > 128 || N < 0 || B < 0 || n_x < 0 || K <= 0) return -1;
+               int64_t B, int64_t K, T *cache, void *ws, int64_t ws_bytes, hipStream_t s) {
+    if (M <= 0 || M > 128 || N < 0 || B < 0 || n_x < 0 || K <= 0 || ws_bytes < 0) return -1;
     if (N == 0 || B == 0) return 0;
     int64_t n_tiles = (N + TILE - 1) / TILE;
     int64_t nblk = n_tiles * B * K;
     if (nblk > 0x7fffffffLL) return -1;
+    // row buffer (two-pass forward) when the workspace holds the index arrays
+    const int64_t E = B * K, G = (E + FG - 1) / FG, ib = fwd_index_bytes(E);
+    FwdRows fr{nullptr, nullptr, nullptr, nullptr, -1};
+    if (ws != nullptr && ws_bytes >= ib + 256 && G * n_tiles <= 0x7fffffffLL) {
+        unsigned char *w = static_cast<unsigned char *>(ws);
+        const int64_t o1 = align256(8 * (G + 1)), o2 = o1 + align256(8 * E);
+        fr.goff = reinterpret_cast<const int64_t *>(w);
+        fr.eoff = reinterpret_cast<const int64_t *>(w + o1);
+        fr.Lw = w + o2;
+        fr.rows = w + ib;
+        fr.cap = (ws_bytes - ib) / (int64_t)sizeof(T);
+    }
     // grouped kernel: the most expressions per block whose tapes fit in LDS
     for (int EW = 8; EW >= 2; EW /= 2) {
         const size_t lds = (size_t)EW * ((size_t)M * WAVE + 1) * sizeof(T);
@@ -368,6 +586,20 @@ int forward_mb(const T *X, int64_t N, int64_t n_x, const int64_t *ops, const int
         const int64_t groups = (B * K + EW - 1) / EW;
         const int64_t nb = n_tiles * groups;
         if (nb > 0x7fffffffLL) break;
+        if (fr.cap >= 0) {
+            hipLaunchKernelGGL(k_fwd_plan, dim3((unsigned)G), dim3(64), 0, s, ops, (int)M, B, K, N,
+                               const_cast<int64_t *>(fr.goff), const_cast<int64_t *>(fr.eoff),
+                               const_cast<uint8_t *>(fr.Lw));
+            hipLaunchKernelGGL(k_fwd_scan, dim3(1), dim3(1024), 0, s, const_cast<int64_t *>(fr.goff), G);
+            // (run one after the other: concurrent chunks of the two passes
+            // slow each other down more than they overlap)
+            const int64_t nrb = (E * ((n_tiles + FR_TILES - 1) / FR_TILES) + FR_WAVES - 1) / FR_WAVES;
+            hipLaunchKernelGGL(k_fwd_rows<T>, dim3((unsigned)nrb), dim3(FR_WAVES * 64), 0, s, X, N, (int)n_x, ops, ch, C,
+                               (int)M, B, K, n_tiles, (int64_t)0, E, fr, cache);
+            hipLaunchKernelGGL(k_fwd_transpose<T>, dim3((unsigned)(G * n_tiles)), dim3(TR_WAVES * 64), 0, s, (int)M, N,
+                               E, (int64_t)0, G, fr, cache);
+            return ret_last();
+        }
         hipLaunchKernelGGL(k_full_forward_grp<T>, dim3((unsigned)nb), dim3(64 * EW), lds, s, X, N, (int)n_x, ops, ch,
                            C, (int)M, B, K, n_tiles, EW, cache);
         return ret_last();
@@ -463,11 +695,28 @@ int sr_validate_mb(const int64_t *ops, const int64_t *ch, int64_t M, int64_t B, 
 
 int sr_forward_mb_f32(const float *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch, const float *C,
                       int64_t M, int64_t B, float *cache, void *stream) {
-    return forward_mb<float>(X, N, n_x, ops, ch, C, M, B, 1, cache, (hipStream_t)stream);
+    return forward_mb<float>(X, N, n_x, ops, ch, C, M, B, 1, cache, nullptr, 0, (hipStream_t)stream);
 }
 int sr_forward_mb_f64(const double *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
                       const double *C, int64_t M, int64_t B, double *cache, void *stream) {
-    return forward_mb<double>(X, N, n_x, ops, ch, C, M, B, 1, cache, (hipStream_t)stream);
+    return forward_mb<double>(X, N, n_x, ops, ch, C, M, B, 1, cache, nullptr, 0, (hipStream_t)stream);
+}
+
+int64_t sr_forward_mb_workspace_bytes(int64_t M, int64_t N, int64_t B, int64_t K, int is_f64) {
+    if (M <= 0 || N <= 0 || B <= 0 || K <= 0) return 0;
+    const int64_t E = B * K;
+    const int64_t rows = (M * N * E + FWD_ROWS_SHARE - 1) / FWD_ROWS_SHARE;  // a quarter of the cache
+    return fwd_index_bytes(E) + align256(rows * (is_f64 ? 8 : 4));
+}
+int sr_forward_mb_ws_f32(const float *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                         const float *C, int64_t M, int64_t B, float *cache, void *workspace, int64_t ws_bytes,
+                         void *stream) {
+    return forward_mb<float>(X, N, n_x, ops, ch, C, M, B, 1, cache, workspace, ws_bytes, (hipStream_t)stream);
+}
+int sr_forward_mb_ws_f64(const double *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                         const double *C, int64_t M, int64_t B, double *cache, void *workspace, int64_t ws_bytes,
+                         void *stream) {
+    return forward_mb<double>(X, N, n_x, ops, ch, C, M, B, 1, cache, workspace, ws_bytes, (hipStream_t)stream);
 }
 
 int64_t sr_backward_mb_workspace_bytes(int64_t B, int64_t N, int64_t n_x, int is_f64) {
@@ -489,11 +738,21 @@ int sr_backward_mb_f64(double *grad_out, const double *cache, const int64_t *ops
 
 int sr_multi_forward_mb_f32(const float *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
                             const float *C, int64_t M, int64_t B, int64_t K, float *cache, void *stream) {
-    return forward_mb<float>(X, N, n_x, ops, ch, C, M, B, K, cache, (hipStream_t)stream);
+    return forward_mb<float>(X, N, n_x, ops, ch, C, M, B, K, cache, nullptr, 0, (hipStream_t)stream);
 }
 int sr_multi_forward_mb_f64(const double *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
                             const double *C, int64_t M, int64_t B, int64_t K, double *cache, void *stream) {
-    return forward_mb<double>(X, N, n_x, ops, ch, C, M, B, K, cache, (hipStream_t)stream);
+    return forward_mb<double>(X, N, n_x, ops, ch, C, M, B, K, cache, nullptr, 0, (hipStream_t)stream);
+}
+int sr_multi_forward_mb_ws_f32(const float *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                               const float *C, int64_t M, int64_t B, int64_t K, float *cache, void *workspace,
+                               int64_t ws_bytes, void *stream) {
+    return forward_mb<float>(X, N, n_x, ops, ch, C, M, B, K, cache, workspace, ws_bytes, (hipStream_t)stream);
+}
+int sr_multi_forward_mb_ws_f64(const double *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                               const double *C, int64_t M, int64_t B, int64_t K, double *cache, void *workspace,
+                               int64_t ws_bytes, void *stream) {
+    return forward_mb<double>(X, N, n_x, ops, ch, C, M, B, K, cache, workspace, ws_bytes, (hipStream_t)stream);
 }
 int64_t sr_multi_backward_mb_workspace_bytes(int64_t B, int64_t N, int64_t n_x, int64_t K, int is_f64) {
     return is_f64 ? backward_mb_ws_bytes<double>(B, N, n_x, K) : backward_mb_ws_bytes<float>(B, N, n_x, K);

@@ -224,9 +224,10 @@ __device__ inline PointCtx<T> load_point(const T *__restrict__ X, int64_t n, boo
 }
 
 // Forward sweep of one tile over nodes [0, L): tape[k*64 + lane].
-template <typename T>
-__device__ inline void forward_tile(const Program<T> &p, const PointCtx<T> &pc, T *tape, int L) {
-    const int lane = lane_id();
+// Forward sweep over nodes [0, L) on any tape: at(k) is this lane's element
+// of node k's row (an LDS tape, or rows in global memory).
+template <typename T, typename At>
+__device__ inline void forward_tile_at(const Program<T> &p, const PointCtx<T> &pc, At at, int L) {
     for (int k = 0; k < L; ++k) {
         uint32_t w = p.word(k);
         int op = node_op(w);
@@ -241,12 +242,18 @@ __device__ inline void forward_tile(const Program<T> &p, const PointCtx<T> &pc, 
             r = pc.var(op - VAR_START_ID);
         } else {
             int ar = arity_of(op);
-            T a0 = ar >= 1 ? tape[node_c0(w) * WAVE + lane] : T(0);
-            T a1 = ar == 2 ? tape[node_c1(w) * WAVE + lane] : T(0);
+            T a0 = ar >= 1 ? at(node_c0(w)) : T(0);
+            T a1 = ar == 2 ? at(node_c1(w)) : T(0);
             r = fwd_node<T>(op, a0, a1);
         }
-        tape[k * WAVE + lane] = r;
+        at(k) = r;
     }
+}
+
+template <typename T>
+__device__ inline void forward_tile(const Program<T> &p, const PointCtx<T> &pc, T *tape, int L) {
+    const int lane = lane_id();
+    forward_tile_at<T>(p, pc, [&](int k) -> T & { return tape[k * WAVE + lane]; }, L);
 }
 
 // Per-lane accumulators of one expression over all of its tiles.

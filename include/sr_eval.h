@@ -51,6 +51,21 @@ int sr_forward_mb_f32(const float *X, int64_t N, int64_t n_x, const int64_t *ops
 int sr_forward_mb_f64(const double *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
                       const double *C, int64_t M, int64_t B, double *cache, void *stream);
 
+/* The same forward with a workspace (same results, same replaced reference
+ * interface): the nodes of each group of 64 consecutive expressions go
+ * through a row buffer in the workspace, and the cache is written 64
+ * expressions at a time (whole 256-byte runs in FP32).  Groups whose rows do
+ * not fit are written as without a workspace, so any size is valid; the
+ * recommended size holds rows for a quarter of the cache plus the index
+ * arrays (K = 1 for the single-constant-set entry points). */
+int64_t sr_forward_mb_workspace_bytes(int64_t M, int64_t N, int64_t B, int64_t K, int is_f64);
+int sr_forward_mb_ws_f32(const float *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                         const float *C, int64_t M, int64_t B, float *cache, void *workspace, int64_t ws_bytes,
+                         void *stream);
+int sr_forward_mb_ws_f64(const double *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                         const double *C, int64_t M, int64_t B, double *cache, void *workspace, int64_t ws_bytes,
+                         void *stream);
+
 /* Full-cache backward: grad_cache (M,N,B) holds the upstream gradient on
  * entry; its contents on exit are unspecified (scratch: for M <= 64 in FP32
  * it is only read, the adjoints live in LDS); writes grad_C (M,B) and grad_X
@@ -77,6 +92,14 @@ int sr_multi_forward_mb_f32(const float *X, int64_t N, int64_t n_x, const int64_
                             const float *C, int64_t M, int64_t B, int64_t K, float *cache, void *stream);
 int sr_multi_forward_mb_f64(const double *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
                             const double *C, int64_t M, int64_t B, int64_t K, double *cache, void *stream);
+/* With a workspace of sr_forward_mb_workspace_bytes(M, N, B, K, is_f64) bytes
+ * (or any other size), as sr_forward_mb_ws_f32. */
+int sr_multi_forward_mb_ws_f32(const float *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                               const float *C, int64_t M, int64_t B, int64_t K, float *cache, void *workspace,
+                               int64_t ws_bytes, void *stream);
+int sr_multi_forward_mb_ws_f64(const double *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                               const double *C, int64_t M, int64_t B, int64_t K, double *cache, void *workspace,
+                               int64_t ws_bytes, void *stream);
 int64_t sr_multi_backward_mb_workspace_bytes(int64_t B, int64_t N, int64_t n_x, int64_t K, int is_f64);
 int sr_multi_backward_mb_f32(float *grad_cache, const float *cache, const int64_t *ops, const int64_t *ch,
                              int64_t M, int64_t B, int64_t N, int64_t n_x, int64_t K, float *grad_C, float *grad_X,
