@@ -174,11 +174,17 @@ def bench_full_cache(dev, steps: int, seed: int = 1, B: int = 65536, N: int = 10
     gup[root, :, torch.arange(B, device=dev)] = torch.randn(B, N, generator=gen, device=dev)
     gC, gX = torch.empty_like(C_mb), torch.empty_like(X)
     ws = torch.empty(max(int(lib.sr_backward_mb_workspace_bytes(B, N, 3, 0)), 256), dtype=torch.uint8, device=dev)
+    rws = torch.empty(int(lib.sr_backward_mb_rows_workspace_bytes(M, N, B, 1, 0)), dtype=torch.uint8, device=dev)
     err = torch.empty(1, dtype=torch.int32, device=dev)
-    bwd = lambda: _lib.check_status(lib.sr_backward_mb_f32(_lib.ptr(gup), _lib.ptr(cache), _lib.ptr(ops_mb),  # noqa
-                                                           _lib.ptr(ch_mb), M, B, N, 3, _lib.ptr(gC), _lib.ptr(gX),
-                                                           _lib.ptr(ws), ws.numel(), _lib.ptr(err), s), "backward")
-    ms_b, _ = timed(bwd, steps, 1)
+    bwd_direct = lambda: _lib.check_status(lib.sr_backward_mb_f32(  # noqa: E731
+        _lib.ptr(gup), _lib.ptr(cache), _lib.ptr(ops_mb), _lib.ptr(ch_mb), M, B, N, 3, _lib.ptr(gC), _lib.ptr(gX),
+        _lib.ptr(ws), ws.numel(), _lib.ptr(err), s), "backward")
+    bwd = lambda: _lib.check_status(lib.sr_backward_mb_rows_f32(  # noqa: E731
+        _lib.ptr(gup), _lib.ptr(cache), _lib.ptr(ops_mb), _lib.ptr(ch_mb), M, B, N, 3, _lib.ptr(gC), _lib.ptr(gX),
+        _lib.ptr(ws), ws.numel(), _lib.ptr(rws), rws.numel(), _lib.ptr(err), s), "backward")
+    ms_bd, _ = timed(bwd_direct, steps, 1)
+    ms_b, _ = timed(bwd, steps, 1)  # the API's path
+    del rws
     L = float((ops != 0).sum(1).double().mean())
     fb = M * N * B * 4
     bb = M * N * B * 4 + int(L * N * B * 4)
@@ -192,8 +198,8 @@ def bench_full_cache(dev, steps: int, seed: int = 1, B: int = 65536, N: int = 10
                              ms_without_workspace=round(ms_fd, 3)),
                 backward=dict(ms=round(ms_b, 3), bytes=bb, achieved_GBps=gbs(bb, ms_b),
                               frac=round(gbs(bb, ms_b) / HBM_PEAK_GBS, 4),
-                              note="the reverse sweep works on a scratch copy of the upstream gradient in the "
-                                   "API; here the kernel reads it directly (it is read-only)"))
+                              note="gathered rows, then the reverse sweep (sr_backward_mb_rows_f32, the API's path)",
+                              ms_without_row_workspace=round(ms_bd, 3)))
 
 
 def bench_ragged(dev, steps: int, seed: int = 3, B: int = 32768):

@@ -199,7 +199,8 @@ class _SymbolicEvaluation(torch.autograd.Function):
         # size uses it as scratch (the grouped kernel only reads it).
         lib0 = _lib.hip()
         g = grad_output.to(dtype=cache.dtype).contiguous()
-        if not lib0.sr_backward_mb_reads_only(Ops.shape[0], int(X.dtype == torch.float64)):
+        grouped = bool(lib0.sr_backward_mb_reads_only(Ops.shape[0], int(X.dtype == torch.float64)))
+        if not grouped:
             g = g.clone()
         grad_C = torch.empty_like(C)
         want_x = ctx.needs_input_grad[0]
@@ -207,14 +208,18 @@ class _SymbolicEvaluation(torch.autograd.Function):
         is64 = X.dtype == torch.float64
         lib = _lib.hip()
         ws = _workspace(lib.sr_backward_mb_workspace_bytes(B, N, n_x, int(is64)) if want_x else 0, X.device)
+        # gathered gradient and value rows (read by the kernel that only reads g)
+        rws = _workspace(lib.sr_backward_mb_rows_workspace_bytes(M, N, B, 1, int(is64)) if grouped else 0, X.device)
         err = torch.empty(1, dtype=torch.int32, device=X.device)
         _lib.check_status(
-            getattr(lib, f"sr_backward_mb_{'f64' if is64 else 'f32'}")(
+            getattr(lib, f"sr_backward_mb_rows_{'f64' if is64 else 'f32'}")(
                 _lib.ptr(g), _lib.ptr(cache), _lib.ptr(Ops), _lib.ptr(Ch), M, B, N, n_x, _lib.ptr(grad_C),
-                _lib.ptr(grad_X), _lib.ptr(ws), ws.numel(), _lib.ptr(err), _lib.stream_of(X.device),
+                _lib.ptr(grad_X), _lib.ptr(ws), ws.numel(), _lib.ptr(rws), rws.numel() if grouped else 0,
+                _lib.ptr(err), _lib.stream_of(X.device),
             ),
             "symbolic backward",
         )
+        del rws
         code = int(err.item())
         if code != 0:
             raise RuntimeError(get_error_message(code, "symbolic evaluation backward pass"))
@@ -308,7 +313,8 @@ class _SymbolicEvaluationMultiple(torch.autograd.Function):
         M, B, K = C.shape
         N, n_x = X.shape
         g = grad_output.to(dtype=cache.dtype).contiguous()
-        if not _lib.hip().sr_backward_mb_reads_only(M, int(X.dtype == torch.float64)):
+        grouped = bool(_lib.hip().sr_backward_mb_reads_only(M, int(X.dtype == torch.float64)))
+        if not grouped:
             g = g.clone()  # the kernel of this size uses it as scratch
         grad_C = torch.empty_like(C)
         want_x = ctx.needs_input_grad[0]
@@ -317,14 +323,17 @@ class _SymbolicEvaluationMultiple(torch.autograd.Function):
         lib = _lib.hip()
         ws = _workspace(lib.sr_multi_backward_mb_workspace_bytes(B, N, n_x, K, int(is64)) if want_x else 0,
                         X.device)
+        rws = _workspace(lib.sr_backward_mb_rows_workspace_bytes(M, N, B, K, int(is64)) if grouped else 0, X.device)
         err = torch.empty(1, dtype=torch.int32, device=X.device)
         _lib.check_status(
-            getattr(lib, f"sr_multi_backward_mb_{'f64' if is64 else 'f32'}")(
+            getattr(lib, f"sr_multi_backward_mb_rows_{'f64' if is64 else 'f32'}")(
                 _lib.ptr(g), _lib.ptr(cache), _lib.ptr(Ops), _lib.ptr(Ch), M, B, N, n_x, K, _lib.ptr(grad_C),
-                _lib.ptr(grad_X), _lib.ptr(ws), ws.numel(), _lib.ptr(err), _lib.stream_of(X.device),
+                _lib.ptr(grad_X), _lib.ptr(ws), ws.numel(), _lib.ptr(rws), rws.numel() if grouped else 0,
+                _lib.ptr(err), _lib.stream_of(X.device),
             ),
             "symbolic multiple backward",
         )
+        del rws
         code = int(err.item())
         if code != 0:
             raise RuntimeError(get_error_message(code, "symbolic evaluation (multiple constants) backward pass"))

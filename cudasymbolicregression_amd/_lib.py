@@ -62,6 +62,9 @@ def hip() -> ctypes.CDLL:
             _declare(lib, f"sr_forward_mb_{sfx}", I, [P, I64, I64, P, P, P, I64, I64, P, P])
             _declare(lib, f"sr_forward_mb_ws_{sfx}", I, [P, I64, I64, P, P, P, I64, I64, P, P, I64, P])
             _declare(lib, f"sr_multi_forward_mb_ws_{sfx}", I, [P, I64, I64, P, P, P, I64, I64, I64, P, P, I64, P])
+            _declare(lib, f"sr_backward_mb_rows_{sfx}", I, [P, P, P, P, I64, I64, I64, I64, P, P, P, I64, P, I64, P, P])
+            _declare(lib, f"sr_multi_backward_mb_rows_{sfx}", I,
+                     [P, P, P, P, I64, I64, I64, I64, I64, P, P, P, I64, P, I64, P, P])
             _declare(lib, f"sr_backward_mb_{sfx}", I, [P, P, P, P, I64, I64, I64, I64, P, P, P, I64, P, P])
             _declare(lib, f"sr_fused_root_{sfx}", I,
                      [P, I64, I64, P, P, P, I64, I64, P, I, I, P, P, P, P, P, P, P, I64, P])
@@ -71,6 +74,7 @@ def hip() -> ctypes.CDLL:
             _declare(lib, f"sr_multi_backward_mb_{sfx}", I,
                      [P, P, P, P, I64, I64, I64, I64, I64, P, P, P, I64, P, P])
         _declare(lib, "sr_forward_mb_workspace_bytes", I64, [I64, I64, I64, I64, I])
+        _declare(lib, "sr_backward_mb_rows_workspace_bytes", I64, [I64, I64, I64, I64, I])
         _declare(lib, "sr_backward_mb_workspace_bytes", I64, [I64, I64, I64, I])
         _declare(lib, "sr_backward_mb_reads_only", I, [I64, I])
         _declare(lib, "sr_multi_backward_mb_workspace_bytes", I64, [I64, I64, I64, I64, I])

@@ -433,28 +433,111 @@ __global__ __launch_bounds__(BWD_WAVES * 64) void k_full_backward(
 // as in k_full_backward.
 constexpr int BWD_BATCH = 16;
 
+// Backward gather (with a row-buffer workspace).  The grouped backward
+// below reads the upstream gradient and the cached values per (node, point)
+// with the expressions of its block innermost: BWD_WAVES * sizeof(T)
+// contiguous bytes.  Here one block per (group g of 64 expressions, 64-point
+// tile) reads every (k, n) row of the group 64 expressions at a time, keeps
+// rows k < L_e in LDS tiles, and writes them to per-expression gradient rows
+// (fr.rows) and value rows (vrows), N contiguous values per row, which the
+// grouped backward then reads with lanes = points.  An upstream gradient on a
+// NO_OP padding row (k >= L_e) that is not zero is reported as
+// ERR_GRAD_NO_OP, as the grouped backward does for the groups it reads itself
+// (those without room in the buffer).
+constexpr int GA_WAVES = 8;
 template <typename T>
+__global__ __launch_bounds__(GA_WAVES * 64) void k_bwd_gather(const T *__restrict__ grad_cache,
+                                                             const T *__restrict__ cache, int M, int64_t N, int64_t E,
+                                                             int64_t G, FwdRows fr, T *__restrict__ vrows,
+                                                             int32_t *err_flag) {
+    __shared__ T gtile[FG][WAVE + 1];
+    __shared__ T vtile[FG][WAVE + 1];
+    __shared__ int64_t base_s[FG];
+    __shared__ int L_s[FG];
+    const int wave = uniform_i32((int)(threadIdx.x / WAVE));
+    const int lane = lane_id();
+    const int64_t g = (int64_t)blockIdx.x % G, tl = (int64_t)blockIdx.x / G;
+    if (fr.goff[g + 1] > fr.cap) return;  // read by the grouped backward itself
+    const int64_t e = g * FG + lane, n0 = tl * WAVE;
+    if (wave == 0) {
+        L_s[lane] = e < E ? (int)fr.Lw[e] : 0;
+        base_s[lane] = e < E ? fr.goff[g] + fr.eoff[e] : 0;
+    }
+    __syncthreads();
+    int Lmax = 0;
+    for (int q = 0; q < FG; ++q) Lmax = L_s[q] > Lmax ? L_s[q] : Lmax;
+    const int myL = L_s[lane];
+    T *grows = reinterpret_cast<T *>(fr.rows);
+    constexpr int PW = WAVE / GA_WAVES;  // points per wave
+    int32_t lerr = 0;
+    for (int k = 0; k < M; ++k) {
+        // row k at the wave's points, lanes = expressions (all loads issued
+        // before the first use)
+        T gv[PW], vv[PW];
+#pragma unroll
+        for (int i = 0; i < PW; ++i) {
+            const int64_t n = n0 + wave + i * GA_WAVES;
+            const bool ok = n < N && e < E;
+            const int64_t off = ok ? ((int64_t)k * N + n) * E + e : 0;
+            gv[i] = ok ? grad_cache[off] : T(0);
+            vv[i] = (ok && k < myL) ? cache[off] : T(0);
+        }
+#pragma unroll
+        for (int i = 0; i < PW; ++i) {
+            const int pp = wave + i * GA_WAVES;
+            if (k < myL) {
+                gtile[lane][pp] = gv[i];
+                vtile[lane][pp] = vv[i];
+            } else if (gv[i] != T(0)) {
+                lerr = ERR_GRAD_NO_OP;
+            }
+        }
+        if (k < Lmax) {
+            __syncthreads();
+            const int64_t n = n0 + lane;
+            for (int q = wave; q < FG; q += GA_WAVES) {
+                if (k < L_s[q] && n < N) {
+                    const int64_t o = base_s[q] + (int64_t)k * N + n;
+                    grows[o] = gtile[q][lane];
+                    vrows[o] = vtile[q][lane];
+                }
+            }
+            __syncthreads();
+        }
+    }
+    lerr = wave_max_i32(lerr);
+    if (lane == 0 && lerr) atomicMax(err_flag, lerr);
+}
+
+// ROWS: the blocks whose group was gathered (fr), with the gathered value
+// and gradient rows as the tapes (the adjoints accumulate in place in the
+// gradient rows, a copy), so the tapes take no LDS; otherwise the other
+// blocks, with LDS tapes loaded from the cache layout.
+template <typename T, bool ROWS>
 __global__ __launch_bounds__(BWD_WAVES * 64) void k_full_backward_grp(
     const T *__restrict__ grad_cache, const T *__restrict__ cache, const int64_t *__restrict__ ops,
     const int64_t *__restrict__ ch, int M, int64_t B, int64_t K, int64_t N, int n_x, T *__restrict__ grad_C,
-    T *__restrict__ gx_partial, T *__restrict__ grad_X_direct, int32_t *err_flag) {
+    T *__restrict__ gx_partial, T *__restrict__ grad_X_direct, int32_t *err_flag, FwdRows fr,
+    const T *__restrict__ vrows) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int Ls[BWD_WAVES];
     const int wave = uniform_i32((int)(threadIdx.x / WAVE));
     const int lane = lane_id();
     const size_t tstride = (size_t)M * WAVE + 1;
+    const size_t tb = ROWS ? 0 : (size_t)BWD_WAVES * tstride;  // elements of one block's tapes
     T *vals = reinterpret_cast<T *>(smem);
-    T *adjs = vals + (size_t)BWD_WAVES * tstride;
-    double *cextra = reinterpret_cast<double *>(adjs + (size_t)BWD_WAVES * tstride +
-                                                 ((size_t)BWD_WAVES * tstride * sizeof(T) % 8 ? 1 : 0)) +
-                     (size_t)wave * M;
-    T *gxs = reinterpret_cast<T *>(reinterpret_cast<double *>(adjs + (size_t)BWD_WAVES * tstride +
-                                                              ((size_t)BWD_WAVES * tstride * sizeof(T) % 8 ? 1 : 0)) +
+    T *adjs = vals + tb;
+    double *cextra = reinterpret_cast<double *>(adjs + tb + (tb * sizeof(T) % 8 ? 1 : 0)) + (size_t)wave * M;
+    T *gxs = reinterpret_cast<T *>(reinterpret_cast<double *>(adjs + tb + (tb * sizeof(T) % 8 ? 1 : 0)) +
                                    (size_t)BWD_WAVES * M);  // [4][64][NXREG]
     const int64_t E = B * K;
     // XCD-remapped so that neighbouring expression groups share one XCD's L2
     const int64_t blk = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
     const int64_t e0 = blk * BWD_WAVES;
+    // the block's expressions lie in one row-buffer group (BWD_WAVES divides FG)
+    const int64_t grp = e0 / FG;
+    const bool from_rows = fr.cap >= 0 && fr.goff[grp + 1] <= fr.cap;  // uniform over the block
+    if (from_rows != ROWS) return;  // the other launch handles this block
     const int64_t e_id = e0 + wave;
     const bool bvalid = e_id < E;
     const int64_t b = bvalid ? e_id / K : 0, j = bvalid ? e_id % K : 0;
@@ -474,9 +557,12 @@ __global__ __launch_bounds__(BWD_WAVES * 64) void k_full_backward_grp(
     const int el = t % BWD_WAVES, nl = t / BWD_WAVES;  // nl < 64
     const int64_t el_e = e0 + el;
     const int64_t n_tiles = (N + TILE - 1) / TILE;
+    const int64_t rbase = (from_rows && bvalid) ? fr.goff[grp] + fr.eoff[e_id] : 0;
+    const int Lr = (from_rows && bvalid) ? (p.L < (int)fr.Lw[e_id] ? p.L : (int)fr.Lw[e_id]) : 0;
     __syncthreads();
     const int Lel = Ls[el];
     for (int64_t tt = 0; tt < n_tiles; ++tt) {
+        if constexpr (!ROWS) {
         // cooperative load, expression index fastest
         {
             const int64_t n = tt * TILE + nl;
@@ -505,16 +591,29 @@ __global__ __launch_bounds__(BWD_WAVES * 64) void k_full_backward_grp(
                 }
             }
         }
+        }
         __syncthreads();
         const int64_t n = tt * TILE + lane;
         const bool active = bvalid && n < N;
         const int64_t nn = active ? n : 0;
 #pragma unroll
         for (int q = 0; q < NXREG; ++q) acc.gxl[q] = T(0);
-        auto val = [&](int k) -> T { return vt[(size_t)k * WAVE + lane]; };
-        auto adj = [&](int k) -> T & { return at[(size_t)k * WAVE + lane]; };
-        int32_t e = backward_tile<T, true>(p, val, adj, p.L, active, acc, cextra, n_x,
-                                           grad_X_direct ? grad_X_direct + nn * n_x : nullptr);
+        int32_t e;
+        if constexpr (ROWS) {
+            // rows clamped below Lr (a valid program only names children
+            // below their parent; an unvalidated one stays in bounds)
+            T *gr = reinterpret_cast<T *>(fr.rows) + rbase + nn;
+            const T *vr = vrows + rbase + nn;
+            auto val = [&](int k) -> T { return vr[(int64_t)(k < Lr ? k : Lr - 1) * N]; };
+            auto adj = [&](int k) -> T & { return gr[(int64_t)(k < Lr ? k : Lr - 1) * N]; };
+            e = backward_tile<T, true>(p, val, adj, Lr, active, acc, cextra, n_x,
+                                       grad_X_direct ? grad_X_direct + nn * n_x : nullptr);
+        } else {
+            auto val = [&](int k) -> T { return vt[(size_t)k * WAVE + lane]; };
+            auto adj = [&](int k) -> T & { return at[(size_t)k * WAVE + lane]; };
+            e = backward_tile<T, true>(p, val, adj, p.L, active, acc, cextra, n_x,
+                                       grad_X_direct ? grad_X_direct + nn * n_x : nullptr);
+        }
         err = e > err ? e : err;
         if (want_gx && n_x <= NXREG) {
 #pragma unroll
@@ -541,8 +640,8 @@ __global__ __launch_bounds__(BWD_WAVES * 64) void k_full_backward_grp(
 }
 
 template <typename T>
-inline size_t backward_grp_lds(int64_t M) {
-    const size_t tapes = 2 * (size_t)BWD_WAVES * ((size_t)M * WAVE + 1) * sizeof(T);
+inline size_t backward_grp_lds(int64_t M, bool rows = false) {
+    const size_t tapes = rows ? 0 : 2 * (size_t)BWD_WAVES * ((size_t)M * WAVE + 1) * sizeof(T);
     return (tapes + 8) + (size_t)BWD_WAVES * M * sizeof(double) + (size_t)BWD_WAVES * WAVE * NXREG * sizeof(T);
 }
 
@@ -620,8 +719,8 @@ int64_t backward_mb_ws_bytes(int64_t B, int64_t N, int64_t n_x, int64_t K = 1) {
 template <typename T>
 int backward_mb(T *grad_out, const T *cache, const int64_t *ops, const int64_t *ch, int64_t M, int64_t B,
                 int64_t K, int64_t N, int64_t n_x, T *grad_C, T *grad_X, void *workspace, int64_t ws_bytes,
-                int32_t *d_err, hipStream_t s) {
-    if (M <= 0 || M > 128 || N < 0 || B < 0 || n_x < 0 || K <= 0) return -1;
+                int32_t *d_err, hipStream_t s, void *rows_ws = nullptr, int64_t rows_bytes = 0) {
+    if (M <= 0 || M > 128 || N < 0 || B < 0 || n_x < 0 || K <= 0 || rows_bytes < 0) return -1;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, d_err, 0, (int64_t)1);
     int64_t nblk = (B * K + BWD_WAVES - 1) / BWD_WAVES;
     const bool gx_regs = n_x <= NXREG;
@@ -654,9 +753,38 @@ int backward_mb(T *grad_out, const T *cache, const int64_t *ops, const int64_t *
         gx_partial = (T *)workspace;
     }
     if (backward_grp_lds<T>(M) <= lds_limit_device()) {
-        hipLaunchKernelGGL(k_full_backward_grp<T>, dim3((unsigned)nblk), dim3(BWD_WAVES * 64), backward_grp_lds<T>(M),
-                           s, grad_out, cache, ops, ch, (int)M, B, K, N, (int)n_x, grad_C, gx_partial,
-                           (grad_X && !gx_regs) ? grad_X : nullptr, d_err);
+        // gradient and value rows of each group through the row workspace
+        // (half each), when it holds the index arrays
+        const int64_t E = B * K, G = (E + FG - 1) / FG, ib = fwd_index_bytes(E);
+        const int64_t n_tiles = (N + TILE - 1) / TILE;
+        FwdRows fr{nullptr, nullptr, nullptr, nullptr, -1};
+        T *vrows = nullptr;
+        if (rows_ws != nullptr && rows_bytes >= ib + 512 && G * n_tiles <= 0x7fffffffLL) {
+            unsigned char *w = static_cast<unsigned char *>(rows_ws);
+            const int64_t o1 = align256(8 * (G + 1)), o2 = o1 + align256(8 * E);
+            const int64_t half = (rows_bytes - ib) / 2 / 256 * 256;
+            fr.goff = reinterpret_cast<const int64_t *>(w);
+            fr.eoff = reinterpret_cast<const int64_t *>(w + o1);
+            fr.Lw = w + o2;
+            fr.rows = w + ib;
+            fr.cap = half / (int64_t)sizeof(T);
+            vrows = reinterpret_cast<T *>(w + ib + half);
+            hipLaunchKernelGGL(k_fwd_plan, dim3((unsigned)G), dim3(64), 0, s, ops, (int)M, B, K, N,
+                               const_cast<int64_t *>(fr.goff), const_cast<int64_t *>(fr.eoff),
+                               const_cast<uint8_t *>(fr.Lw));
+            hipLaunchKernelGGL(k_fwd_scan, dim3(1), dim3(1024), 0, s, const_cast<int64_t *>(fr.goff), G);
+            hipLaunchKernelGGL(k_bwd_gather<T>, dim3((unsigned)(G * n_tiles)), dim3(GA_WAVES * 64), 0, s, grad_out,
+                               cache, (int)M, N, E, G, fr, vrows, d_err);
+        }
+        if (fr.cap >= 0)  // the gathered groups
+            hipLaunchKernelGGL((k_full_backward_grp<T, true>), dim3((unsigned)nblk), dim3(BWD_WAVES * 64),
+                               backward_grp_lds<T>(M, true), s, grad_out, cache, ops, ch, (int)M, B, K, N, (int)n_x,
+                               grad_C, gx_partial, (grad_X && !gx_regs) ? grad_X : nullptr, d_err, fr,
+                               (const T *)vrows);
+        // the others (all of them without a row workspace)
+        hipLaunchKernelGGL((k_full_backward_grp<T, false>), dim3((unsigned)nblk), dim3(BWD_WAVES * 64),
+                           backward_grp_lds<T>(M), s, grad_out, cache, ops, ch, (int)M, B, K, N, (int)n_x, grad_C,
+                           gx_partial, (grad_X && !gx_regs) ? grad_X : nullptr, d_err, fr, (const T *)vrows);
     } else {
         size_t lds = (size_t)BWD_WAVES * M * sizeof(double) + (size_t)BWD_WAVES * WAVE * NXREG * sizeof(T);
         if (lds > lds_limit()) return -3;
@@ -736,6 +864,25 @@ int sr_backward_mb_f64(double *grad_out, const double *cache, const int64_t *ops
                                (hipStream_t)stream);
 }
 
+int64_t sr_backward_mb_rows_workspace_bytes(int64_t M, int64_t N, int64_t B, int64_t K, int is_f64) {
+    if (M <= 0 || N <= 0 || B <= 0 || K <= 0) return 0;
+    const int64_t E = B * K;
+    const int64_t rows = (M * N * E + FWD_ROWS_SHARE - 1) / FWD_ROWS_SHARE;  // a quarter of the cache, twice
+    return fwd_index_bytes(E) + 2 * align256(rows * (is_f64 ? 8 : 4));
+}
+int sr_backward_mb_rows_f32(float *grad_out, const float *cache, const int64_t *ops, const int64_t *ch, int64_t M,
+                            int64_t B, int64_t N, int64_t n_x, float *grad_C, float *grad_X, void *workspace,
+                            int64_t ws_bytes, void *rows_ws, int64_t rows_bytes, int32_t *d_err, void *stream) {
+    return backward_mb<float>(grad_out, cache, ops, ch, M, B, 1, N, n_x, grad_C, grad_X, workspace, ws_bytes, d_err,
+                              (hipStream_t)stream, rows_ws, rows_bytes);
+}
+int sr_backward_mb_rows_f64(double *grad_out, const double *cache, const int64_t *ops, const int64_t *ch, int64_t M,
+                            int64_t B, int64_t N, int64_t n_x, double *grad_C, double *grad_X, void *workspace,
+                            int64_t ws_bytes, void *rows_ws, int64_t rows_bytes, int32_t *d_err, void *stream) {
+    return backward_mb<double>(grad_out, cache, ops, ch, M, B, 1, N, n_x, grad_C, grad_X, workspace, ws_bytes, d_err,
+                               (hipStream_t)stream, rows_ws, rows_bytes);
+}
+
 int sr_multi_forward_mb_f32(const float *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
                             const float *C, int64_t M, int64_t B, int64_t K, float *cache, void *stream) {
     return forward_mb<float>(X, N, n_x, ops, ch, C, M, B, K, cache, nullptr, 0, (hipStream_t)stream);
@@ -768,6 +915,20 @@ int sr_multi_backward_mb_f64(double *grad_out, const double *cache, const int64_
                              void *workspace, int64_t ws_bytes, int32_t *d_err, void *stream) {
     return backward_mb<double>(grad_out, cache, ops, ch, M, B, K, N, n_x, grad_C, grad_X, workspace, ws_bytes, d_err,
                                (hipStream_t)stream);
+}
+int sr_multi_backward_mb_rows_f32(float *grad_out, const float *cache, const int64_t *ops, const int64_t *ch,
+                                  int64_t M, int64_t B, int64_t N, int64_t n_x, int64_t K, float *grad_C,
+                                  float *grad_X, void *workspace, int64_t ws_bytes, void *rows_ws, int64_t rows_bytes,
+                                  int32_t *d_err, void *stream) {
+    return backward_mb<float>(grad_out, cache, ops, ch, M, B, K, N, n_x, grad_C, grad_X, workspace, ws_bytes, d_err,
+                              (hipStream_t)stream, rows_ws, rows_bytes);
+}
+int sr_multi_backward_mb_rows_f64(double *grad_out, const double *cache, const int64_t *ops, const int64_t *ch,
+                                  int64_t M, int64_t B, int64_t N, int64_t n_x, int64_t K, double *grad_C,
+                                  double *grad_X, void *workspace, int64_t ws_bytes, void *rows_ws,
+                                  int64_t rows_bytes, int32_t *d_err, void *stream) {
+    return backward_mb<double>(grad_out, cache, ops, ch, M, B, K, N, n_x, grad_C, grad_X, workspace, ws_bytes, d_err,
+                               (hipStream_t)stream, rows_ws, rows_bytes);
 }
 
 }  // extern "C"

@@ -85,6 +85,23 @@ int sr_backward_mb_f64(double *grad_cache, const double *cache, const int64_t *o
                        int64_t B, int64_t N, int64_t n_x, double *grad_C, double *grad_X, void *workspace,
                        int64_t ws_bytes, int32_t *d_err, void *stream);
 
+/* The same backward with a second, row workspace (same results, same
+ * replaced reference interface): the upstream gradient and the cached values
+ * of each group of 64 consecutive expressions are first gathered 64
+ * expressions at a time into per-expression rows, which the reverse sweep
+ * then reads point-contiguously.  Groups whose rows do not fit are read as
+ * without it, so any rows_bytes is valid; the recommended size
+ * (sr_backward_mb_rows_workspace_bytes, K = 1 here) holds rows for a quarter
+ * of the cache.  grad_cache is only read. */
+int64_t sr_backward_mb_rows_workspace_bytes(int64_t M, int64_t N, int64_t B, int64_t K, int is_f64);
+int sr_backward_mb_rows_f32(float *grad_cache, const float *cache, const int64_t *ops, const int64_t *ch, int64_t M,
+                            int64_t B, int64_t N, int64_t n_x, float *grad_C, float *grad_X, void *workspace,
+                            int64_t ws_bytes, void *rows_ws, int64_t rows_bytes, int32_t *d_err, void *stream);
+int sr_backward_mb_rows_f64(double *grad_cache, const double *cache, const int64_t *ops, const int64_t *ch,
+                            int64_t M, int64_t B, int64_t N, int64_t n_x, double *grad_C, double *grad_X,
+                            void *workspace, int64_t ws_bytes, void *rows_ws, int64_t rows_bytes, int32_t *d_err,
+                            void *stream);
+
 /* K constant sets per expression: C (M,B,K), cache (M,N,B,K).  Replaces
  * evaluate_multiple_constant_backend and its kernels
  * (src/symbolic_evaluation.cpp:171-305, src/evaluation_kernels_cuda.cu:761-1364). */
@@ -107,6 +124,16 @@ int sr_multi_backward_mb_f32(float *grad_cache, const float *cache, const int64_
 int sr_multi_backward_mb_f64(double *grad_cache, const double *cache, const int64_t *ops, const int64_t *ch,
                              int64_t M, int64_t B, int64_t N, int64_t n_x, int64_t K, double *grad_C,
                              double *grad_X, void *workspace, int64_t ws_bytes, int32_t *d_err, void *stream);
+/* With a row workspace of sr_backward_mb_rows_workspace_bytes(M, N, B, K,
+ * is_f64) bytes (or any other size), as sr_backward_mb_rows_f32. */
+int sr_multi_backward_mb_rows_f32(float *grad_cache, const float *cache, const int64_t *ops, const int64_t *ch,
+                                  int64_t M, int64_t B, int64_t N, int64_t n_x, int64_t K, float *grad_C,
+                                  float *grad_X, void *workspace, int64_t ws_bytes, void *rows_ws, int64_t rows_bytes,
+                                  int32_t *d_err, void *stream);
+int sr_multi_backward_mb_rows_f64(double *grad_cache, const double *cache, const int64_t *ops, const int64_t *ch,
+                                  int64_t M, int64_t B, int64_t N, int64_t n_x, int64_t K, double *grad_C,
+                                  double *grad_X, void *workspace, int64_t ws_bytes, void *rows_ws,
+                                  int64_t rows_bytes, int32_t *d_err, void *stream);
 
 /* Fused root-only forward + backward over batch-first layouts.  Computes
  * what optimize's closure needs (symbolic_torch/optimization.py:63-101:

@@ -1,5 +1,7 @@
 """The full-cache forward with a workspace (row buffer, then a transposing
-pass) against the same forward without one, through the C-ABI.
+pass) against the same forward without one, and the full-cache backward with
+a row workspace (gathered rows) against the same backward without one,
+through the C-ABI.
 
 Both compute every node with the same kernel code, so the caches must be
 identical bit for bit (NaN padding included), for every workspace size: the
@@ -27,9 +29,9 @@ def _t(a):
     return torch.from_numpy(np.ascontiguousarray(a)).to(_dev())
 
 
-def _inputs(seed, B, M, N, K, dtype):
+def _inputs(seed, B, M, N, K, dtype, **kw):
     rng = np.random.default_rng(seed)
-    ops_bm, ch_bm = helpers.random_batch(seed, B, M, 3, depth=6)
+    ops_bm, ch_bm = helpers.random_batch(seed, B, M, 3, depth=6, **kw)
     X = rng.uniform(-3, 3, (N, 3)).astype(dtype)
     ops, ch = helpers.to_mb(ops_bm, ch_bm)
     C = (rng.standard_normal((M, B, K)) * (ops == 1)[:, :, None]).astype(dtype)
@@ -118,3 +120,84 @@ def test_row_buffer_forward_twice_in_a_row():
     X2, ops2, ch2, C2 = _inputs(12, 70, 48, 96, 1, np.float32)
     b = _written(lib, X2, ops2, ch2, C2, ws_bytes, ws)
     assert np.array_equal(b, _written(lib, X2, ops2, ch2, C2, None), equal_nan=True)
+
+
+def _backward(lib, ops, ch, cache, g, n_x, K, rows_bytes, is64):
+    """(grad_C, grad_X, err) of the multi-constant backward, direct
+    (rows_bytes None) or through a row workspace of rows_bytes bytes."""
+    from cudasymbolicregression_amd import _lib
+
+    M, N, B = cache.shape[0], cache.shape[1], cache.shape[2]
+    sfx = "f64" if is64 else "f32"
+    dev = cache.device
+    gC = torch.full((M, B, K), 5.0, dtype=cache.dtype, device=dev)
+    gX = torch.full((N, n_x), 5.0, dtype=cache.dtype, device=dev)
+    ws = torch.empty(max(int(lib.sr_multi_backward_mb_workspace_bytes(B, N, n_x, K, int(is64))), 256),
+                     dtype=torch.uint8, device=dev)
+    err = torch.empty(1, dtype=torch.int32, device=dev)
+    gg = g.clone()  # a kernel that does not only read it may use it as scratch
+    s = _lib.stream_of(dev)
+    if rows_bytes is None:
+        rc = getattr(lib, f"sr_multi_backward_mb_{sfx}")(
+            _lib.ptr(gg), _lib.ptr(cache), _lib.ptr(ops), _lib.ptr(ch), M, B, N, n_x, K, _lib.ptr(gC), _lib.ptr(gX),
+            _lib.ptr(ws), ws.numel(), _lib.ptr(err), s)
+    else:
+        rws = torch.empty(max(rows_bytes, 1), dtype=torch.uint8, device=dev)
+        rc = getattr(lib, f"sr_multi_backward_mb_rows_{sfx}")(
+            _lib.ptr(gg), _lib.ptr(cache), _lib.ptr(ops), _lib.ptr(ch), M, B, N, n_x, K, _lib.ptr(gC), _lib.ptr(gX),
+            _lib.ptr(ws), ws.numel(), _lib.ptr(rws), rows_bytes, _lib.ptr(err), s)
+    _lib.check_status(rc, "backward")
+    torch.cuda.synchronize()
+    return gC.cpu().numpy(), gX.cpu().numpy(), int(err.item())
+
+
+@pytest.mark.parametrize("dtype,M", [(np.float32, 64), (np.float64, 32)])
+@pytest.mark.parametrize("K", [1, 3])
+def test_row_buffer_backward_equals_direct(dtype, M, K):
+    """Gathered rows against the direct backward: grad_C, grad_X and the
+    error code identical bit for bit, with the upstream gradient on every
+    live row; then with only a nonzero gradient on one NO_OP padding row,
+    which both report as 201."""
+    from cudasymbolicregression_amd import _lib
+
+    lib = _lib.hip()
+    B, N = 200, 150
+    is64 = dtype == np.float64
+    print(f"case row-buffer backward dtype={np.dtype(dtype).name} K={K} B={B} M={M} N={N}", flush=True)
+    X, ops, ch, C = _inputs(31 + K, B, M, N, K, dtype)
+    cache = torch.from_numpy(_written(lib, X, ops, ch, C, None)).to(_dev())  # (M, N, B, K)
+    rng = np.random.default_rng(40 + K)
+    L = ((ops.cpu().numpy() != 0) * np.arange(1, M + 1)[:, None]).max(0)  # per expression b
+    live = (np.arange(M)[:, None] < L[None, :])[:, None, :, None]  # (M, 1, B, 1)
+    g_np = rng.standard_normal((M, N, B, K)) * live
+    g_np[np.isnan(cache.cpu().numpy())] = 0
+    g = torch.from_numpy(g_np.astype(dtype)).to(_dev())
+    E = B * K
+    rec = int(lib.sr_backward_mb_rows_workspace_bytes(M, N, B, K, int(is64)))
+    rows = np.repeat(L, K) * N
+    some = 2 * int(rows[:64 * 2].sum()) * np.dtype(dtype).itemsize  # about the first two groups
+    sizes = (rec, _index_bytes(E) + 512 + some, _index_bytes(E) + 512)
+    ref = _backward(lib, ops, ch, cache, g, 3, K, None, is64)
+    for rb in sizes:
+        got = _backward(lib, ops, ch, cache, g, 3, K, rb, is64)
+        same = all(np.array_equal(a, b, equal_nan=True) for a, b in zip(got[:2], ref[:2])) and got[2] == ref[2]
+        print(f"  rows workspace {rb} B: identical={same} err={got[2]}", flush=True)
+        assert same
+    # only a gradient on a padding row of expression 3 (in the first group,
+    # which fits in the recommended and the partial workspaces), on programs
+    # without division: the backward reports the largest code, and a zero
+    # denominator raises 204 whatever the upstream gradient
+    X, ops, ch, C = _inputs(51 + K, B, M, N, K, dtype, unary=[10, 11, 12, 14], binary=[40, 41, 42])
+    cache = torch.from_numpy(_written(lib, X, ops, ch, C, None)).to(_dev())
+    L = ((ops.cpu().numpy() != 0) * np.arange(1, M + 1)[:, None]).max(0)
+    b3 = 3
+    assert L[b3] < M
+    g2 = torch.zeros_like(cache)
+    g2[L[b3], 7, b3, 0] = 1.0
+    assert _backward(lib, ops, ch, cache, torch.zeros_like(cache), 3, K, None, is64)[2] == 0
+    ref2 = _backward(lib, ops, ch, cache, g2, 3, K, None, is64)
+    assert ref2[2] == 201
+    for rb in sizes:
+        got2 = _backward(lib, ops, ch, cache, g2, 3, K, rb, is64)
+        print(f"  NO_OP gradient, rows workspace {rb} B: err={got2[2]}", flush=True)
+        assert got2[2] == 201
