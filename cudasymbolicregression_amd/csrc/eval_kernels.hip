@@ -656,6 +656,30 @@ __global__ void k_reduce_partials(const T *__restrict__ part, int64_t P, int64_t
     out[i] = (T)s;
 }
 
+// The same sum in two stages (the single-stage kernel gives each column one
+// thread, so a few thousand columns over thousands of rows leave the GPU
+// almost idle): stage 1 sums chunks of RED_ROWS rows per column in FP64,
+// stage 2 adds the chunk sums in chunk order.  Deterministic.
+constexpr int64_t RED_ROWS = 64;
+template <typename T>
+__global__ void k_reduce_partials_1(const T *__restrict__ part, int64_t P, int64_t len, double *__restrict__ part64) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    const int64_t r0 = (int64_t)blockIdx.y * RED_ROWS, r1 = r0 + RED_ROWS < P ? r0 + RED_ROWS : P;
+    double s = 0.0;
+    for (int64_t r = r0; r < r1; ++r) s += (double)part[r * len + i];
+    part64[(int64_t)blockIdx.y * len + i] = s;
+}
+template <typename T>
+__global__ void k_reduce_partials_2(const double *__restrict__ part64, int64_t chunks, int64_t len,
+                                    T *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    double s = 0.0;
+    for (int64_t c = 0; c < chunks; ++c) s += part64[c * len + i];
+    out[i] = (T)s;
+}
+
 inline int ret_last() { return (int)hipGetLastError(); }
 
 template <typename T>
@@ -712,8 +736,11 @@ int forward_mb(const T *X, int64_t N, int64_t n_x, const int64_t *ops, const int
 
 template <typename T>
 int64_t backward_mb_ws_bytes(int64_t B, int64_t N, int64_t n_x, int64_t K = 1) {
-    int64_t nblk = (B * K + BWD_WAVES - 1) / BWD_WAVES;
-    return nblk * N * n_x * (int64_t)sizeof(T) + 256;
+    // per-block partial rows of grad_X, then the FP64 chunk sums of the reduction
+    const int64_t nblk = (B * K + BWD_WAVES - 1) / BWD_WAVES;
+    const int64_t part = (nblk * N * n_x * (int64_t)sizeof(T) + 255) / 256 * 256;
+    const int64_t chunks = (nblk + RED_ROWS - 1) / RED_ROWS;
+    return part + chunks * N * n_x * 8 + 256;
 }
 
 template <typename T>
@@ -793,9 +820,19 @@ int backward_mb(T *grad_out, const T *cache, const int64_t *ops, const int64_t *
                            (grad_X && !gx_regs) ? grad_X : nullptr, d_err);
     }
     if (gx_partial) {
-        int64_t len = N * n_x;
-        hipLaunchKernelGGL(k_reduce_partials<T>, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s,
-                           (const T *)gx_partial, nblk, len, grad_X, 0);
+        const int64_t len = N * n_x;
+        const int64_t chunks = (nblk + RED_ROWS - 1) / RED_ROWS;
+        const int64_t part = (nblk * len * (int64_t)sizeof(T) + 255) / 256 * 256;
+        double *part64 = reinterpret_cast<double *>(static_cast<unsigned char *>(workspace) + part);
+        if (chunks <= 65535) {
+            hipLaunchKernelGGL(k_reduce_partials_1<T>, dim3((unsigned)((len + 255) / 256), (unsigned)chunks), dim3(256),
+                               0, s, (const T *)gx_partial, nblk, len, part64);
+            hipLaunchKernelGGL(k_reduce_partials_2<T>, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s,
+                               (const double *)part64, chunks, len, grad_X);
+        } else {
+            hipLaunchKernelGGL(k_reduce_partials<T>, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s,
+                               (const T *)gx_partial, nblk, len, grad_X, 0);
+        }
     }
     return ret_last();
 }
