@@ -511,8 +511,10 @@ __global__ __launch_bounds__(GA_WAVES * 64) void k_bwd_gather(const T *__restric
 
 // ROWS: the blocks whose group was gathered (fr), with the gathered value
 // and gradient rows as the tapes (the adjoints accumulate in place in the
-// gradient rows, a copy), so the tapes take no LDS; otherwise the other
-// blocks, with LDS tapes loaded from the cache layout.
+// gradient rows, a copy), so the tapes take no LDS and the kernel runs at
+// full occupancy (small LDS tapes for short programs, 32 KB a block, were
+// slower); otherwise the other blocks, with LDS tapes loaded from the cache
+// layout.
 template <typename T, bool ROWS>
 __global__ __launch_bounds__(BWD_WAVES * 64) void k_full_backward_grp(
     const T *__restrict__ grad_cache, const T *__restrict__ cache, const int64_t *__restrict__ ops,
