@@ -161,6 +161,19 @@ def _workspace(nbytes: int, device) -> torch.Tensor:
     return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
 
 
+def _speed_workspace(nbytes: int, device):
+    """An optional workspace (row buffers of the full-cache evaluation): any
+    size is valid there, so when memory is short the call runs without it
+    (the direct path, same results) instead of failing.  Returns (tensor or
+    None, its size in bytes)."""
+    if nbytes <= 0:
+        return None, 0
+    try:
+        return torch.empty(int(nbytes), dtype=torch.uint8, device=device), int(nbytes)
+    except torch.cuda.OutOfMemoryError:
+        return None, 0
+
+
 class _SymbolicEvaluation(torch.autograd.Function):
     """Autograd function of the full-cache evaluation
     (reference src/symbolic_evaluation.cpp:39-168)."""
@@ -173,11 +186,11 @@ class _SymbolicEvaluation(torch.autograd.Function):
         is64 = X.dtype == torch.float64
         sfx = "f64" if is64 else "f32"
         lib = _lib.hip()
-        ws = _workspace(lib.sr_forward_mb_workspace_bytes(M, N, B, 1, int(is64)), X.device)
+        ws, ws_n = _speed_workspace(lib.sr_forward_mb_workspace_bytes(M, N, B, 1, int(is64)), X.device)
         _lib.check_status(
             getattr(lib, f"sr_forward_mb_ws_{sfx}")(
                 _lib.ptr(X), N, n_x, _lib.ptr(Ops), _lib.ptr(Ch), _lib.ptr(C), M, B, _lib.ptr(cache),
-                _lib.ptr(ws), ws.numel(), _lib.stream_of(X.device),
+                _lib.ptr(ws), ws_n, _lib.stream_of(X.device),
             ),
             "symbolic forward",
         )
@@ -209,12 +222,13 @@ class _SymbolicEvaluation(torch.autograd.Function):
         lib = _lib.hip()
         ws = _workspace(lib.sr_backward_mb_workspace_bytes(B, N, n_x, int(is64)) if want_x else 0, X.device)
         # gathered gradient and value rows (read by the kernel that only reads g)
-        rws = _workspace(lib.sr_backward_mb_rows_workspace_bytes(M, N, B, 1, int(is64)) if grouped else 0, X.device)
+        rws, rws_n = _speed_workspace(lib.sr_backward_mb_rows_workspace_bytes(M, N, B, 1, int(is64)) if grouped else 0,
+                                      X.device)
         err = torch.empty(1, dtype=torch.int32, device=X.device)
         _lib.check_status(
             getattr(lib, f"sr_backward_mb_rows_{'f64' if is64 else 'f32'}")(
                 _lib.ptr(g), _lib.ptr(cache), _lib.ptr(Ops), _lib.ptr(Ch), M, B, N, n_x, _lib.ptr(grad_C),
-                _lib.ptr(grad_X), _lib.ptr(ws), ws.numel(), _lib.ptr(rws), rws.numel() if grouped else 0,
+                _lib.ptr(grad_X), _lib.ptr(ws), ws.numel(), _lib.ptr(rws), rws_n,
                 _lib.ptr(err), _lib.stream_of(X.device),
             ),
             "symbolic backward",
@@ -293,11 +307,11 @@ class _SymbolicEvaluationMultiple(torch.autograd.Function):
         is64 = X.dtype == torch.float64
         sfx = "f64" if is64 else "f32"
         lib = _lib.hip()
-        ws = _workspace(lib.sr_forward_mb_workspace_bytes(M, N, B, K, int(is64)), X.device)
+        ws, ws_n = _speed_workspace(lib.sr_forward_mb_workspace_bytes(M, N, B, K, int(is64)), X.device)
         _lib.check_status(
             getattr(lib, f"sr_multi_forward_mb_ws_{sfx}")(
                 _lib.ptr(X), N, n_x, _lib.ptr(Ops), _lib.ptr(Ch), _lib.ptr(C), M, B, K, _lib.ptr(cache),
-                _lib.ptr(ws), ws.numel(), _lib.stream_of(X.device),
+                _lib.ptr(ws), ws_n, _lib.stream_of(X.device),
             ),
             "symbolic multiple forward",
         )
@@ -323,12 +337,13 @@ class _SymbolicEvaluationMultiple(torch.autograd.Function):
         lib = _lib.hip()
         ws = _workspace(lib.sr_multi_backward_mb_workspace_bytes(B, N, n_x, K, int(is64)) if want_x else 0,
                         X.device)
-        rws = _workspace(lib.sr_backward_mb_rows_workspace_bytes(M, N, B, K, int(is64)) if grouped else 0, X.device)
+        rws, rws_n = _speed_workspace(lib.sr_backward_mb_rows_workspace_bytes(M, N, B, K, int(is64)) if grouped else 0,
+                                      X.device)
         err = torch.empty(1, dtype=torch.int32, device=X.device)
         _lib.check_status(
             getattr(lib, f"sr_multi_backward_mb_rows_{'f64' if is64 else 'f32'}")(
                 _lib.ptr(g), _lib.ptr(cache), _lib.ptr(Ops), _lib.ptr(Ch), M, B, N, n_x, K, _lib.ptr(grad_C),
-                _lib.ptr(grad_X), _lib.ptr(ws), ws.numel(), _lib.ptr(rws), rws.numel() if grouped else 0,
+                _lib.ptr(grad_X), _lib.ptr(ws), ws.numel(), _lib.ptr(rws), rws_n,
                 _lib.ptr(err), _lib.stream_of(X.device),
             ),
             "symbolic multiple backward",

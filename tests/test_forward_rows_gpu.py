@@ -6,8 +6,9 @@ through the C-ABI.
 Both compute every node with the same kernel code, so the caches must be
 identical bit for bit (NaN padding included), for every workspace size: the
 recommended one (every group of 64 expressions through the row buffer), one
-that holds only some groups (the others written directly), and the index
-arrays alone (no group fits).  Shapes cover a partial last group of
+that holds only some groups (the others written directly), the index arrays
+alone (no group fits), and none (a NULL workspace, which the API passes when
+memory is short).  Shapes cover a partial last group of
 expressions (B = 200), a partial last tile of points (N = 150), programs up to
 M = 64 nodes, FP32 and FP64, and K = 3 constant sets.  The values themselves
 are pinned against the oracle by test_eval_gpu.py, which runs the same path
@@ -50,9 +51,9 @@ def _forward(lib, X, ops, ch, C, ws_bytes, ws=None, fill=7.0):
         rc = getattr(lib, f"sr_multi_forward_mb_{sfx}")(_lib.ptr(X), N, 3, _lib.ptr(ops), _lib.ptr(ch), _lib.ptr(C),
                                                         M, B, K, _lib.ptr(cache), s)
     else:
-        if ws is None:
-            ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=X.device)
-        assert ws.numel() >= ws_bytes
+        if ws is None and ws_bytes > 0:
+            ws = torch.empty(ws_bytes, dtype=torch.uint8, device=X.device)
+        assert ws_bytes == 0 or ws.numel() >= ws_bytes  # 0: a NULL workspace
         if K == 1:
             rc = getattr(lib, f"sr_forward_mb_ws_{sfx}")(_lib.ptr(X), N, 3, _lib.ptr(ops), _lib.ptr(ch),
                                                          _lib.ptr(C), M, B, _lib.ptr(cache), _lib.ptr(ws), ws_bytes,
@@ -98,7 +99,7 @@ def test_row_buffer_forward_equals_direct(dtype, K):
     rows = np.repeat(L.max(0), K) * N  # row-buffer elements per expression
     some = int(rows[:64 * 2].sum())  # room for about the first two groups
     item = np.dtype(dtype).itemsize
-    for ws_bytes in (rec, _index_bytes(E) + 256 + some * item, _index_bytes(E) + 256):
+    for ws_bytes in (rec, _index_bytes(E) + 256 + some * item, _index_bytes(E) + 256, 0):
         got = _written(lib, X, ops, ch, C, ws_bytes)
         same = np.array_equal(got, ref, equal_nan=True)
         print(f"  workspace {ws_bytes} B: identical={same}", flush=True)
@@ -142,7 +143,7 @@ def _backward(lib, ops, ch, cache, g, n_x, K, rows_bytes, is64):
             _lib.ptr(gg), _lib.ptr(cache), _lib.ptr(ops), _lib.ptr(ch), M, B, N, n_x, K, _lib.ptr(gC), _lib.ptr(gX),
             _lib.ptr(ws), ws.numel(), _lib.ptr(err), s)
     else:
-        rws = torch.empty(max(rows_bytes, 1), dtype=torch.uint8, device=dev)
+        rws = torch.empty(rows_bytes, dtype=torch.uint8, device=dev) if rows_bytes > 0 else None  # 0: NULL
         rc = getattr(lib, f"sr_multi_backward_mb_rows_{sfx}")(
             _lib.ptr(gg), _lib.ptr(cache), _lib.ptr(ops), _lib.ptr(ch), M, B, N, n_x, K, _lib.ptr(gC), _lib.ptr(gX),
             _lib.ptr(ws), ws.numel(), _lib.ptr(rws), rows_bytes, _lib.ptr(err), s)
@@ -176,7 +177,7 @@ def test_row_buffer_backward_equals_direct(dtype, M, K):
     rec = int(lib.sr_backward_mb_rows_workspace_bytes(M, N, B, K, int(is64)))
     rows = np.repeat(L, K) * N
     some = 2 * int(rows[:64 * 2].sum()) * np.dtype(dtype).itemsize  # about the first two groups
-    sizes = (rec, _index_bytes(E) + 512 + some, _index_bytes(E) + 512)
+    sizes = (rec, _index_bytes(E) + 512 + some, _index_bytes(E) + 512, 0)
     ref = _backward(lib, ops, ch, cache, g, 3, K, None, is64)
     for rb in sizes:
         got = _backward(lib, ops, ch, cache, g, 3, K, rb, is64)
