@@ -161,12 +161,17 @@ def _workspace(nbytes: int, device) -> torch.Tensor:
     return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
 
 
-def _speed_workspace(nbytes: int, device):
+# Caches smaller than this are evaluated without row workspaces: their extra
+# launches would cost more than the narrow cache accesses they avoid.
+ROW_WORKSPACE_MIN_CACHE_BYTES = 64 << 20
+
+
+def _speed_workspace(nbytes: int, device, cache_bytes: int):
     """An optional workspace (row buffers of the full-cache evaluation): any
-    size is valid there, so when memory is short the call runs without it
-    (the direct path, same results) instead of failing.  Returns (tensor or
-    None, its size in bytes)."""
-    if nbytes <= 0:
+    size is valid there, so for small caches, or when memory is short, the
+    call runs without it (the direct path, same results) instead of failing.
+    Returns (tensor or None, its size in bytes)."""
+    if nbytes <= 0 or cache_bytes < ROW_WORKSPACE_MIN_CACHE_BYTES:
         return None, 0
     try:
         return torch.empty(int(nbytes), dtype=torch.uint8, device=device), int(nbytes)
@@ -186,7 +191,8 @@ class _SymbolicEvaluation(torch.autograd.Function):
         is64 = X.dtype == torch.float64
         sfx = "f64" if is64 else "f32"
         lib = _lib.hip()
-        ws, ws_n = _speed_workspace(lib.sr_forward_mb_workspace_bytes(M, N, B, 1, int(is64)), X.device)
+        ws, ws_n = _speed_workspace(lib.sr_forward_mb_workspace_bytes(M, N, B, 1, int(is64)), X.device,
+                                    cache.numel() * cache.element_size())
         _lib.check_status(
             getattr(lib, f"sr_forward_mb_ws_{sfx}")(
                 _lib.ptr(X), N, n_x, _lib.ptr(Ops), _lib.ptr(Ch), _lib.ptr(C), M, B, _lib.ptr(cache),
@@ -223,7 +229,7 @@ class _SymbolicEvaluation(torch.autograd.Function):
         ws = _workspace(lib.sr_backward_mb_workspace_bytes(B, N, n_x, int(is64)) if want_x else 0, X.device)
         # gathered gradient and value rows (read by the kernel that only reads g)
         rws, rws_n = _speed_workspace(lib.sr_backward_mb_rows_workspace_bytes(M, N, B, 1, int(is64)) if grouped else 0,
-                                      X.device)
+                                      X.device, cache.numel() * cache.element_size())
         err = torch.empty(1, dtype=torch.int32, device=X.device)
         _lib.check_status(
             getattr(lib, f"sr_backward_mb_rows_{'f64' if is64 else 'f32'}")(
@@ -307,7 +313,8 @@ class _SymbolicEvaluationMultiple(torch.autograd.Function):
         is64 = X.dtype == torch.float64
         sfx = "f64" if is64 else "f32"
         lib = _lib.hip()
-        ws, ws_n = _speed_workspace(lib.sr_forward_mb_workspace_bytes(M, N, B, K, int(is64)), X.device)
+        ws, ws_n = _speed_workspace(lib.sr_forward_mb_workspace_bytes(M, N, B, K, int(is64)), X.device,
+                                    cache.numel() * cache.element_size())
         _lib.check_status(
             getattr(lib, f"sr_multi_forward_mb_ws_{sfx}")(
                 _lib.ptr(X), N, n_x, _lib.ptr(Ops), _lib.ptr(Ch), _lib.ptr(C), M, B, K, _lib.ptr(cache),
@@ -338,7 +345,7 @@ class _SymbolicEvaluationMultiple(torch.autograd.Function):
         ws = _workspace(lib.sr_multi_backward_mb_workspace_bytes(B, N, n_x, K, int(is64)) if want_x else 0,
                         X.device)
         rws, rws_n = _speed_workspace(lib.sr_backward_mb_rows_workspace_bytes(M, N, B, K, int(is64)) if grouped else 0,
-                                      X.device)
+                                      X.device, cache.numel() * cache.element_size())
         err = torch.empty(1, dtype=torch.int32, device=X.device)
         _lib.check_status(
             getattr(lib, f"sr_multi_backward_mb_rows_{'f64' if is64 else 'f32'}")(

@@ -202,3 +202,39 @@ def test_row_buffer_backward_equals_direct(dtype, M, K):
         got2 = _backward(lib, ops, ch, cache, g2, 3, K, rb, is64)
         print(f"  NO_OP gradient, rows workspace {rb} B: err={got2[2]}", flush=True)
         assert got2[2] == 201
+
+
+def test_api_row_workspaces_match_direct(monkeypatch):
+    """evaluate_backend forward and backward through autograd at a 64 MB cache
+    (where the API uses the row workspaces) equal the same calls with the
+    workspaces switched off: cache, grad_X and grad_C bit for bit."""
+    from cudasymbolicregression_amd import _C
+    from cudasymbolicregression_amd.evaluation import evaluate_backend
+
+    B, M, N = 256, 64, 1024
+    print(f"case API row workspaces B={B} M={M} N={N}", flush=True)
+    assert M * N * B * 4 >= _C.ROW_WORKSPACE_MIN_CACHE_BYTES
+    # no division: a zero denominator would raise 204 in the backward
+    X, ops, ch, C = _inputs(61, B, M, N, 1, np.float32, unary=[10, 11, 12, 14], binary=[40, 41, 42])
+    C2 = C[:, :, 0].contiguous()
+    L = ((ops.cpu().numpy() != 0) * np.arange(1, M + 1)[:, None]).max(0)
+    live = torch.from_numpy(np.arange(M)[:, None] < L[None, :]).to(_dev())[:, None, :]  # (M, 1, B)
+
+    def run():
+        Xr = X.clone().requires_grad_(True)
+        Cr = C2.clone().requires_grad_(True)
+        y = evaluate_backend(Xr, ops, ch, Cr)
+        gen = torch.Generator(device=_dev()).manual_seed(3)
+        g = torch.randn(y.shape, generator=gen, device=_dev()) * live
+        g = torch.where(torch.isnan(y.detach()), torch.zeros_like(g), g)
+        y.backward(g)
+        torch.cuda.synchronize()
+        return y.detach().cpu().numpy(), Xr.grad.cpu().numpy(), Cr.grad.cpu().numpy()
+
+    a = run()
+    monkeypatch.setattr(_C, "ROW_WORKSPACE_MIN_CACHE_BYTES", 1 << 62)
+    b = run()
+    for name, u, v in zip(("cache", "grad_X", "grad_C"), a, b):
+        same = np.array_equal(u, v, equal_nan=True)
+        print(f"  {name}: identical={same}", flush=True)
+        assert same
