@@ -23,8 +23,9 @@ trained beforehand (untimed, reported separately) on (Ops, Y, fitted C)
 tuples labelled by the plain L-BFGS, from a disjoint sample of problems.
 
 Also reported: the roofline of the fitting kernel (algorithmic FP32 slots per
-closure x closures / kernel time, against the FP32 peak), the C2 evaluate
-measurement (fused root-only, and the API full-cache forward/backward), the uneven-batching (C4) fit of 32768 problems with their own
+closure x closures / kernel time, against the FP32 peak), the evaluate
+measurement at 65536 expressions x 1024 points (fused root-only, and the API
+full-cache forward/backward), the uneven-batching fit of 32768 problems with their own
 datasets, and a CPU baseline (the oracle's restatement of the
 reference's CPU algorithm on a bounded sample, single core).
 """
@@ -77,7 +78,7 @@ FP32_PEAK_TFLOPS = 157.3
 VALU_LANE_PEAK = 256 * 4 * 16 * 2.4e9  # unpacked FP32 lane-instructions/s (3.93e13)
 HBM_PEAK_GBS = 8000.0
 FIT_TRAFFIC_PROFILED = 101.3e6  # HBM bytes per fit call, rocprofv3 FETCH_SIZE/WRITE_SIZE (calibrated)
-C2_TRAFFIC_PROFILED = 887.6e6  # HBM bytes per C2 evaluate call, same counters
+EVAL_TRAFFIC_PROFILED = 887.6e6  # HBM bytes per fused evaluate call (65536 x 1024), same counters
 
 
 def log(msg: str) -> None:
@@ -126,7 +127,7 @@ def bench_evaluate(dev, steps: int, seed: int = 1, B: int = 65536, N: int = 1024
     return dict(workload=f"G2 B={B}, padded 64, N={N}, n_x=3, fused fwd+bwd with grad_X", ms=round(ms, 4),
                 hbm=dict(achieved_GBps=round(nbytes / t / 1e9, 1), peak_GBps=HBM_PEAK_GBS,
                          frac=round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4), bytes=nbytes,
-                         traffic=C2_TRAFFIC_PROFILED, traffic_source="profiles/evaluate_c2_hbm_traffic.md"),
+                         traffic=EVAL_TRAFFIC_PROFILED, traffic_source="profiles/evaluate_c2_hbm_traffic.md"),
                 valu=dict(achieved_TFLOPs=round(slots / t / 1e12, 4), peak_TFLOPs=FP32_PEAK_TFLOPS,
                           frac=round(slots / t / 1e12 / FP32_PEAK_TFLOPS, 5),
                           frac_unpacked_lane_peak=round(slots / t / VALU_LANE_PEAK, 4), slots=slots),
@@ -135,7 +136,7 @@ def bench_evaluate(dev, steps: int, seed: int = 1, B: int = 65536, N: int = 1024
 
 
 def bench_full_cache(dev, steps: int, seed: int = 1, B: int = 65536, N: int = 1024):
-    """The API-compatible evaluate at the C2 size: the forward writes the
+    """The API-compatible evaluate at 65536 expressions x 1024 points: the forward writes the
     (M, N, B) cache of every node (M = 64: 17.2 GB) and the backward reads an
     upstream gradient of the same shape (nonzero on each root row) and the
     cached values; both through the C-ABI, timed with HIP events.  HBM bytes:
@@ -427,18 +428,18 @@ def main() -> int:
     ev = None
     if not args.no_evaluate:
         ev = bench_evaluate(dev, steps=5)
-        log(f"evaluate (C2): {ev['ms']:.3f} ms, {ev['hbm']['achieved_GBps']} GB/s, VALU frac {ev['valu']['frac']}")
+        log(f"evaluate (65536 x 1024): {ev['ms']:.3f} ms, {ev['hbm']['achieved_GBps']} GB/s, VALU frac {ev['valu']['frac']}")
 
     fc = None
     if not args.no_evaluate:
         fc = bench_full_cache(dev, steps=3)
-        log(f"full-cache evaluate (C2): forward {fc['forward']['ms']:.2f} ms ({fc['forward']['achieved_GBps']} GB/s), "
+        log(f"full-cache evaluate (65536 x 1024): forward {fc['forward']['ms']:.2f} ms ({fc['forward']['achieved_GBps']} GB/s), "
             f"backward {fc['backward']['ms']:.2f} ms ({fc['backward']['achieved_GBps']} GB/s)")
 
     rg = None
     if not args.no_ragged:
         rg = bench_ragged(dev, steps=2)
-        log(f"ragged (C4): {rg['ms']:.1f} ms, {rg['fitted']} fitted, {rg['fitted_per_s']:.0f} fitted/s")
+        log(f"ragged (32768 own datasets): {rg['ms']:.1f} ms, {rg['fitted']} fitted, {rg['fitted_per_s']:.0f} fitted/s")
 
     cpu = None
     if not args.no_cpu:
