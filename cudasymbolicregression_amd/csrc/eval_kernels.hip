@@ -593,8 +593,8 @@ __global__ __launch_bounds__(BWD_WAVES * 64) void k_full_backward_grp(
                 }
             }
         }
+        __syncthreads();  // the LDS tapes are loaded (the row tapes need none)
         }
-        __syncthreads();
         const int64_t n = tt * TILE + lane;
         const bool active = bvalid && n < N;
         const int64_t nn = active ? n : 0;
