@@ -432,6 +432,7 @@ __global__ __launch_bounds__(BWD_WAVES * 64) void k_full_backward(
 // copy of the upstream gradient and is only read.  Outputs and reductions
 // as in k_full_backward.
 constexpr int BWD_BATCH = 16;
+constexpr int BWR_WPE = 8;  // waves per SIMD the FP32 row-tape sweep's registers are sized for (FP64 would spill)
 
 // Backward gather (with a row-buffer workspace).  The grouped backward
 // below reads the upstream gradient and the cached values per (node, point)
@@ -516,7 +517,8 @@ __global__ __launch_bounds__(GA_WAVES * 64) void k_bwd_gather(const T *__restric
 // slower); otherwise the other blocks, with LDS tapes loaded from the cache
 // layout.
 template <typename T, bool ROWS>
-__global__ __launch_bounds__(BWD_WAVES * 64) void k_full_backward_grp(
+__global__ __launch_bounds__(BWD_WAVES * 64) __attribute__((amdgpu_waves_per_eu(ROWS && sizeof(T) == 4 ? BWR_WPE : 1))) void
+k_full_backward_grp(
     const T *__restrict__ grad_cache, const T *__restrict__ cache, const int64_t *__restrict__ ops,
     const int64_t *__restrict__ ch, int M, int64_t B, int64_t K, int64_t N, int n_x, T *__restrict__ grad_C,
     T *__restrict__ gx_partial, T *__restrict__ grad_X_direct, int32_t *err_flag, FwdRows fr,
