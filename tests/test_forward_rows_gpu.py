@@ -238,3 +238,41 @@ def test_api_row_workspaces_match_direct(monkeypatch):
         same = np.array_equal(u, v, equal_nan=True)
         print(f"  {name}: identical={same}", flush=True)
         assert same
+
+
+def test_api_multiple_row_workspaces_match_direct(monkeypatch):
+    """evaluate_multiple (K constant sets) forward and backward at a 67 MB
+    cache, with and without the row workspaces: cache, grad_X and grad_C bit
+    for bit."""
+    from cudasymbolicregression_amd import _C
+    from cudasymbolicregression_amd.evaluation import evaluate_multiple
+
+    B, M, N, K = 64, 64, 1024, 4
+    print(f"case API multiple row workspaces B={B} M={M} N={N} K={K}", flush=True)
+    assert M * N * B * K * 4 >= _C.ROW_WORKSPACE_MIN_CACHE_BYTES
+    rng = np.random.default_rng(71)
+    ops_bm, ch_bm = helpers.random_batch(71, B, M, 3, depth=6, unary=[10, 11, 12, 14], binary=[40, 41, 42])
+    X = _t(rng.uniform(-3, 3, (N, 3)).astype(np.float32))
+    C = _t((rng.standard_normal((B, M, K)) * (ops_bm == 1)[:, :, None]).astype(np.float32))
+    ops, ch = _t(ops_bm), _t(ch_bm)
+    L = ((ops_bm != 0) * np.arange(1, M + 1)[None, :]).max(1)  # per expression b
+    live = torch.from_numpy(np.arange(M)[None, :] < L[:, None]).to(_dev())[:, :, None, None]  # (B, M, 1, 1)
+
+    def run():
+        Xr = X.clone().requires_grad_(True)
+        Cr = C.clone().requires_grad_(True)
+        y = evaluate_multiple(Xr, ops, ch, Cr)  # (B, M, N, K)
+        gen = torch.Generator(device=_dev()).manual_seed(5)
+        g = torch.randn(y.shape, generator=gen, device=_dev()) * live
+        g = torch.where(torch.isnan(y.detach()), torch.zeros_like(g), g)
+        y.backward(g)
+        torch.cuda.synchronize()
+        return y.detach().cpu().numpy(), Xr.grad.cpu().numpy(), Cr.grad.cpu().numpy()
+
+    a = run()
+    monkeypatch.setattr(_C, "ROW_WORKSPACE_MIN_CACHE_BYTES", 1 << 62)
+    b = run()
+    for name, u, v in zip(("cache", "grad_X", "grad_C"), a, b):
+        same = np.array_equal(u, v, equal_nan=True)
+        print(f"  {name}: identical={same}", flush=True)
+        assert same
