@@ -29,33 +29,34 @@ inline size_t queue_ws_bytes(int64_t B, int NB) {
 namespace {
 
 // Pass 2: one block of 256 threads; thread t owns a contiguous range of runs.
+// The 256 thread totals are scanned per wave with shuffles, then across the
+// 4 wave totals (integer sums: the offsets do not depend on the order).
 __global__ __launch_bounds__(256) void k_queue_scan(const int32_t *__restrict__ runcnt, int64_t R, int NB,
                                                    int32_t *__restrict__ runoff, int32_t *__restrict__ counts) {
-    __shared__ int32_t part[256];
-    const int t = threadIdx.x;
+    __shared__ int32_t wtot[256 / WAVE];
+    const int t = threadIdx.x, lane = lane_id(), w = t / WAVE;
     const int64_t per = (R + 255) / 256;
     const int64_t r0 = t * per, r1 = r0 + per < R ? r0 + per : R;
     for (int k = 0; k < NB; ++k) {
         int32_t s = 0;
         for (int64_t r = r0; r < r1; ++r) s += runcnt[r * NB + k];
-        part[t] = s;
-        __syncthreads();
-        if (t == 0) {  // exclusive scan of the 256 thread totals, in order
-            int32_t acc = 0;
-            for (int i = 0; i < 256; ++i) {
-                const int32_t v = part[i];
-                part[i] = acc;
-                acc += v;
-            }
-            counts[k] = acc;
+        int32_t incl = s;
+#pragma unroll
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const int32_t up = __shfl_up(incl, d, WAVE);
+            if (lane >= d) incl += up;
         }
+        if (lane == WAVE - 1) wtot[w] = incl;
         __syncthreads();
-        int32_t acc = part[t];
+        int32_t before = 0;
+        for (int q = 0; q < w; ++q) before += wtot[q];
+        if (t == 255) counts[k] = before + incl;
+        int32_t acc = before + incl - s;
         for (int64_t r = r0; r < r1; ++r) {
             runoff[r * NB + k] = acc;
             acc += runcnt[r * NB + k];
         }
-        __syncthreads();
+        __syncthreads();  // wtot is rewritten for the next bucket
     }
 }
 
