@@ -81,14 +81,12 @@ def hip() -> ctypes.CDLL:
         _declare(lib, "sr_fused_workspace_bytes", I64, [I64, I64, I64, I64, I, I])
         _declare(lib, "sr_lbfgs_fit", I,
                  [P, P, I64, I64, P, P, P, I64, I64, P, P, P, P, P, P, P, P, P, P, I, I, c_double, I, I, c_double,
-                  P, I64, P])
+                  P, P, I64, P])
         _declare(lib, "sr_lbfgs_workspace_bytes", I64, [I64, I64])
         _declare(lib, "sr_fit_num_buckets", I, [])
         _declare(lib, "sr_fit_num_variants", I, [I])
-        _declare(lib, "sr_fit_get_variant", I, [I])
-        _declare(lib, "sr_fit_set_variant", I, [I, I])
+        _declare(lib, "sr_fit_default_variant", I, [I])
         _declare(lib, "sr_fit_variant_info", I, [I, I, POINTER(c_int), POINTER(c_int)])
-        _declare(lib, "sr_fit_set_stationary_exit", I, [I])
         F = c_float
         _declare(lib, "sr_mlp_gemm_nt", I, [P, P, P, I64, I64, I64, I64, I64, I, P, P, P, P])
         _declare(lib, "sr_mlp_onehot_gelu", I, [P, P, P, P, I64, I64, I64, I, I, I64, P, P, P, P])

@@ -1,22 +1,53 @@
-"""Autotuner for the fit kernels' launch variants.
+"""Autotuner for the launch variants of the fit kernels.
 
-The device L-BFGS (`optimize`) buckets problems by tape rows and runs
-one kernel per bucket; each bucket has a few launch variants that trade
+The device L-BFGS (`optimize`) buckets problems by tape rows and runs one
+kernel per bucket; each bucket has a few launch variants that trade
 per-problem latency against throughput: W waves cooperating on one problem
-(points split between them) and PPT points per lane per instruction dispatch.
-Which variant is fastest depends on the workload (program lengths, points,
-iteration counts), so `tune_fit` times them on a representative batch with
-coordinate descent over the buckets and keeps the fastest for the process.
+(points split between them) and PPT points per lane per instruction
+dispatch.  Which variant is fastest depends on the workload (program
+lengths, points, iteration counts), so `tune_fit` times them on a
+representative batch with coordinate descent over the buckets.
+
+The chosen configuration is stored in a workload-keyed cache (device name,
+padded length, point count, variables, history, line search); `optimize`
+looks its own workload up there and otherwise runs the built-in defaults.
+The configuration is a per-call option of the C-ABI (`sr_fit_options`), so
+the library itself holds no tuning state and calls on several threads do
+not interfere.  The cache is guarded by a lock and can be saved to / loaded
+from a JSON file.
 """
 from __future__ import annotations
 
 import ctypes
+import json
+import math
+import threading
 import time
-from typing import Callable, Dict, List, Optional
+from typing import Callable, Dict, List, Optional, Sequence, Tuple
 
 import torch
 
 from . import _lib
+
+N_BUCKETS = 5  # SR_FIT_BUCKETS of include/sr_eval.h
+
+
+class FitOptions(ctypes.Structure):
+    """`sr_fit_options` of include/sr_eval.h."""
+
+    _fields_ = [("variant", ctypes.c_int * N_BUCKETS), ("stationary_exit", ctypes.c_int)]
+
+
+def make_options(config: Optional[Sequence[int]] = None, stationary_exit: bool = True) -> FitOptions:
+    """Options for one sr_lbfgs_fit call (config None: the defaults)."""
+    o = FitOptions()
+    cfg = list(config) if config is not None else [-1] * N_BUCKETS
+    if len(cfg) != N_BUCKETS:
+        raise ValueError(f"a fit configuration names {N_BUCKETS} variants, got {len(cfg)}")
+    for b, v in enumerate(cfg):
+        o.variant[b] = int(v)
+    o.stationary_exit = 1 if stationary_exit else 0
+    return o
 
 
 def variants() -> List[List[Dict[str, int]]]:
@@ -33,18 +64,78 @@ def variants() -> List[List[Dict[str, int]]]:
     return out
 
 
-def get_config() -> List[int]:
+def default_config() -> List[int]:
     lib = _lib.hip()
-    return [lib.sr_fit_get_variant(b) for b in range(lib.sr_fit_num_buckets())]
+    return [int(lib.sr_fit_default_variant(b)) for b in range(lib.sr_fit_num_buckets())]
 
 
-def set_config(config: List[int]) -> None:
-    lib = _lib.hip()
-    for b, v in enumerate(config):
-        if lib.sr_fit_set_variant(b, int(v)) != 0:
-            raise ValueError(f"invalid variant {v} for bucket {b}")
+def validate_config(config: Sequence[int]) -> List[int]:
+    n_var = [len(v) for v in variants()]
+    cfg = [int(v) for v in config]
+    if len(cfg) != len(n_var) or any(not (0 <= v < n) for v, n in zip(cfg, n_var)):
+        raise ValueError(f"invalid fit configuration {cfg} (variants per bucket: {n_var})")
+    return cfg
 
 
+# --------------------------------------------------------------- the cache
+def workload_key(device: torch.device, M: int, N: int, n_x: int, history_size: int = 10,
+                 line_search_fn: str = "strong_wolfe") -> Tuple:
+    """Cache key of a fit workload: the device's name, the padded length, the
+    point count rounded to a power of two, the variables, the history size
+    and the line search."""
+    device = torch.device(device)
+    name = torch.cuda.get_device_name(device) if device.type == "cuda" else device.type
+    n_bucket = 1 << max(0, int(math.ceil(math.log2(max(int(N), 1)))))
+    return (name, int(M), n_bucket, int(n_x), int(history_size), str(line_search_fn))
+
+
+_cache_lock = threading.Lock()
+_cache: Dict[Tuple, Dict[str, object]] = {}
+
+
+def cache_put(key: Tuple, config: Sequence[int], ms: Optional[float] = None) -> None:
+    cfg = validate_config(config)
+    with _cache_lock:
+        _cache[tuple(key)] = dict(config=cfg, ms=ms)
+
+
+def cache_get(key: Tuple) -> Optional[List[int]]:
+    with _cache_lock:
+        e = _cache.get(tuple(key))
+        return list(e["config"]) if e else None
+
+
+def cache_clear() -> None:
+    with _cache_lock:
+        _cache.clear()
+
+
+def cache_save(path: str) -> None:
+    with _cache_lock:
+        rows = [dict(key=list(k), **v) for k, v in _cache.items()]
+    with open(path, "w") as f:
+        json.dump(rows, f, indent=1)
+
+
+def cache_load(path: str) -> int:
+    """Adds the entries of a saved cache; returns how many were read."""
+    with open(path) as f:
+        rows = json.load(f)
+    for r in rows:
+        cache_put(tuple(r["key"]), r["config"], r.get("ms"))
+    return len(rows)
+
+
+def lookup(device: torch.device, M: int, N: int, n_x: int, history_size: int = 10,
+           line_search_fn: str = "strong_wolfe") -> Optional[List[int]]:
+    """The tuned configuration of this workload, or None (defaults)."""
+    with _cache_lock:
+        if not _cache:
+            return None
+    return cache_get(workload_key(device, M, N, n_x, history_size, line_search_fn))
+
+
+# ------------------------------------------------------------------ tuning
 def _time(fn: Callable[[], object], reps: int) -> float:
     fn()
     torch.cuda.synchronize()
@@ -56,24 +147,40 @@ def _time(fn: Callable[[], object], reps: int) -> float:
 
 
 def tune_fit(X: torch.Tensor, Y: torch.Tensor, Ops: torch.Tensor, Ch: torch.Tensor, C_init: torch.Tensor,
-             max_iter: int = 100, reps: int = 2, rounds: int = 1, verbose: bool = False,
-             **kw) -> Dict[str, object]:
-    """Coordinate descent over the buckets' variants, timing `optimize` on
-    the given batch (keyword options: history_size, tolerance_grad,
-    line_search_fn, stop_loss as in the fit); leaves the fastest
-    configuration active and returns {config, ms, trials}."""
+             max_iter: int = 100, reps: int = 2, rounds: int = 1, verbose: bool = False, store: bool = True,
+             log: Optional[List[str]] = None, **kw) -> Dict[str, object]:
+    """Coordinate descent over the buckets' variants, timing the device fit
+    on the given batch (keyword options: history_size, tolerance_grad,
+    line_search_fn, stop_loss as in the fit).  Returns {config, ms, trials,
+    key}; with store=True the fastest configuration goes into the workload
+    cache, where `optimize` finds it.  Every trial is appended to `log`
+    (and printed with verbose=True)."""
     from .optimization import _fit
 
     Xc, Yc = X.contiguous(), Y.to(torch.float32).contiguous()
     Oc, Cc = Ops.to(torch.int64).contiguous(), Ch.to(torch.int64).contiguous()
     C0 = C_init.to(torch.float32).contiguous()
     stop = kw.get("stop_loss")
-    run = lambda: _fit(Xc, Yc, Oc, Cc, C0, max_iter, kw.get("history_size", 10), kw.get("tolerance_grad", 1e-7),
-                       kw.get("line_search_fn", "strong_wolfe"), stop_loss=stop)
-    best = get_config()
-    best_t = _time(run, reps)
+    hist = kw.get("history_size", 10)
+    ls = kw.get("line_search_fn", "strong_wolfe")
+
+    def run_with(cfg):
+        return lambda: _fit(Xc, Yc, Oc, Cc, C0, max_iter, hist, kw.get("tolerance_grad", 1e-7), ls,
+                            stop_loss=stop, config=cfg)
+
+    best = default_config()
+    best_t = _time(run_with(best), reps)
     trials = [(list(best), best_t)]
     n_var = [len(v) for v in variants()]
+
+    def note(cfg, t):
+        line = f"[tune_fit] {cfg}: {t * 1e3:.3f} ms"
+        if log is not None:
+            log.append(line)
+        if verbose:
+            print(line, flush=True)
+
+    note(best, best_t)
     for _ in range(rounds):
         for b in range(len(best)):
             for v in range(n_var[b]):
@@ -81,12 +188,12 @@ def tune_fit(X: torch.Tensor, Y: torch.Tensor, Ops: torch.Tensor, Ch: torch.Tens
                     continue
                 cfg = list(best)
                 cfg[b] = v
-                set_config(cfg)
-                t = _time(run, reps)
+                t = _time(run_with(cfg), reps)
                 trials.append((cfg, t))
-                if verbose:
-                    print(f"[tune_fit] {cfg}: {t * 1e3:.2f} ms", flush=True)
+                note(cfg, t)
                 if t < best_t:
                     best, best_t = cfg, t
-            set_config(best)
-    return dict(config=best, ms=best_t * 1e3, trials=trials)
+    key = workload_key(X.device, Ops.shape[1], X.shape[0], X.shape[1], hist, ls)
+    if store:
+        cache_put(key, best, best_t * 1e3)
+    return dict(config=best, ms=best_t * 1e3, trials=trials, key=key)

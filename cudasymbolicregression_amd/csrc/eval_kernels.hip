@@ -360,11 +360,47 @@ __global__ __launch_bounds__(TR_WAVES * 64) void k_fwd_transpose(int M, int64_t 
 // ---------------------------------------------------------------------------
 constexpr int BWD_WAVES = 4;
 
+// The block's per-tile grad_X partial: each wave's variable adjoints at its
+// lane's point (gxl, from the sweep when n_x <= NXREG, else recomputed per
+// chunk of NXREG variables with var_adjoints) go through LDS (gxs), and wave
+// 0 adds the BWD_WAVES waves in wave order into the block's partial row
+// gx_row (N x n_x).  Every wave of the block calls it (it has barriers); the
+// caller puts a barrier between it and the next write of gxs.  MANYX
+// (n_x > NXREG) is a separate instantiation of the kernels, so the common
+// case carries no code (or registers) for the chunked sums.
 template <typename T>
+__device__ inline void flush_gx_chunk(const T (&gxl)[NXREG], T *gxs, int wave, int lane, int64_t n, int64_t N,
+                                      int n_x, int vb, int vn, T *gx_row) {
+#pragma unroll
+    for (int q = 0; q < NXREG; ++q) gxs[(wave * WAVE + lane) * NXREG + q] = gxl[q];
+    __syncthreads();
+    if (wave == 0 && n < N) {
+        for (int v = 0; v < vn; ++v) {
+            T sum = gxs[(0 * WAVE + lane) * NXREG + v];
+            for (int w2 = 1; w2 < BWD_WAVES; ++w2) sum += gxs[(w2 * WAVE + lane) * NXREG + v];
+            gx_row[n * n_x + vb + v] = sum;
+        }
+    }
+}
+template <typename T, bool MANYX, typename Adj>
+__device__ inline void flush_gx(const Program<T> &p, Adj adj, int L, bool active, int n_x, T (&gxl)[NXREG], T *gxs,
+                                int wave, int lane, int64_t n, int64_t N, T *gx_row) {
+    if constexpr (!MANYX) {
+        flush_gx_chunk<T>(gxl, gxs, wave, lane, n, N, n_x, 0, n_x, gx_row);
+    } else {
+        for (int vb = 0; vb < n_x; vb += NXREG) {
+            if (vb > 0) __syncthreads();  // gxs is rewritten
+            var_adjoints<T>(p, adj, L, active, vb, gxl);
+            flush_gx_chunk<T>(gxl, gxs, wave, lane, n, N, n_x, vb, n_x - vb < NXREG ? n_x - vb : NXREG, gx_row);
+        }
+    }
+}
+
+template <typename T, bool MANYX>
 __global__ __launch_bounds__(BWD_WAVES * 64) void k_full_backward(
     T *__restrict__ grad_cache, const T *__restrict__ cache, const int64_t *__restrict__ ops,
     const int64_t *__restrict__ ch, int M, int64_t B, int64_t K, int64_t N, int n_x, T *__restrict__ grad_C,
-    T *__restrict__ gx_partial, T *__restrict__ grad_X_direct, int32_t *err_flag) {
+    T *__restrict__ gx_partial, int32_t *err_flag) {
     // Wave w of block i handles expression e = 4*i + w, i.e. constant set
     // j = e % K of expression b = e / K (K = 1 for the single-set layout).
     extern __shared__ __align__(16) unsigned char smem[];
@@ -375,7 +411,7 @@ __global__ __launch_bounds__(BWD_WAVES * 64) void k_full_backward(
     const int64_t e_id = (int64_t)blockIdx.x * BWD_WAVES + wave;
     const bool bvalid = e_id < B * K;
     const int64_t b = bvalid ? e_id / K : 0, j = bvalid ? e_id % K : 0;
-    const bool want_gx = gx_partial != nullptr || grad_X_direct != nullptr;
+    const bool want_gx = gx_partial != nullptr;
     Program<T> p;
     if (bvalid) p = load_program<T>(ops, ch, (const T *)nullptr, b, 1, B, 1, B, M, n_x);
     else { p.L = 0; p.w0 = p.w1 = 0; p.cmask0 = p.cmask1 = 0; p.root = 0; p.verr = 0; p.c0 = p.c1 = T(0); }
@@ -397,21 +433,12 @@ __global__ __launch_bounds__(BWD_WAVES * 64) void k_full_backward(
         }
         auto val = [&](int k) -> T { return cache[(((int64_t)k * N + nn) * B + b) * K + j]; };
         auto adj = [&](int k) -> T & { return grad_cache[(((int64_t)k * N + nn) * B + b) * K + j]; };
-        int32_t e = backward_tile<T, true>(p, val, adj, p.L, active, acc, cextra, n_x,
-                                           grad_X_direct ? grad_X_direct + nn * n_x : nullptr);
+        int32_t e = backward_tile<T, true>(p, val, adj, p.L, active, acc, cextra, n_x);
         err = e > err ? e : err;
-        if (want_gx && n_x <= NXREG) {
-#pragma unroll
-            for (int q = 0; q < NXREG; ++q) gxs[(wave * WAVE + lane) * NXREG + q] = acc.gxl[q];
-            __syncthreads();
-            if (wave == 0 && n < N) {
-                for (int v = 0; v < n_x; ++v) {
-                    T s = gxs[(0 * WAVE + lane) * NXREG + v];
-                    for (int w2 = 1; w2 < BWD_WAVES; ++w2) s += gxs[(w2 * WAVE + lane) * NXREG + v];
-                    gx_partial[(int64_t)blockIdx.x * N * n_x + n * n_x + v] = s;
-                }
-            }
-            __syncthreads();
+        if (want_gx) {
+            flush_gx<T, MANYX>(p, adj, p.L, active, n_x, acc.gxl, gxs, wave, lane, n, N,
+                               gx_partial + (int64_t)blockIdx.x * N * n_x);
+            __syncthreads();  // gxs is rewritten for the next tile
         }
     }
     if (!bvalid) return;
@@ -516,12 +543,12 @@ __global__ __launch_bounds__(GA_WAVES * 64) void k_bwd_gather(const T *__restric
 // full occupancy (small LDS tapes for short programs, 32 KB a block, were
 // slower); otherwise the other blocks, with LDS tapes loaded from the cache
 // layout.
-template <typename T, bool ROWS>
-__global__ __launch_bounds__(BWD_WAVES * 64) __attribute__((amdgpu_waves_per_eu(ROWS && sizeof(T) == 4 ? BWR_WPE : 1))) void
+template <typename T, bool ROWS, bool MANYX>
+__global__ __launch_bounds__(BWD_WAVES * 64) __attribute__((amdgpu_waves_per_eu(ROWS && !MANYX && sizeof(T) == 4 ? BWR_WPE : 1))) void
 k_full_backward_grp(
     const T *__restrict__ grad_cache, const T *__restrict__ cache, const int64_t *__restrict__ ops,
     const int64_t *__restrict__ ch, int M, int64_t B, int64_t K, int64_t N, int n_x, T *__restrict__ grad_C,
-    T *__restrict__ gx_partial, T *__restrict__ grad_X_direct, int32_t *err_flag, FwdRows fr,
+    T *__restrict__ gx_partial, int32_t *err_flag, FwdRows fr,
     const T *__restrict__ vrows) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int Ls[BWD_WAVES];
@@ -545,7 +572,7 @@ k_full_backward_grp(
     const int64_t e_id = e0 + wave;
     const bool bvalid = e_id < E;
     const int64_t b = bvalid ? e_id / K : 0, j = bvalid ? e_id % K : 0;
-    const bool want_gx = gx_partial != nullptr || grad_X_direct != nullptr;
+    const bool want_gx = gx_partial != nullptr;
     Program<T> p;
     if (bvalid) p = load_program<T>(ops, ch, (const T *)nullptr, b, 1, B, 1, B, M, n_x);
     else { p.L = 0; p.w0 = p.w1 = 0; p.cmask0 = p.cmask1 = 0; p.root = 0; p.verr = 0; p.c0 = p.c1 = T(0); }
@@ -610,24 +637,32 @@ k_full_backward_grp(
             const T *vr = vrows + rbase + nn;
             auto val = [&](int k) -> T { return vr[(int64_t)(k < Lr ? k : Lr - 1) * N]; };
             auto adj = [&](int k) -> T & { return gr[(int64_t)(k < Lr ? k : Lr - 1) * N]; };
-            e = backward_tile<T, true>(p, val, adj, Lr, active, acc, cextra, n_x,
-                                       grad_X_direct ? grad_X_direct + nn * n_x : nullptr);
+            e = backward_tile<T, true>(p, val, adj, Lr, active, acc, cextra, n_x);
+            if constexpr (MANYX)
+                if (want_gx)
+                    flush_gx<T, MANYX>(p, adj, Lr, active, n_x, acc.gxl, gxs, wave, lane, n, N,
+                                       gx_partial + blk * N * n_x);
         } else {
             auto val = [&](int k) -> T { return vt[(size_t)k * WAVE + lane]; };
             auto adj = [&](int k) -> T & { return at[(size_t)k * WAVE + lane]; };
-            e = backward_tile<T, true>(p, val, adj, p.L, active, acc, cextra, n_x,
-                                       grad_X_direct ? grad_X_direct + nn * n_x : nullptr);
+            e = backward_tile<T, true>(p, val, adj, p.L, active, acc, cextra, n_x);
+            if constexpr (MANYX)
+                if (want_gx)
+                    flush_gx<T, MANYX>(p, adj, p.L, active, n_x, acc.gxl, gxs, wave, lane, n, N,
+                                       gx_partial + blk * N * n_x);
         }
         err = e > err ? e : err;
-        if (want_gx && n_x <= NXREG) {
+        if constexpr (!MANYX) {
+            if (want_gx) {
 #pragma unroll
-            for (int q = 0; q < NXREG; ++q) gxs[(wave * WAVE + lane) * NXREG + q] = acc.gxl[q];
-            __syncthreads();
-            if (wave == 0 && n < N) {
-                for (int v = 0; v < n_x; ++v) {
-                    T s = gxs[(0 * WAVE + lane) * NXREG + v];
-                    for (int w2 = 1; w2 < BWD_WAVES; ++w2) s += gxs[(w2 * WAVE + lane) * NXREG + v];
-                    gx_partial[blk * N * n_x + n * n_x + v] = s;
+                for (int q = 0; q < NXREG; ++q) gxs[(wave * WAVE + lane) * NXREG + q] = acc.gxl[q];
+                __syncthreads();
+                if (wave == 0 && n < N) {
+                    for (int v = 0; v < n_x; ++v) {
+                        T sum = gxs[(0 * WAVE + lane) * NXREG + v];
+                        for (int w2 = 1; w2 < BWD_WAVES; ++w2) sum += gxs[(w2 * WAVE + lane) * NXREG + v];
+                        gx_partial[blk * N * n_x + n * n_x + v] = sum;
+                    }
                 }
             }
         }
@@ -754,7 +789,6 @@ int backward_mb(T *grad_out, const T *cache, const int64_t *ops, const int64_t *
     if (M <= 0 || M > 128 || N < 0 || B < 0 || n_x < 0 || K <= 0 || rows_bytes < 0) return -1;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, d_err, 0, (int64_t)1);
     int64_t nblk = (B * K + BWD_WAVES - 1) / BWD_WAVES;
-    const bool gx_regs = n_x <= NXREG;
     if (grad_X) {
         int64_t len = N * n_x;
         if (len > 0) {
@@ -779,7 +813,7 @@ int backward_mb(T *grad_out, const T *cache, const int64_t *ops, const int64_t *
         return ret_last();
     }
     T *gx_partial = nullptr;
-    if (grad_X && gx_regs) {
+    if (grad_X) {
         if (ws_bytes < backward_mb_ws_bytes<T>(B, N, n_x, K)) return -2;
         gx_partial = (T *)workspace;
     }
@@ -807,21 +841,22 @@ int backward_mb(T *grad_out, const T *cache, const int64_t *ops, const int64_t *
             hipLaunchKernelGGL(k_bwd_gather<T>, dim3((unsigned)(G * n_tiles)), dim3(GA_WAVES * 64), 0, s, grad_out,
                                cache, (int)M, N, E, G, fr, vrows, d_err);
         }
+        const bool manyx = n_x > NXREG;
         if (fr.cap >= 0)  // the gathered groups
-            hipLaunchKernelGGL((k_full_backward_grp<T, true>), dim3((unsigned)nblk), dim3(BWD_WAVES * 64),
-                               backward_grp_lds<T>(M, true), s, grad_out, cache, ops, ch, (int)M, B, K, N, (int)n_x,
-                               grad_C, gx_partial, (grad_X && !gx_regs) ? grad_X : nullptr, d_err, fr,
+            hipLaunchKernelGGL(manyx ? (k_full_backward_grp<T, true, true>) : (k_full_backward_grp<T, true, false>),
+                               dim3((unsigned)nblk), dim3(BWD_WAVES * 64), backward_grp_lds<T>(M, true), s, grad_out,
+                               cache, ops, ch, (int)M, B, K, N, (int)n_x, grad_C, gx_partial, d_err, fr,
                                (const T *)vrows);
         // the others (all of them without a row workspace)
-        hipLaunchKernelGGL((k_full_backward_grp<T, false>), dim3((unsigned)nblk), dim3(BWD_WAVES * 64),
-                           backward_grp_lds<T>(M), s, grad_out, cache, ops, ch, (int)M, B, K, N, (int)n_x, grad_C,
-                           gx_partial, (grad_X && !gx_regs) ? grad_X : nullptr, d_err, fr, (const T *)vrows);
+        hipLaunchKernelGGL(manyx ? (k_full_backward_grp<T, false, true>) : (k_full_backward_grp<T, false, false>),
+                           dim3((unsigned)nblk), dim3(BWD_WAVES * 64), backward_grp_lds<T>(M), s, grad_out, cache,
+                           ops, ch, (int)M, B, K, N, (int)n_x, grad_C, gx_partial, d_err, fr, (const T *)vrows);
     } else {
         size_t lds = (size_t)BWD_WAVES * M * sizeof(double) + (size_t)BWD_WAVES * WAVE * NXREG * sizeof(T);
         if (lds > lds_limit()) return -3;
-        hipLaunchKernelGGL(k_full_backward<T>, dim3((unsigned)nblk), dim3(BWD_WAVES * 64), lds, s, grad_out, cache,
-                           ops, ch, (int)M, B, K, N, (int)n_x, grad_C, gx_partial,
-                           (grad_X && !gx_regs) ? grad_X : nullptr, d_err);
+        hipLaunchKernelGGL(n_x > NXREG ? (k_full_backward<T, true>) : (k_full_backward<T, false>), dim3((unsigned)nblk),
+                           dim3(BWD_WAVES * 64), lds, s, grad_out, cache, ops, ch, (int)M, B, K, N, (int)n_x, grad_C,
+                           gx_partial, d_err);
     }
     if (gx_partial) {
         const int64_t len = N * n_x;

@@ -44,6 +44,7 @@
 #include "sr_device_info.h"
 #include "sr_queue.h"
 #include "sr_streams.h"
+#include "sr_eval.h"
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -804,9 +805,9 @@ int grid_for(int hist) {
 }
 
 // Bucket kernels <LT, NX, SL, W, MEM, WPE, PPT>.  Each bucket has a few
-// launch variants (waves per problem W x points per lane PPT); one is active
-// per bucket and process, chosen by sr_fit_set_variant (the autotuner) or the
-// built-in default.
+// launch variants (waves per problem W x points per lane PPT); each call
+// names the one it runs per bucket (sr_fit_options, from the autotuner) or
+// takes the built-in default.
 constexpr int NXR = NXR_ROWS;  // variables in tape rows (the rest are read from X)
 constexpr int NBUCKETS = 5;
 static_assert(NBUCKETS == NB_CLASSIFY, "k_classify and the launch plan disagree on the buckets");
@@ -847,13 +848,13 @@ const FitVariant kB4[] = {{launch_variant<256, 2, 1, true, 2, 1>, 1, 1}};
 
 const FitVariant *const kVariants[NBUCKETS] = {kB0, kB1, kB2, kB3, kB4};
 constexpr int kNumVariants[NBUCKETS] = {5, 4, 4, 4, 1};
-int g_variant[NBUCKETS] = {2, 2, 2, 0, 0};  // fastest on the bench workload (autotune.tune_fit)
-int g_stationary_exit = 1;
+// Fastest on the benchmark workload (autotune.tune_fit); constant.
+constexpr int kDefaultVariant[NBUCKETS] = {2, 2, 2, 0, 0};
 
 // The selected variant, or the bucket's first one when the selected one
 // needs more LDS than a workgroup may have at this history size.
-inline int launch_bucket(int k, const FitArgs &a, hipStream_t s) {
-    const int r = kVariants[k][g_variant[k]].launch(a, s);
+inline int launch_bucket(int k, const int *variant, const FitArgs &a, hipStream_t s) {
+    const int r = kVariants[k][variant[k]].launch(a, s);
     return r == -3 ? kVariants[k][0].launch(a, s) : r;
 }
 
@@ -877,16 +878,8 @@ int64_t sr_lbfgs_workspace_bytes(int64_t B, int64_t M) { return fit_ws(B, M); }
 // Launch-variant control of the fit kernels (see include/sr_eval.h).
 int sr_fit_num_buckets(void) { return NBUCKETS; }
 int sr_fit_num_variants(int bucket) { return (bucket < 0 || bucket >= NBUCKETS) ? -1 : kNumVariants[bucket]; }
-int sr_fit_get_variant(int bucket) { return (bucket < 0 || bucket >= NBUCKETS) ? -1 : g_variant[bucket]; }
-int sr_fit_set_stationary_exit(int on) {
-    const int prev = g_stationary_exit;
-    g_stationary_exit = on ? 1 : 0;
-    return prev;
-}
-int sr_fit_set_variant(int bucket, int variant) {
-    if (bucket < 0 || bucket >= NBUCKETS || variant < 0 || variant >= kNumVariants[bucket]) return -1;
-    g_variant[bucket] = variant;
-    return 0;
+int sr_fit_default_variant(int bucket) {
+    return (bucket < 0 || bucket >= NBUCKETS) ? -1 : kDefaultVariant[bucket];
 }
 int sr_fit_variant_info(int bucket, int variant, int *waves, int *ppt) {
     if (bucket < 0 || bucket >= NBUCKETS || variant < 0 || variant >= kNumVariants[bucket]) return -1;
@@ -901,11 +894,19 @@ int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const i
                  const float *C_init, int64_t M, int64_t B, const int64_t *x_off, const int64_t *y_off,
                  const int32_t *n_pts, const double *stop_loss, float *C_out, double *loss_out, int32_t *iters_out,
                  int32_t *nclos_out, int32_t *err_out, int32_t *flags, int max_iter, int history, double tol_grad,
-                 int line_search, int ls_max_iter, double eps, void *workspace, int64_t ws_bytes, void *stream) {
+                 int line_search, int ls_max_iter, double eps, const sr_fit_options *opts, void *workspace,
+                 int64_t ws_bytes, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     if (M <= 0 || M > 128 || B < 0 || n_x < 0 || n_x > MAX_NX_CODE + 1 || history < 1 || history > MHMAX ||
         max_iter < 0 || ls_max_iter < 0 || (line_search != 0 && line_search != 1))
         return -1;
+    int variant[NBUCKETS];
+    for (int k = 0; k < NBUCKETS; ++k) {
+        const int v = opts ? opts->variant[k] : -1;
+        if (v >= kNumVariants[k]) return -1;
+        variant[k] = v < 0 ? kDefaultVariant[k] : v;
+    }
+    const int stationary_exit = opts ? (opts->stationary_exit != 0) : 1;
     if (n_pts == nullptr && N <= 0) return -1;
     if (ws_bytes < sr_lbfgs_workspace_bytes(B, M)) return -2;
     hipLaunchKernelGGL(k_fill_i32_f, dim3(1), dim3(64), 0, s, flags, 0, (int64_t)2);
@@ -926,7 +927,7 @@ int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const i
     a.flags = flags;
     a.max_iter = max_iter; a.history = history; a.tol_grad = tol_grad; a.line_search = line_search;
     a.ls_max_iter = ls_max_iter; a.eps = eps;
-    a.stationary_exit = g_stationary_exit;
+    a.stationary_exit = stationary_exit;
     a.gtape = M > 32 ? (float *)(ws + fit_gt_off(B)) : nullptr;
     auto bucket_args = [&](int k) {
         FitArgs r = a;
@@ -939,28 +940,22 @@ int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const i
     if (r) return r;
     // A program has at most 2 M - 1 tape rows and M live slots, so short
     // rows only need the first buckets.
-    if (M <= 4) return launch_bucket(0, bucket_args(0), s);
+    if (M <= 4) return launch_bucket(0, variant, bucket_args(0), s);
     // Buckets 4+3, 2 and 1 on side streams, bucket 0 on the caller's stream,
-    // joined back before returning.
-    Fork &f = fork_for_device();
-    if (!f.ok) return (int)hipErrorOutOfMemory;
-    if ((r = (int)hipEventRecord(f.ev_fork, s))) return r;
-    for (int i = 0; i < 3; ++i)
-        if ((r = (int)hipStreamWaitEvent(f.side[i], f.ev_fork, 0))) return r;
+    // joined back before returning (also after a failed launch).
+    ForkScope fk(s);
+    if ((r = fk.fork())) return fk.finish(r);
     // Buckets 4 and 3 share a stream.  With M <= 64, bucket 4 only takes
     // programs with extra adjoint rows (shared or unused nodes, rare), so it
     // goes second: a persistent grid that is not yet resident holds back the
     // next kernel on its stream.
-    if (M > 64 && (r = launch_bucket(4, bucket_args(4), f.side[0]))) return r;
-    if (M > 16 && (r = launch_bucket(3, bucket_args(3), f.side[0]))) return r;
-    if (M > 32 && M <= 64 && (r = launch_bucket(4, bucket_args(4), f.side[0]))) return r;
-    if (M > 8 && (r = launch_bucket(2, bucket_args(2), f.side[1]))) return r;
-    if ((r = launch_bucket(1, bucket_args(1), f.side[2]))) return r;
-    if ((r = launch_bucket(0, bucket_args(0), s))) return r;
-    for (int i = 0; i < 3; ++i) {
-        if ((r = (int)hipEventRecord(f.ev_join[i], f.side[i]))) return r;
-        if ((r = (int)hipStreamWaitEvent(s, f.ev_join[i], 0))) return r;
-    }
+    if (M > 64 && (r = launch_bucket(4, variant, bucket_args(4), fk.side(0)))) return fk.finish(r);
+    if (M > 16 && (r = launch_bucket(3, variant, bucket_args(3), fk.side(0)))) return fk.finish(r);
+    if (M > 32 && M <= 64 && (r = launch_bucket(4, variant, bucket_args(4), fk.side(0)))) return fk.finish(r);
+    if (M > 8 && (r = launch_bucket(2, variant, bucket_args(2), fk.side(1)))) return fk.finish(r);
+    if ((r = launch_bucket(1, variant, bucket_args(1), fk.side(2)))) return fk.finish(r);
+    if ((r = launch_bucket(0, variant, bucket_args(0), s))) return fk.finish(r);
+    if ((r = fk.finish(0))) return r;
     return (int)hipGetLastError();
 }
 

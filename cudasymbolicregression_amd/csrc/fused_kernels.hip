@@ -18,9 +18,14 @@
 // evaluated over tile groups of 64*PPT points with the tape in LDS (values,
 // and adjoints in place).  Expressions are bucketed by tape rows (<= 8,
 // <= 16, <= 32, <= 64, the rest) so each bucket's kernel is sized for its
-// tape; the buckets run concurrently on forked streams.  Variable adjoints are summed per block in LDS (float atomics)
-// and written as one partial row per block; two final kernels add the rows
-// in FP64 in a fixed order.
+// tape; the buckets run concurrently on forked streams.  Expressions come
+// from a shared work counter, except when grad_X is wanted: then every wave
+// takes a fixed, cost-balanced segment of its bucket list (sr_queue.h) and
+// sums its variable adjoints into a wave-private partial row (in LDS, or in
+// the workspace when the rows do not fit), with plain adds in a fixed order.
+// A block adds its waves' LDS rows in wave order into one partial row; two
+// final kernels add all partial rows in FP64 in row order.  No float
+// atomics: grad_X is the same bit for bit on every call.
 #include "sr_code.h"
 #include "sr_device_info.h"
 #include "sr_queue.h"
@@ -36,7 +41,12 @@ namespace {
 constexpr int F_NCREG = 8;  // constants accumulated in per-lane FP64 registers
 constexpr int F_NX = 4;     // variables held in registers per point
 constexpr int F_NBUCKETS = 5;
-constexpr size_t GX_LDS_MAX2 = 24 * 1024;  // largest per-block grad_X partial kept in LDS
+// Wave-private grad_X rows of a block are kept in LDS only when they are
+// small: larger rows would cut the waves per CU (LDS rows of 12 KB per wave
+// made the 65536 x 1024 x 3 evaluation 27% slower than rows in the
+// workspace, which stay mostly in the caches).
+constexpr size_t GX_LDS_ROWS_MAX = 8 * 1024;
+constexpr size_t GX_GLOBAL_BUDGET = 256u << 20;  // wave-private rows in the workspace, at most
 // Expressions per work-queue ticket: short programs are cheap enough that one
 // atomic per expression would serialise the queue; longer ones take one each
 // so that the last expressions spread over all waves.
@@ -55,9 +65,10 @@ struct FArgs {
     void *y_out;        // (B,N) or nullptr
     double *loss_out;   // (B) MODE 1, or nullptr
     void *grad_C;       // (B,M) unless fwd_only
-    void *gx_partial;   // [rows][N*n_x] per-block partials, or nullptr
+    void *gx_partial;   // [rows][N*n_x] partial rows (grad_X wanted), or nullptr
     int64_t gx_row0;    // first partial row of this launch
-    void *grad_X_direct;  // atomics target when partial rows are not used (nullable)
+    int gx_lds;         // 1: wave rows in LDS, one partial row per block; 0: one per wave
+    const int32_t *seg; // static segments of this bucket's list (grad_X wanted), else nullptr
     int32_t *err_b;     // (B) or nullptr
     int32_t *flags;     // [0] validation (max), [1] backward (max)
     const int32_t *list;
@@ -88,32 +99,46 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
     const int n_x = a.n_x;
     const int64_t gx_len = N * n_x;
     const bool want_gx = a.want_gx != 0 && !a.fwd_only;
-    const bool gx_lds = want_gx && a.gx_partial != nullptr;
-    T *gx_blk = reinterpret_cast<T *>(smem);  // [N*n_x] when gx_lds
-    const size_t gx_bytes = gx_lds ? (((size_t)gx_len * sizeof(T) + 15) / 16 * 16) : 0;
+    const bool gx_lds = want_gx && a.gx_lds;
+    const size_t gx_bytes = gx_lds ? (((size_t)WPB * gx_len * sizeof(T) + 15) / 16 * 16) : 0;
     Tape<T, PPT> t;
     {
         T *slot = MEM ? (T *)a.gtape + ((size_t)blockIdx.x * WPB + wave) * tape_elems(LT, PPT)
                       : reinterpret_cast<T *>(smem + gx_bytes) + (size_t)wave * tape_elems(LT, PPT);
         t.v = slot;
     }
-    if (gx_lds)
-        for (int64_t i = threadIdx.x; i < gx_len; i += WPB * WAVE) gx_blk[i] = T(0);
-    __syncthreads();
+    // this wave's grad_X partial row; each element is only ever touched by
+    // the lane that owns its point (n = lane mod 64), zeroing included
+    T *gxrow = nullptr;
+    if (want_gx) {
+        gxrow = gx_lds ? reinterpret_cast<T *>(smem) + (size_t)wave * gx_len
+                      : (T *)a.gx_partial + (a.gx_row0 + (int64_t)blockIdx.x * WPB + wave) * gx_len;
+        for (int64_t nz = lane; nz < N; nz += WAVE)
+            for (int v = 0; v < n_x; ++v) gxrow[nz * n_x + v] = T(0);
+    }
     const int64_t count = (int64_t)*a.list_count;
     const T *const X0 = X;
-
-    // expressions are fetched CH at a time: one atomic per chunk
+    // work: a fixed segment of the list (grad_X wanted), else chunks of CH
+    // expressions from the shared counter (one atomic per chunk)
+    const int64_t gw = (int64_t)blockIdx.x * WPB + wave;
+    int32_t seg_i = a.seg ? uniform_i32(a.seg[gw]) : 0;
+    const int32_t seg_e = a.seg ? uniform_i32(a.seg[gw + 1]) : 0;
     constexpr int CH = fetch_chunk(LT);
     int32_t chunk = 0, left = 0;
     for (;;) {
-        if (left == 0) {
-            chunk = wave_fetch_add_n(a.work_counter, CH);
-            left = CH;
+        int i;
+        if (a.seg) {
+            if (seg_i >= seg_e) break;
+            i = seg_i++;
+        } else {
+            if (left == 0) {
+                chunk = wave_fetch_add_n(a.work_counter, CH);
+                left = CH;
+            }
+            i = chunk + (CH - left);
+            --left;
+            if (i >= count) break;
         }
-        const int i = chunk + (CH - left);
-        --left;
-        if (i >= count) break;
         const int64_t b = (int64_t)a.list[i];
         const CProg<T> p = load_cprog<T>(a.ops, a.ch, (const T *)a.C, b, a.M, n_x);
         if (p.verr) {
@@ -344,10 +369,8 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
                                 const uint32_t w = c.word_lt<LT>(ii);
                                 const int v = desc_idx(k == 0 ? instr_d0(w) : instr_d1(w));
 #pragma unroll
-                                for (int q = 0; q < PPT; ++q) {
-                                    const T gvx = (act[q] && add[q]) ? go[q] : T(0);
-                                    if (gvx != T(0)) atomicAdd(&((T *)a.grad_X_direct)[n[q] * n_x + v], gvx);
-                                }
+                                for (int q = 0; q < PPT; ++q)
+                                    if (act[q] && add[q] && go[q] != T(0)) gxrow[n[q] * n_x + v] += go[q];
                             }
                         }
                     } else if (mode == PM_NOOP) {
@@ -392,8 +415,7 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
 #pragma unroll
                         for (int u = 0; u < F_NX; ++u) gxv = (u == v) ? gxr[q][u] : gxv;
                         if (gxv == T(0)) continue;
-                        if (gx_lds) atomicAdd(&gx_blk[n[q] * n_x + v], gxv);
-                        else atomicAdd(&((T *)a.grad_X_direct)[n[q] * n_x + v], gxv);
+                        gxrow[n[q] * n_x + v] += gxv;
                     }
                 }
             }
@@ -431,25 +453,36 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
             if (a.err_b) a.err_b[b] = e;
         }
     }
-    __syncthreads();
-    if (gx_lds) {
+    if (gx_lds) {  // the block's partial row: its waves' rows added in wave order
+        __syncthreads();
+        const T *rows = reinterpret_cast<const T *>(smem);
         T *dst = (T *)a.gx_partial + (a.gx_row0 + blockIdx.x) * gx_len;
-        for (int64_t i = threadIdx.x; i < gx_len; i += WPB * WAVE) dst[i] = gx_blk[i];
+        for (int64_t i = threadIdx.x; i < gx_len; i += WPB * WAVE) {
+            T v = rows[i];
+            for (int w = 1; w < WPB; ++w) v += rows[(size_t)w * gx_len + i];
+            dst[i] = v;
+        }
     }
 }
 
 // Buckets by tape rows (count_rows_wave) and constant count nc: rows <= 8,
 // <= 16, <= 32, <= 64 (all with nc <= 64), the rest.  One wave per
 // expression; the run counts of the deterministic bucket lists follow in
-// k_queue_count (sr_queue.h).
+// k_queue_count (sr_queue.h).  cost_of (nullable): the expression's share of
+// its bucket's work for the static segments, rows + 2 (the per-expression
+// set-up costs about two instructions).
 __global__ __launch_bounds__(256) void k_classify_fused(const int64_t *__restrict__ ops, const int64_t *__restrict__ ch,
-                                                        int M, int64_t B, int n_x, int8_t *bucket_of) {
+                                                        int M, int64_t B, int n_x, int8_t *bucket_of,
+                                                        int32_t *cost_of) {
     const int64_t b = (int64_t)blockIdx.x * 4 + uniform_i32((int)(threadIdx.x / WAVE));
     if (b >= B) return;
     int nc = 0;
     const int nr = count_rows_wave(ops + b * M, ch + b * M * 2, M, n_x, nc);
     const int bucket = nc > 64 ? 4 : (nr <= 8 ? 0 : (nr <= 16 ? 1 : (nr <= 32 ? 2 : (nr <= 64 ? 3 : 4))));
-    if (lane_id() == 0) bucket_of[b] = (int8_t)bucket;
+    if (lane_id() == 0) {
+        bucket_of[b] = (int8_t)bucket;
+        if (cost_of) cost_of[b] = nr + 2;
+    }
 }
 
 __global__ void k_fill_i32_e(int32_t *p, int32_t v, int64_t n) {
@@ -494,26 +527,46 @@ template <> struct FCfg<double> {
     static constexpr int B0_PPT = 1;
 };
 
+// grad_X rows of one bucket's launch: mode 0 none, 1 wave rows in LDS (one
+// partial row per block), 2 wave rows in the workspace (one per wave).
+template <typename T, int WPB>
+int gx_mode(bool want_gx, int64_t gx_len) {
+    if (!want_gx) return 0;
+    return (size_t)WPB * gx_len * sizeof(T) <= GX_LDS_ROWS_MAX ? 1 : 2;
+}
+
 template <typename T, int LT, int SL, int PPT, int WPB, bool MEM>
-size_t fused_lds(bool gx_lds, int64_t gx_len) {
-    return (gx_lds ? (((size_t)gx_len * sizeof(T) + 15) / 16 * 16) : 0) +
+size_t fused_lds(int mode, int64_t gx_len) {
+    return (mode == 1 ? (((size_t)WPB * gx_len * sizeof(T) + 15) / 16 * 16) : 0) +
            (MEM ? 0 : (size_t)WPB * tape_elems(LT, PPT) * sizeof(T));
 }
 
+// Persistent grid of one bucket: as many blocks as the CUs hold at once; with
+// wave rows in the workspace, few enough that the rows stay within budget.
 template <typename T, int LT, int SL, int PPT, int WPB, bool MEM>
-int fused_grid(bool gx_lds, int64_t gx_len, int n_x) {
+int fused_grid(int mode, int64_t gx_len, int n_x) {
     int per_cu = 0;
-    const size_t lds = fused_lds<T, LT, SL, PPT, WPB, MEM>(gx_lds, gx_len);
-    const void *fn = n_x <= F_NX ? (const void *)k_fused_code<T, LT, SL, PPT, WPB, MEM, false>
-                                 : (const void *)k_fused_code<T, LT, SL, PPT, WPB, MEM, true>;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, WPB * 64, lds) != hipSuccess || per_cu <= 0)
-        per_cu = 1;
-    return num_cus() * per_cu;
+    if (MEM) {
+        per_cu = 2;
+    } else {
+        const size_t lds = fused_lds<T, LT, SL, PPT, WPB, MEM>(mode, gx_len);
+        const void *fn = n_x <= F_NX ? (const void *)k_fused_code<T, LT, SL, PPT, WPB, MEM, false>
+                                     : (const void *)k_fused_code<T, LT, SL, PPT, WPB, MEM, true>;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, WPB * 64, lds) != hipSuccess || per_cu <= 0)
+            per_cu = 1;
+    }
+    int64_t grid = (int64_t)num_cus() * per_cu;
+    if (mode == 2) {
+        const int64_t row_bytes = gx_len * (int64_t)sizeof(T) * WPB;
+        const int64_t cap = (int64_t)GX_GLOBAL_BUDGET / (row_bytes > 0 ? row_bytes : 1);
+        grid = grid < cap ? grid : (cap > 1 ? cap : 1);
+    }
+    return (int)grid;
 }
 
 template <typename T, int LT, int SL, int PPT, int WPB, bool MEM>
-int fused_launch(const FArgs &a, int grid, bool gx_lds, hipStream_t s) {
-    const size_t lds = fused_lds<T, LT, SL, PPT, WPB, MEM>(gx_lds, a.N * a.n_x);
+int fused_launch(const FArgs &a, int grid, int mode, hipStream_t s) {
+    const size_t lds = fused_lds<T, LT, SL, PPT, WPB, MEM>(mode, a.N * a.n_x);
     if (lds > lds_limit_device()) return -3;
     if (a.n_x <= F_NX)
         hipLaunchKernelGGL((k_fused_code<T, LT, SL, PPT, WPB, MEM, false>), dim3((unsigned)grid), dim3(WPB * 64), lds,
@@ -529,39 +582,57 @@ int fused_launch(const FArgs &a, int grid, bool gx_lds, hipStream_t s) {
 #define SR_F2(T) T, 32, 1, 1, 2, false
 #define SR_F3(T) T, 64, 1, 1, 2, false
 #define SR_F4(T) T, 256, 2, 1, 2, true
+constexpr int F_WPB[F_NBUCKETS] = {4, 4, 2, 2, 2};
 
-constexpr int F4_BLOCKS_PER_CU = 2;
-
-// Workspace: lists [4][B] int32, counters [4]+[4] | grad_X partial rows |
-// FP64 chunk sums of the rows | global tape (bucket 3).
+// Workspace: lists [5][B] int32, counters [5]+[5] | queue | segment starts
+// and segment queue workspace (grad_X wanted) | grad_X partial rows | FP64
+// chunk sums of the rows | global tape (bucket 4).
 struct FWs {
-    size_t q_off, gx_off, g64_off, gt_off, total;
-    int g0, g1, g2, g3, g4;
-    bool gx_lds;
+    size_t q_off, seg_off, sq_off, gx_off, g64_off, gt_off, total;
+    int grid[F_NBUCKETS];   // blocks per bucket launch (0: not launched)
+    int mode[F_NBUCKETS];   // grad_X row mode per bucket
+    int64_t row0[F_NBUCKETS];  // first partial row per bucket
+    int64_t rows;           // partial rows in all
+    int64_t W[F_NBUCKETS];  // waves per bucket launch (segments)
 };
 template <typename T>
 FWs fused_layout(int64_t N, int64_t n_x, int64_t B, int64_t M, bool want_gx) {
     FWs w;
     const int64_t gx_len = N * n_x;
-    w.gx_lds = want_gx && n_x <= F_NX && (size_t)gx_len * sizeof(T) <= GX_LDS_MAX2;
-    w.g0 = fused_grid<SR_F0(T)>(w.gx_lds, gx_len, (int)n_x);
+    const int ni = (int)n_x;
+    w.mode[0] = gx_mode<T, 4>(want_gx, gx_len);
+    w.mode[1] = gx_mode<T, 4>(want_gx, gx_len);
+    w.mode[2] = gx_mode<T, 2>(want_gx, gx_len);
+    w.mode[3] = gx_mode<T, 2>(want_gx, gx_len);
+    w.mode[4] = gx_mode<T, 2>(want_gx, gx_len);
+    w.grid[0] = fused_grid<SR_F0(T)>(w.mode[0], gx_len, ni);
     // a program of M nodes needs at most 2 M - 1 tape rows
-    w.g1 = M > 4 ? fused_grid<SR_F1(T)>(w.gx_lds, gx_len, (int)n_x) : 0;
-    w.g2 = M > 8 ? fused_grid<SR_F2(T)>(w.gx_lds, gx_len, (int)n_x) : 0;
-    w.g3 = M > 16 ? fused_grid<SR_F3(T)>(w.gx_lds, gx_len, (int)n_x) : 0;
-    w.g4 = M > 32 ? num_cus() * F4_BLOCKS_PER_CU : 0;
-    const int64_t rows = (int64_t)w.g0 + w.g1 + w.g2 + w.g3 + w.g4;
+    w.grid[1] = M > 4 ? fused_grid<SR_F1(T)>(w.mode[1], gx_len, ni) : 0;
+    w.grid[2] = M > 8 ? fused_grid<SR_F2(T)>(w.mode[2], gx_len, ni) : 0;
+    w.grid[3] = M > 16 ? fused_grid<SR_F3(T)>(w.mode[3], gx_len, ni) : 0;
+    w.grid[4] = M > 32 ? fused_grid<SR_F4(T)>(w.mode[4], gx_len, ni) : 0;
+    w.rows = 0;
+    for (int k = 0; k < F_NBUCKETS; ++k) {
+        w.W[k] = (int64_t)w.grid[k] * F_WPB[k];
+        w.row0[k] = w.rows;
+        w.rows += w.mode[k] == 1 ? w.grid[k] : (w.mode[k] == 2 ? w.W[k] : 0);
+    }
     size_t off = (((size_t)F_NBUCKETS * B * 4 + 64) + 255) / 256 * 256;
     w.q_off = off;
     off += (queue_ws_bytes(B, F_NBUCKETS) + 255) / 256 * 256;
+    w.seg_off = off;
+    if (want_gx)
+        for (int k = 0; k < F_NBUCKETS; ++k) off += ((size_t)(w.W[k] + 1) * 4 + 255) / 256 * 256;
+    w.sq_off = off;
+    if (want_gx) off += (queue_seg_ws_bytes(B, F_NBUCKETS) + 255) / 256 * 256;
     w.gx_off = off;
-    if (w.gx_lds) off += (size_t)rows * gx_len * sizeof(T);
+    if (want_gx) off += (size_t)w.rows * gx_len * sizeof(T);
     off = (off + 255) / 256 * 256;
     w.g64_off = off;
-    if (w.gx_lds) off += (size_t)((rows + ROW_CHUNK - 1) / ROW_CHUNK) * gx_len * sizeof(double);
+    if (want_gx) off += (size_t)((w.rows + ROW_CHUNK - 1) / ROW_CHUNK) * gx_len * sizeof(double);
     off = (off + 255) / 256 * 256;
     w.gt_off = off;
-    if (w.g4) off += (size_t)w.g4 * 2 * tape_elems(256, 1) * sizeof(T);
+    if (w.grid[4]) off += (size_t)w.grid[4] * 2 * tape_elems(256, 1) * sizeof(T);
     w.total = off + 256;
     return w;
 }
@@ -584,15 +655,8 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
     if (workspace_bytes < (int64_t)L.total) return -2;
     hipLaunchKernelGGL(k_fill_i32_e, dim3(1), dim3(64), 0, s, flags, 0, (int64_t)2);
     const int64_t gx_len = N * n_x;
-    if (want_gx && !L.gx_lds) {  // direct atomics into grad_X
-        const int64_t nb = (gx_len + 255) / 256;
-        if (sizeof(T) == 4)
-            hipLaunchKernelGGL(k_fill_i32_e, dim3((unsigned)nb), dim3(256), 0, s, (int32_t *)grad_X, 0, gx_len);
-        else
-            hipLaunchKernelGGL(k_fill_u64_e, dim3((unsigned)nb), dim3(256), 0, s, (uint64_t *)grad_X, 0ull, gx_len);
-    }
     if (B == 0) {
-        if (want_gx && L.gx_lds) {
+        if (want_gx) {
             const int64_t nb = (gx_len + 255) / 256;
             if (sizeof(T) == 4)
                 hipLaunchKernelGGL(k_fill_i32_e, dim3((unsigned)nb), dim3(256), 0, s, (int32_t *)grad_X, 0, gx_len);
@@ -607,57 +671,67 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
     int32_t *counters = (int32_t *)(ws + (size_t)F_NBUCKETS * B * 4);  // [0..4] counts, [5..9] work
     hipLaunchKernelGGL(k_fill_i32_e, dim3(1), dim3(64), 0, s, counters, 0, (int64_t)(2 * F_NBUCKETS));
     void *qws = ws + L.q_off;
+    QueueSegs segs{};
+    if (want_gx) {
+        size_t o = L.seg_off;
+        for (int k = 0; k < F_NBUCKETS; ++k) {
+            segs.seg[k] = (int32_t *)(ws + o);
+            segs.W[k] = L.W[k];
+            o += ((size_t)(L.W[k] + 1) * 4 + 255) / 256 * 256;
+        }
+    }
+    void *sqws = want_gx ? (void *)(ws + L.sq_off) : nullptr;
     hipLaunchKernelGGL(k_classify_fused, dim3((unsigned)((B + 3) / 4)), dim3(256), 0, s, ops, ch, (int)M, B,
-                       (int)n_x, queue_bucket_of(qws));
-    queue_finish(qws, B, F_NBUCKETS, lists, counters, s);
+                       (int)n_x, queue_bucket_of(qws), want_gx ? queue_cost_of(sqws) : (int32_t *)nullptr);
+    queue_finish(qws, B, F_NBUCKETS, lists, counters, s, want_gx ? &segs : nullptr, sqws);
     FArgs a;
     memset(&a, 0, sizeof(a));
     a.X = X; a.N = N; a.n_x = (int)n_x; a.ops = ops; a.ch = ch; a.C = C; a.M = (int)M; a.B = B; a.dy = dy;
     a.y_out = y_out; a.loss_out = loss_out; a.grad_C = grad_C;
-    a.gx_partial = L.gx_lds ? (void *)(ws + L.gx_off) : nullptr;
-    a.grad_X_direct = (want_gx && !L.gx_lds) ? grad_X : nullptr;
+    a.gx_partial = want_gx ? (void *)(ws + L.gx_off) : nullptr;
     a.err_b = err_b; a.flags = flags;
-    a.gtape = L.g4 ? (void *)(ws + L.gt_off) : nullptr;
+    a.gtape = L.grid[4] ? (void *)(ws + L.gt_off) : nullptr;
     a.mode = mode; a.fwd_only = fwd_only; a.want_gx = want_gx ? 1 : 0;
     a.x_off = x_off; a.y_off = y_off; a.n_pts = n_pts;
-    auto bucket_args = [&](int k, int64_t row0) {
+    auto bucket_args = [&](int k) {
         FArgs r = a;
         r.list = lists + (size_t)k * B;
         r.list_count = counters + k;
         r.work_counter = counters + F_NBUCKETS + k;
-        r.gx_row0 = row0;
+        r.gx_row0 = L.row0[k];
+        r.gx_lds = L.mode[k] == 1 ? 1 : 0;
+        r.seg = want_gx ? segs.seg[k] : nullptr;
         return r;
     };
     int r = (int)hipGetLastError();
     if (r) return r;
     if (M <= 4) {
-        if ((r = fused_launch<SR_F0(T)>(bucket_args(0, 0), L.g0, L.gx_lds, s))) return r;
+        if ((r = fused_launch<SR_F0(T)>(bucket_args(0), L.grid[0], L.mode[0], s))) return r;
     } else {
-        // buckets 4+3, 2 and 1 on side streams, bucket 0 on the caller's stream
-        Fork &f = fork_for_device();
-        if (!f.ok) return (int)hipErrorOutOfMemory;
-        if ((r = (int)hipEventRecord(f.ev_fork, s))) return r;
-        for (int i = 0; i < 3; ++i)
-            if ((r = (int)hipStreamWaitEvent(f.side[i], f.ev_fork, 0))) return r;
-        const int64_t r3 = (int64_t)L.g0 + L.g1 + L.g2;
+        // buckets 4+3, 2 and 1 on side streams, bucket 0 on the caller's
+        // stream, joined back (also after a failed launch)
+        ForkScope fk(s);
+        if ((r = fk.fork())) return fk.finish(r);
         // buckets 4 and 3 share a stream; with M <= 64 bucket 4 only takes
         // programs with extra adjoint rows (rare) and goes second
-        if (M > 64 && (r = fused_launch<SR_F4(T)>(bucket_args(4, r3 + L.g3), L.g4, L.gx_lds, f.side[0]))) return r;
-        if (L.g3 && (r = fused_launch<SR_F3(T)>(bucket_args(3, r3), L.g3, L.gx_lds, f.side[0]))) return r;
-        if (L.g4 && M <= 64 && (r = fused_launch<SR_F4(T)>(bucket_args(4, r3 + L.g3), L.g4, L.gx_lds, f.side[0])))
-            return r;
-        if (L.g2 && (r = fused_launch<SR_F2(T)>(bucket_args(2, L.g0 + L.g1), L.g2, L.gx_lds, f.side[1]))) return r;
-        if (L.g1 && (r = fused_launch<SR_F1(T)>(bucket_args(1, L.g0), L.g1, L.gx_lds, f.side[2]))) return r;
-        if ((r = fused_launch<SR_F0(T)>(bucket_args(0, 0), L.g0, L.gx_lds, s))) return r;
-        for (int i = 0; i < 3; ++i) {
-            if ((r = (int)hipEventRecord(f.ev_join[i], f.side[i]))) return r;
-            if ((r = (int)hipStreamWaitEvent(s, f.ev_join[i], 0))) return r;
-        }
+        if (M > 64 && (r = fused_launch<SR_F4(T)>(bucket_args(4), L.grid[4], L.mode[4], fk.side(0))))
+            return fk.finish(r);
+        if (L.grid[3] && (r = fused_launch<SR_F3(T)>(bucket_args(3), L.grid[3], L.mode[3], fk.side(0))))
+            return fk.finish(r);
+        if (L.grid[4] && M <= 64 && (r = fused_launch<SR_F4(T)>(bucket_args(4), L.grid[4], L.mode[4], fk.side(0))))
+            return fk.finish(r);
+        if (L.grid[2] && (r = fused_launch<SR_F2(T)>(bucket_args(2), L.grid[2], L.mode[2], fk.side(1))))
+            return fk.finish(r);
+        if (L.grid[1] && (r = fused_launch<SR_F1(T)>(bucket_args(1), L.grid[1], L.mode[1], fk.side(2))))
+            return fk.finish(r);
+        if ((r = fused_launch<SR_F0(T)>(bucket_args(0), L.grid[0], L.mode[0], s))) return fk.finish(r);
+        if ((r = fk.finish(0))) return r;
     }
-    if (L.gx_lds) {
-        const int64_t rows = (int64_t)L.g0 + L.g1 + L.g2 + L.g3 + L.g4;
+    if (want_gx) {
+        const int64_t rows = L.rows;
         const int64_t chunks = (rows + ROW_CHUNK - 1) / ROW_CHUNK;
         const unsigned nb = (unsigned)((gx_len + 255) / 256);
+        if (chunks > 65535) return -1;
         hipLaunchKernelGGL(k_sum_rows_1<T>, dim3(nb, (unsigned)chunks), dim3(256), 0, s, (const T *)(ws + L.gx_off),
                            rows, gx_len, (double *)(ws + L.g64_off));
         hipLaunchKernelGGL(k_sum_rows_2<T>, dim3(nb), dim3(256), 0, s, (const double *)(ws + L.g64_off), chunks,

@@ -268,13 +268,14 @@ struct GradAcc {
 // reference to its adjoint (LDS tape, a global tape, or the caller's
 // grad_cache).  Constant adjoints go to acc.cacc (first NCREG constants) or,
 // beyond that, to a wave-uniform FP64 LDS slot cextra[k].  Variable adjoints
-// go to acc.gxl[v] (n_x <= NXREG) or straight to gx_global_row with atomics.
+// go to acc.gxl[v] when n_x <= NXREG; with more variables the caller sums
+// them after the sweep with var_adjoints (same values, same order).
 // Inactive lanes (points past N) read but never write and contribute
 // nothing.  Returns this lane's error code (0, 201 or 204; the larger one if
 // both occur).
 template <typename T, bool WANT_GX, typename Val, typename Adj>
 __device__ inline int32_t backward_tile(const Program<T> &p, Val val, Adj adj, int L, bool active,
-                                        GradAcc<T> &acc, double *cextra, int n_x, T *gx_global_row) {
+                                        GradAcc<T> &acc, double *cextra, int n_x) {
     const int lane = lane_id();
     int32_t err = 0;
     for (int k = L - 1; k >= 0; --k) {
@@ -300,15 +301,11 @@ __device__ inline int32_t backward_tile(const Program<T> &p, Val val, Adj adj, i
         }
         if (op == UNKNOWN_LEAF || op < SIN) continue;  // hard constants and 0-valued leaves
         if (op >= VAR_START_ID) {
-            if (WANT_GX && active && g != T(0)) {
+            if (WANT_GX && active && g != T(0) && n_x <= NXREG) {
                 int v = op - VAR_START_ID;
-                if (n_x <= NXREG) {
 #pragma unroll
-                    for (int q = 0; q < NXREG; ++q)
-                        if (q == v) acc.gxl[q] += g;
-                } else {
-                    atomicAdd(&gx_global_row[v], g);
-                }
+                for (int q = 0; q < NXREG; ++q)
+                    if (q == v) acc.gxl[q] += g;
             }
             continue;
         }
@@ -324,6 +321,28 @@ __device__ inline int32_t backward_tile(const Program<T> &p, Val val, Adj adj, i
         if (active && add1) adj(c1) += go1;
     }
     return err;
+}
+
+// Adjoint sums of the variables vbase .. vbase + NXREG - 1 at this lane's
+// point, from the final node adjoints adj(k) after a reverse sweep: the
+// variable nodes' adjoints added in reverse node order, as backward_tile adds
+// them during the sweep (a variable node's adjoint is final when the sweep
+// reaches it).  Used when n_x > NXREG, one chunk of variables at a time.
+template <typename T, typename Adj>
+__device__ inline void var_adjoints(const Program<T> &p, Adj adj, int L, bool active, int vbase, T (&gxl)[NXREG]) {
+#pragma unroll
+    for (int q = 0; q < NXREG; ++q) gxl[q] = T(0);
+    for (int k = L - 1; k >= 0; --k) {
+        const int op = node_op(p.word(k));
+        if (op < VAR_START_ID || op == UNKNOWN_LEAF) continue;
+        const int v = op - VAR_START_ID - vbase;
+        if (v < 0 || v >= NXREG) continue;
+        const T g = adj(k);
+        if (active && g != T(0))
+#pragma unroll
+            for (int q = 0; q < NXREG; ++q)
+                if (q == v) gxl[q] += g;
+    }
 }
 
 // Wave-reduces the constant accumulators and returns, in lane k, the FP64

@@ -9,11 +9,12 @@ forward/backward, FP64 losses, inner products and Wolfe tests.
 """
 from __future__ import annotations
 
+import ctypes
 from typing import Dict, Literal, Optional, Tuple, Union
 
 import torch
 
-from . import _lib
+from . import _lib, autotune
 from ._C import _require_gpu, get_error_message
 from .fused import fused_root, ragged_offsets
 
@@ -21,7 +22,10 @@ _LINE_SEARCH = {"strong_wolfe": 0, "backtracking": 1}
 
 
 def _fit(X, Y, Ops, Ch, C_init, max_iter, history_size, tolerance_grad, line_search_fn, stop_loss=None,
-         x_off=None, y_off=None, n_pts=None, line_search_max_iter: int = 10, eps: float = 1e-10):
+         x_off=None, y_off=None, n_pts=None, line_search_max_iter: int = 10, eps: float = 1e-10,
+         config=None, stationary_exit: bool = True):
+    """One sr_lbfgs_fit call.  config: launch variant per tape-row bucket
+    (None: the autotuner's entry for this workload, else the defaults)."""
     dev = Ops.device
     B, M = Ops.shape
     if line_search_fn not in _LINE_SEARCH:
@@ -40,13 +44,17 @@ def _fit(X, Y, Ops, Ch, C_init, max_iter, history_size, tolerance_grad, line_sea
     ws = torch.empty(max(int(lib.sr_lbfgs_workspace_bytes(B, M)), 256), dtype=torch.uint8, device=dev)
     N = X.shape[0]
     n_x = X.shape[1]
+    if config is None:
+        config = autotune.lookup(dev, M, N, n_x, history_size, line_search_fn)
+    opts = autotune.make_options(config, stationary_exit)
     _lib.check_status(
         lib.sr_lbfgs_fit(
             _lib.ptr(X), _lib.ptr(Y), N, n_x, _lib.ptr(Ops), _lib.ptr(Ch), _lib.ptr(C_init), M, B,
             _lib.ptr(x_off), _lib.ptr(y_off), _lib.ptr(n_pts), _lib.ptr(stop_loss),
             _lib.ptr(C_out), _lib.ptr(loss), _lib.ptr(iters), _lib.ptr(nclos), _lib.ptr(err), _lib.ptr(flags),
             int(max_iter), int(history_size), float(tolerance_grad), _LINE_SEARCH[line_search_fn],
-            int(line_search_max_iter), float(eps), _lib.ptr(ws), ws.numel(), _lib.stream_of(dev),
+            int(line_search_max_iter), float(eps), ctypes.byref(opts), _lib.ptr(ws), ws.numel(),
+            _lib.stream_of(dev),
         ),
         "sr_lbfgs_fit",
     )

@@ -172,6 +172,23 @@ int sr_fused_root_ragged_f64(const double *X, int64_t N, int64_t n_x, const int6
                              double *grad_C, int32_t *err_b, int32_t *flags, void *workspace, int64_t workspace_bytes,
                              void *stream);
 
+/* Per-call options of sr_lbfgs_fit (HOST memory; no counterpart in the
+ * reference).  variant[k]: launch variant of tape-row bucket k (waves per
+ * problem x points per lane, see sr_fit_variant_info), -1 = the built-in
+ * default.  stationary_exit: 1 (the default) ends a problem whose remaining
+ * iterations of LBFGS_batch.step (symbolic_torch/lbfgs_batch.py:266-326)
+ * would repeat one accepted zero step: when every stored history pair is
+ * exactly zero, the two-loop direction is zero, and if the objective at x
+ * then reproduces (f, g) bit for bit, each remaining iteration changes
+ * nothing but the history ring, so the problem ends with the results of
+ * running them (same C, loss and iteration count max_iter; only the closure
+ * count differs).  0 runs every iteration. */
+#define SR_FIT_BUCKETS 5
+typedef struct sr_fit_options {
+    int variant[SR_FIT_BUCKETS];
+    int stationary_exit;
+} sr_fit_options;
+
 /* Device-resident batched L-BFGS over the LEARNABLE_CONSTANT slots of B
  * batch-first expressions.  Replaces optimize + LBFGS_batch.step
  * (symbolic_torch/optimization.py:9-112, symbolic_torch/lbfgs_batch.py:61-326)
@@ -182,35 +199,28 @@ int sr_fused_root_ragged_f64(const double *X, int64_t N, int64_t n_x, const int6
  *   stop_loss (B,) optional early stop on the FP64 loss (nullable);
  *   outputs C_out (B,M) (non-constant slots keep C_init, or NaN where the
  *   reference's go NaN), loss_out (B,) FP64 final loss, iters_out (B,),
- *   nclos_out (B,) objective evaluations (nullable), err_out (B,), flags (2).
- * n_x <= 2048. */
+ *   nclos_out (B,) objective evaluations (nullable), err_out (B,), flags (2);
+ *   opts: host pointer, NULL = defaults.
+ * n_x <= 2048.  The call keeps no state between calls; concurrent calls
+ * from several host threads are safe (on one caller stream they are
+ * enqueued one after the other). */
 int64_t sr_lbfgs_workspace_bytes(int64_t B, int64_t M);
 int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
                  const float *C_init, int64_t M, int64_t B, const int64_t *x_off, const int64_t *y_off,
                  const int32_t *n_pts, const double *stop_loss, float *C_out, double *loss_out, int32_t *iters_out,
                  int32_t *nclos_out, int32_t *err_out, int32_t *flags, int max_iter, int history, double tol_grad,
-                 int line_search, int ls_max_iter, double eps, void *workspace, int64_t ws_bytes, void *stream);
+                 int line_search, int ls_max_iter, double eps, const sr_fit_options *opts, void *workspace,
+                 int64_t ws_bytes, void *stream);
 
-/* Launch variants of the fit kernels (no counterpart in the reference; used
- * by the autotuner).  Problems are bucketed by tape rows; each
- * bucket has variants (waves per problem x points per lane per dispatch)
- * of which one is active per process.  Host-only calls; return -1 on an
- * out-of-range bucket or variant. */
+/* Launch-variant table of the fit kernels (constant; no counterpart in the
+ * reference; read by the autotuner).  Problems are bucketed by tape rows;
+ * each bucket has variants (waves per problem x points per lane per
+ * dispatch).  Host-only calls; return -1 on an out-of-range bucket or
+ * variant. */
 int sr_fit_num_buckets(void);
 int sr_fit_num_variants(int bucket);
-int sr_fit_get_variant(int bucket);
-int sr_fit_set_variant(int bucket, int variant);
+int sr_fit_default_variant(int bucket);
 int sr_fit_variant_info(int bucket, int variant, int *waves, int *ppt);
-
-/* Stationary exit of sr_lbfgs_fit (host-only, process-global, on by
- * default; returns the previous setting).  When every stored history pair
- * of a problem is exactly zero, its two-loop direction is zero; if the
- * objective at x then reproduces (f, g) bit for bit, each remaining
- * iteration of LBFGS_batch.step (symbolic_torch/lbfgs_batch.py:266-326)
- * repeats the same accepted zero step, so the problem ends there with the
- * results of running them: same C, loss and iteration count (max_iter).
- * Only the closure count differs.  Off runs every iteration. */
-int sr_fit_set_stationary_exit(int on);
 
 #ifdef __cplusplus
 }

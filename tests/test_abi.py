@@ -56,9 +56,41 @@ def test_fit_variant_table_is_consistent():
     from cudasymbolicregression_amd import autotune
 
     v = autotune.variants()
-    cfg = autotune.get_config()
-    assert len(v) == len(cfg) >= 1
+    cfg = autotune.default_config()
+    assert len(v) == len(cfg) == autotune.N_BUCKETS
     for b, row in enumerate(v):
         assert 0 <= cfg[b] < len(row)
         for d in row:
             assert d["waves"] >= 1 and d["ppt"] >= 1
+    assert autotune.validate_config(cfg) == cfg
+    with pytest.raises(ValueError):
+        autotune.validate_config([len(row) for row in v])
+
+
+def test_autotune_cache_is_keyed_and_persists(tmp_path):
+    """The workload cache: entries are found by their key only, survive a
+    save/load round trip, and bad configurations are refused."""
+    import torch
+
+    from cudasymbolicregression_amd import autotune
+
+    autotune.cache_clear()
+    dev = torch.device("cpu")
+    cfg = [0] * autotune.N_BUCKETS
+    assert autotune.lookup(dev, 64, 512, 3) is None
+    autotune.cache_put(autotune.workload_key(dev, 64, 512, 3), cfg, 1.5)
+    assert autotune.lookup(dev, 64, 512, 3) == cfg
+    assert autotune.lookup(dev, 64, 300, 3) == cfg  # same power-of-two point bucket
+    assert autotune.lookup(dev, 64, 513, 3) is None
+    assert autotune.lookup(dev, 32, 512, 3) is None
+    path = tmp_path / "tune.json"
+    autotune.cache_save(str(path))
+    autotune.cache_clear()
+    assert autotune.lookup(dev, 64, 512, 3) is None
+    assert autotune.cache_load(str(path)) == 1
+    assert autotune.lookup(dev, 64, 512, 3) == cfg
+    autotune.cache_clear()
+    with pytest.raises(ValueError):
+        autotune.cache_put(autotune.workload_key(dev, 64, 512, 3), [99] * autotune.N_BUCKETS)
+    with pytest.raises(ValueError):
+        autotune.make_options([0, 0])

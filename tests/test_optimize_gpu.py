@@ -191,7 +191,6 @@ def test_stationary_exit_matches_running_every_iteration(line_search):
     iterations (badly scaled G2 problems started near their optimum stall
     like that under the reference's unscaled first step)."""
     import cudasymbolicregression_amd as st
-    from cudasymbolicregression_amd import _lib as L
     from cudasymbolicregression_amd import warmstart
     from cudasymbolicregression_amd.optimization import _fit
 
@@ -201,15 +200,10 @@ def test_stationary_exit_matches_running_every_iteration(line_search):
     ops, ch, Ct, Y = warmstart.sample_problems(g, X, 4096, seed=21)
     torch.manual_seed(22)
     C0 = (Ct + 0.01 * torch.randn_like(Ct) * (ops == 1)).contiguous()
-    lib = L.hip()
     res = {}
-    try:
-        for on in (0, 1):
-            lib.sr_fit_set_stationary_exit(on)
-            res[on] = _fit(X, Y, ops, ch, C0, 100, 10, 1e-7, line_search)
-            torch.cuda.synchronize()
-    finally:
-        lib.sr_fit_set_stationary_exit(1)
+    for on in (0, 1):
+        res[on] = _fit(X, Y, ops, ch, C0, 100, 10, 1e-7, line_search, stationary_exit=bool(on))
+        torch.cuda.synchronize()
     (C_a, l_a, it_a, e_a, f_a, n_a), (C_b, l_b, it_b, e_b, f_b, n_b) = res[0], res[1]
     assert torch.equal(C_a.view(torch.int32), C_b.view(torch.int32))
     assert torch.equal(l_a.view(torch.int64), l_b.view(torch.int64))
