@@ -327,65 +327,100 @@ DEFINE_MULTI(double, f64)
 /* err_out[b] receives the per-expression code (max over the sweep).    */
 /* ------------------------------------------------------------------ */
 #define DEFINE_FUSED(T, SFX)                                                                        \
+    static void fused_one_##SFX(const T *X, int64_t N, int64_t n_x, const int64_t *ops,            \
+                                const int64_t *ch, const T *C, int64_t M, const T *dyb, int mode,  \
+                                T *yb, double *loss_b, T *gC, T *grad_X, int32_t *err_b, T *tape,  \
+                                T *adj, double *gacc) {                                            \
+        const T inv_n = (T)1.0 / (T)N;                                                             \
+        int64_t cnt = 0, L = 0;                                                                    \
+        for (int64_t k = 0; k < M; ++k)                                                            \
+            if (ops[k] != NO_OP) { ++cnt; L = k + 1; }                                             \
+        int64_t root = cnt - 1; if (root < 0) root = 0;                                            \
+        /* nodes k >= L are NO_OP padding: NaN values nothing reads, zero adjoints */             \
+        if (L < root + 1) L = root + 1;                                                            \
+        int32_t err = 0;                                                                           \
+        double loss = 0.0;                                                                         \
+        for (int64_t k = 0; k < M; ++k) gC[k] = 0;                                                 \
+        if (gacc) for (int64_t k = 0; k < M; ++k) gacc[k] = 0.0;                                   \
+        for (int64_t n = 0; n < N; ++n) {                                                          \
+            const T *xrow = &X[n * n_x];                                                           \
+            for (int64_t k = 0; k < L; ++k) {                                                      \
+                int op = (int)ops[k];                                                              \
+                if (op == NO_OP) { tape[k] = (T)NAN; continue; }                                   \
+                int ar = arity_of(op);                                                             \
+                T a0 = 0, a1 = 0;                                                                  \
+                if (ar >= 1) a0 = tape[ch[2 * k]];                                                 \
+                if (ar == 2) a1 = tape[ch[2 * k + 1]];                                             \
+                tape[k] = fwd_op_##SFX(op, a0, a1, C[k], xrow, n_x);                               \
+            }                                                                                      \
+            T yh = tape[root];                                                                     \
+            yb[n] = yh;                                                                            \
+            T g0;                                                                                  \
+            if (mode == 1) {                                                                       \
+                T d = yh - dyb[n];                                                                 \
+                loss += (double)d * (double)d;                                                     \
+                g0 = ((T)2.0 * d) * inv_n;                                                         \
+            } else g0 = dyb[n];                                                                    \
+            for (int64_t k = 0; k < L; ++k) adj[k] = 0;                                            \
+            adj[root] = g0;                                                                        \
+            for (int64_t k = L - 1; k >= 0; --k) {                                                 \
+                int op = (int)ops[k];                                                              \
+                int ar = arity_of(op);                                                             \
+                int64_t c0 = ch[2 * k], c1 = ch[2 * k + 1];                                        \
+                T a0 = 0, a1 = 0;                                                                  \
+                if (ar >= 1) a0 = tape[c0];                                                        \
+                if (ar == 2) a1 = tape[c1];                                                        \
+                T go0 = 0, go1 = 0;                                                                \
+                int h0, h1, hc, vi;                                                                \
+                int e = bwd_op_##SFX(op, adj[k], a0, a1, n_x, &go0, &h0, &go1, &h1, &hc, &vi);     \
+                if (e > err) err = e;                                                              \
+                if (h0) adj[c0] += go0;                                                            \
+                if (h1) adj[c1] += go1;                                                            \
+                if (hc) { if (gacc) gacc[k] += (double)adj[k]; else gC[k] += adj[k]; }             \
+                if (vi >= 0 && grad_X) grad_X[n * n_x + vi] += adj[k];                             \
+            }                                                                                      \
+        }                                                                                          \
+        if (gacc) for (int64_t k = 0; k < M; ++k) gC[k] = (T)gacc[k];                              \
+        if (loss_b) *loss_b = (mode == 1) ? loss / (double)N : 0.0;                                \
+        if (err_b) *err_b = err;                                                                   \
+    }                                                                                              \
+    /* Expressions are independent: without grad_X (a sum over expressions, */                   \
+    /* kept in expression order) they run on all OpenMP threads, with the   */                    \
+    /* same results as one after another.  acc64: grad_C summed in double   */                    \
+    /* (the device's FP64 reductions) instead of in the element type.       */                    \
+    static void fused_root_impl_##SFX(const T *X, int64_t N, int64_t n_x, const int64_t *ops_bm,   \
+                                      const int64_t *ch_bm, const T *C_bm, int64_t M, int64_t B,   \
+                                      const T *dy, int mode, T *y_out, double *loss_out, T *grad_C, \
+                                      T *grad_X, int32_t *err_out, int acc64) {                    \
+        _Pragma("omp parallel if (grad_X == NULL && B > 1)")                                       \
+        {                                                                                          \
+            T *tape = (T *)malloc(sizeof(T) * (size_t)(M > 0 ? M : 1));                            \
+            T *adj = (T *)malloc(sizeof(T) * (size_t)(M > 0 ? M : 1));                             \
+            double *gacc = acc64 ? (double *)malloc(sizeof(double) * (size_t)(M > 0 ? M : 1)) : NULL; \
+            _Pragma("omp for schedule(dynamic, 4)")                                                \
+            for (int64_t b = 0; b < B; ++b)                                                        \
+                fused_one_##SFX(X, N, n_x, ops_bm + b * M, ch_bm + b * M * 2, C_bm + b * M, M,     \
+                                dy ? dy + b * N : NULL, mode, y_out + b * N,                       \
+                                loss_out ? loss_out + b : NULL, grad_C + b * M, grad_X,            \
+                                err_out ? err_out + b : NULL, tape, adj, gacc);                    \
+            free(tape);                                                                            \
+            free(adj);                                                                             \
+            free(gacc);                                                                            \
+        }                                                                                          \
+    }                                                                                              \
     void oracle_fused_root_##SFX(const T *X, int64_t N, int64_t n_x, const int64_t *ops_bm,       \
                                  const int64_t *ch_bm, const T *C_bm, int64_t M, int64_t B, const T *dy, \
                                  int mode, T *y_out, double *loss_out, T *grad_C, T *grad_X,        \
                                  int32_t *err_out) {                                               \
-        T *tape = (T *)malloc(sizeof(T) * (size_t)M);                                              \
-        T *adj = (T *)malloc(sizeof(T) * (size_t)M);                                               \
-        const T inv_n = (T)1.0 / (T)N;                                                             \
-        for (int64_t b = 0; b < B; ++b) {                                                          \
-            const int64_t *ops = ops_bm + b * M;                                                   \
-            const int64_t *ch = ch_bm + b * M * 2;                                                 \
-            int64_t cnt = 0;                                                                       \
-            for (int64_t k = 0; k < M; ++k) cnt += (ops[k] != NO_OP);                              \
-            int64_t root = cnt - 1; if (root < 0) root = 0;                                        \
-            int32_t err = 0;                                                                       \
-            double loss = 0.0;                                                                     \
-            for (int64_t k = 0; k < M; ++k) grad_C[b * M + k] = 0;                                 \
-            for (int64_t n = 0; n < N; ++n) {                                                      \
-                const T *xrow = &X[n * n_x];                                                       \
-                for (int64_t k = 0; k < M; ++k) {                                                  \
-                    int op = (int)ops[k];                                                          \
-                    if (op == NO_OP) { tape[k] = (T)NAN; continue; }                               \
-                    int ar = arity_of(op);                                                         \
-                    T a0 = 0, a1 = 0;                                                              \
-                    if (ar >= 1) a0 = tape[ch[2 * k]];                                             \
-                    if (ar == 2) a1 = tape[ch[2 * k + 1]];                                         \
-                    tape[k] = fwd_op_##SFX(op, a0, a1, C_bm[b * M + k], xrow, n_x);                \
-                }                                                                                  \
-                T yh = tape[root];                                                                 \
-                y_out[b * N + n] = yh;                                                             \
-                T g0;                                                                              \
-                if (mode == 1) {                                                                   \
-                    T d = yh - dy[b * N + n];                                                      \
-                    loss += (double)d * (double)d;                                                 \
-                    g0 = ((T)2.0 * d) * inv_n;                                                     \
-                } else g0 = dy[b * N + n];                                                         \
-                for (int64_t k = 0; k < M; ++k) adj[k] = 0;                                        \
-                adj[root] = g0;                                                                    \
-                for (int64_t k = M - 1; k >= 0; --k) {                                             \
-                    int op = (int)ops[k];                                                          \
-                    int ar = arity_of(op);                                                         \
-                    int64_t c0 = ch[2 * k], c1 = ch[2 * k + 1];                                    \
-                    T a0 = 0, a1 = 0;                                                              \
-                    if (ar >= 1) a0 = tape[c0];                                                    \
-                    if (ar == 2) a1 = tape[c1];                                                    \
-                    T go0 = 0, go1 = 0;                                                            \
-                    int h0, h1, hc, vi;                                                            \
-                    int e = bwd_op_##SFX(op, adj[k], a0, a1, n_x, &go0, &h0, &go1, &h1, &hc, &vi); \
-                    if (e > err) err = e;                                                          \
-                    if (h0) adj[c0] += go0;                                                        \
-                    if (h1) adj[c1] += go1;                                                        \
-                    if (hc) grad_C[b * M + k] += adj[k];                                           \
-                    if (vi >= 0 && grad_X) grad_X[n * n_x + vi] += adj[k];                         \
-                }                                                                                  \
-            }                                                                                      \
-            if (loss_out) loss_out[b] = (mode == 1) ? loss / (double)N : 0.0;                      \
-            if (err_out) err_out[b] = err;                                                         \
-        }                                                                                          \
-        free(tape);                                                                                \
-        free(adj);                                                                                 \
+        fused_root_impl_##SFX(X, N, n_x, ops_bm, ch_bm, C_bm, M, B, dy, mode, y_out, loss_out,     \
+                              grad_C, grad_X, err_out, 0);                                         \
+    }                                                                                              \
+    void oracle_fused_root_acc64_##SFX(const T *X, int64_t N, int64_t n_x, const int64_t *ops_bm, \
+                                       const int64_t *ch_bm, const T *C_bm, int64_t M, int64_t B,  \
+                                       const T *dy, int mode, T *y_out, double *loss_out, T *grad_C, \
+                                       T *grad_X, int32_t *err_out) {                              \
+        fused_root_impl_##SFX(X, N, n_x, ops_bm, ch_bm, C_bm, M, B, dy, mode, y_out, loss_out,     \
+                              grad_C, grad_X, err_out, 1);                                         \
     }
 
 DEFINE_FUSED(float, f32)

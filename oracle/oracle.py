@@ -37,8 +37,9 @@ def lib() -> ctypes.CDLL:
             getattr(l, f"oracle_multi_forward_{s}").argtypes = [P, I64, I64, P, P, P, I64, I64, I64, P]
             getattr(l, f"oracle_multi_backward_{s}").restype = c_int32
             getattr(l, f"oracle_multi_backward_{s}").argtypes = [P, P, P, P, I64, I64, I64, I64, I64, P, P]
-            getattr(l, f"oracle_fused_root_{s}").argtypes = [P, I64, I64, P, P, P, I64, I64, P, c_int, P, P, P, P,
-                                                             P]
+            for v in ("", "acc64_"):
+                getattr(l, f"oracle_fused_root_{v}{s}").argtypes = [P, I64, I64, P, P, P, I64, I64, P, c_int, P, P,
+                                                                    P, P, P]
         _lib = l
     return _lib
 
@@ -153,8 +154,9 @@ def multi_backward(grad_out, cache, ops_mb, ch_mb, n_x: int, use_ref: bool = Fal
     return gC, gX, int(e)
 
 
-def fused_root(X, ops_bm, ch_bm, C_bm, dy, mode: int, want_gx: bool = True):
-    """Root-only fused forward+backward over batch-first inputs.
+def fused_root(X, ops_bm, ch_bm, C_bm, dy, mode: int, want_gx: bool = True, acc64: bool = False):
+    """Root-only fused forward+backward over batch-first inputs (acc64:
+    grad_C summed in double, as the device sums it, then rounded).
 
     Returns dict(y (B,N), loss (B,) float64, grad_C (B,M), grad_X (N,n_x), err (B,) int32).
     """
@@ -171,7 +173,7 @@ def fused_root(X, ops_bm, ch_bm, C_bm, dy, mode: int, want_gx: bool = True):
     gC = np.zeros((B, M), dt)
     gX = np.zeros((N, nx), dt)
     err = np.zeros(B, np.int32)
-    getattr(lib(), f"oracle_fused_root_{_dt(dt)}")(_p(X), N, nx, _p(ops_bm), _p(ch_bm), _p(C_bm), M, B, _p(dy),
+    getattr(lib(), f"oracle_fused_root_{'acc64_' if acc64 else ''}{_dt(dt)}")(_p(X), N, nx, _p(ops_bm), _p(ch_bm), _p(C_bm), M, B, _p(dy),
                                                     int(mode), _p(y), _p(loss), _p(gC),
                                                     _p(gX) if want_gx else None, _p(err))
     return dict(y=y, loss=loss, grad_C=gC, grad_X=gX, err=err)

@@ -254,3 +254,124 @@ def rounding_bound(X, ops_bm, ch_bm, C_bm, ulps=8, eps=2.0 ** -24):
         t[(om == 0)[:, None, :].repeat(N, axis=1)] = 0.0
         s = t.sum(axis=0).T  # (B, N)
     return ulps * eps * s
+
+
+def fit_mse_tolerance(mse_ref):
+    """Per-problem parity tolerance of a fit's final MSE: the reference runs in
+    FP32, the device sums in FP64, so trajectories agree to a relative 1e-3."""
+    return 1e-3 * np.abs(mse_ref) + 1e-7
+
+
+def oracle_fits(X, Y, ops, ch, C0, rel_x=2.0 ** -21, **kw):
+    """The oracle's fit with the reference's FP32 arithmetic
+    (lbfgs_oracle.lbfgs_run, pinned bit for bit to the reference's
+    trajectories), with the device's FP64 sums, inner products and Wolfe
+    tests (lbfgs_oracle.lbfgs_run64), and three more times with the device's
+    arithmetic on perturbed inputs: X scaled by 1 +- rel_x, and every
+    closure value scaled by 1 +- 2^-19 (random signs).  The perturbed runs
+    tell which problems the algorithm determines: where one ends elsewhere,
+    the outcome hinges on the last ulp of the node arithmetic (say cos of a
+    1e6 argument, which moves 0.25 rad per FP32 ulp) or of the sums, where
+    the device's transcendentals and reduction order and the host's
+    legitimately differ.  Returns dict(C32, mse32, C64, mse64, C64x,
+    mse64x), the x entries being the perturbed run farthest from the
+    unperturbed one; mse is the FP64 MSE of the final prediction at the
+    unperturbed X."""
+    import lbfgs_oracle
+
+    Y = np.asarray(Y, np.float32)
+    X = np.asarray(X, np.float32)
+    out = {}
+
+    def mse_at(C):
+        y = lbfgs_oracle.oracle.fused_root(X, ops, ch, C, Y, mode=1, want_gx=False)["y"]
+        with np.errstate(all="ignore"):
+            return ((y.astype(np.float64) - Y) ** 2).mean(1)
+
+    def fit(Xr, prec, noise=None):
+        C, _, _, _ = lbfgs_oracle.optimize(Xr.astype(np.float32), Y, ops, ch, C0, precision=prec, noise=noise, **kw)
+        return C, mse_at(C)
+
+    out["C32"], out["mse32"] = fit(X, "fp32")
+    out["C64"], out["mse64"] = fit(X, "fp64")
+    # the perturbed runs; C64x / mse64x hold, per problem, the one farthest
+    # from the unperturbed device-precision result
+    runs = [fit(X * np.float32(1 + rel_x), "fp64"), fit(X * np.float32(1 - rel_x), "fp64"),
+            fit(X, "fp64", noise=(2.0 ** -19, 77))]
+    with np.errstate(invalid="ignore"):
+        dm = np.stack([np.nan_to_num(np.abs(m - out["mse64"]) / fit_mse_tolerance(out["mse64"]), nan=np.inf)
+                       for _, m in runs])
+        dc = np.stack([np.nan_to_num((np.abs(C - out["C64"]) / (1 + np.abs(out["C64"]))).max(1), nan=np.inf)
+                       for C, _ in runs])
+    wm, wc = dm.argmax(0), dc.argmax(0)
+    idx = np.arange(len(Y))
+    out["mse64x"] = np.stack([m for _, m in runs])[wm, idx]
+    out["C64x"] = np.stack([C for C, _ in runs])[wc, idx]
+    return out
+
+
+def compare_fit_to_oracle(mse_gpu, fits, err_gpu, label, max_undetermined=0.15, max_chaotic=0.02, extra_tol=None):
+    """The device's final MSE against the oracle (oracle_fits), per problem
+    without an error code, with tolerance fit_mse_tolerance (+ extra_tol):
+    * against the device-precision oracle on every finite problem that the
+      algorithm determines (every perturbed run ends within the tolerance),
+      except at most max_chaotic of them (at least one), which are listed:
+      the perturbations sample a trajectory's sensitivity without bounding
+      it, and a chaotic one still parts (a jump to a point where every trial
+      is rejected gives a zero step, after which the reference's rules give
+      zero directions for good);
+    * against the reference-precision oracle on EVERY determined problem
+      where the two oracle precisions agree and the device matches the
+      device-precision run.  Where the precisions disagree, the reference's
+      FP32 sums and the device's FP64 ones take the problem to different
+      ends; those are counted.
+    Undetermined problems are counted (at most max_undetermined of the
+    finite ones).  Returns a printable summary; raises AssertionError."""
+    m32, m64, m64x = fits["mse32"], fits["mse64"], fits["mse64x"]
+    extra = 0.0 if extra_tol is None else extra_tol
+    ok = (np.asarray(err_gpu) == 0) & np.isfinite(m64)
+    with np.errstate(invalid="ignore"):
+        det = ok & (np.abs(m64x - m64) <= fit_mse_tolerance(m64))
+        c64 = np.abs(mse_gpu - m64) <= fit_mse_tolerance(m64) + extra
+        agree = np.isfinite(m32) & (np.abs(m32 - m64) <= fit_mse_tolerance(m32))
+        c32 = np.abs(mse_gpu - m32) <= fit_mse_tolerance(m32) + extra
+    und = np.nonzero(ok & ~det)[0]
+    out64 = np.nonzero(det & ~c64)[0]
+    out32 = np.nonzero(det & agree & c64 & ~c32)[0]
+    msg = (f"{label}: {ok.sum()} finite problems, {det.sum()} determined; device vs device-precision oracle: "
+           f"{len(out64)} outside {[(int(i), float(mse_gpu[i]), float(m64[i])) for i in out64[:6]]}; "
+           f"precisions agree on {int((det & agree).sum())}, device vs reference-precision oracle there: "
+           f"{len(out32)} outside {[(int(i), float(mse_gpu[i]), float(m32[i])) for i in out32[:6]]}; "
+           f"undetermined {len(und)} (device off the device-precision oracle on "
+           f"{int((~c64[und]).sum()) if len(und) else 0} of them)")
+    assert len(out64) <= max(1, int(max_chaotic * det.sum())), msg
+    assert len(out32) == 0, msg
+    assert len(und) <= max_undetermined * max(ok.sum(), 1), msg
+    return msg
+
+
+def compare_constants_to_oracle(Cg, fits, label, max_undetermined=0.15):
+    """Short fits: constants within 1e-3 (1 + |C|) of the device-precision
+    oracle's on every finite problem that the algorithm determines (as in
+    compare_fit_to_oracle, judged on the constants), and of the
+    reference-precision oracle's wherever the two precisions agree as well.
+    Returns a printable summary; raises AssertionError."""
+    C32, C64, C64x = fits["C32"], fits["C64"], fits["C64x"]
+
+    def close(a, b):
+        with np.errstate(invalid="ignore"):
+            return (np.abs(a - b) <= 1e-3 * (1 + np.abs(b))).all(1)
+
+    fin = np.isfinite(C64).all(1) & np.isfinite(Cg).all(1)
+    det = fin & close(C64x, C64)
+    out64 = np.nonzero(det & ~close(Cg, C64))[0]
+    agree = np.isfinite(C32).all(1) & close(C32, C64)
+    out32 = np.nonzero(det & agree & ~close(Cg, C32))[0]
+    und = np.nonzero(fin & ~det)[0]
+    msg = (f"{label}: {fin.sum()} finite, {det.sum()} determined; vs device-precision oracle {len(out64)} outside "
+           f"{out64[:8].tolist()}; precisions agree on {int((det & agree).sum())}, vs reference-precision oracle "
+           f"there {len(out32)} outside {out32[:8].tolist()}; undetermined {und[:8].tolist()}")
+    assert len(out64) == 0, msg
+    assert len(out32) == 0, msg
+    assert len(und) <= max_undetermined * max(fin.sum(), 1), msg
+    return msg

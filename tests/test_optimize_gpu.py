@@ -1,10 +1,19 @@
 """GPU tests of `optimize` (device-resident L-BFGS) against the reference's
 known answers and against the CPU restatement of LBFGS_batch (oracle).
 
-Stated parity tolerance (FP64 reductions in the product vs FP32 in the
-reference make exact trajectories impossible): per problem
-|MSE_gpu - MSE_oracle| <= 1e-3 * MSE_oracle + 1e-7, required for at least
-90% of the problems where both runs end finite; the fraction is printed."""
+Stated parity tolerance: per problem |MSE_gpu - MSE_oracle| <= 1e-3 *
+MSE_oracle + 1e-7 (helpers.compare_fit_to_oracle).  The oracle runs with the
+reference's FP32 arithmetic (its trajectory pinned bit for bit to the
+reference's own optimizer by tests/test_lbfgs_golden.py), with the device's
+FP64 sums, inner products and Wolfe tests, and with the latter on X scaled by
+1 +- 2^-21 and with its closure values perturbed.  On the problems the
+algorithm determines (every perturbed run ends within the tolerance) the
+device must meet the tolerance against the device-precision run, all but 2%
+(at least one, listed: chaotic trajectories the probes miss), and against
+the reference-precision run wherever the two precisions agree.  Problems
+whose outcome hinges on the last ulp of the node arithmetic (where device
+transcendentals and the host libm may differ) are counted (at most 15%) and
+listed."""
 import numpy as np
 import pytest
 import torch
@@ -82,6 +91,8 @@ def test_optimize_sin_batch_reduces_loss():
 
 @pytest.mark.parametrize("seed,M", [(0, 16), (1, 32), (2, 64)])
 def test_optimize_matches_oracle_lbfgs(seed, M):
+    """Every stable problem matches the oracle's final MSE to 1e-3 relative
+    (helpers.compare_fit_to_oracle); unstable ones are counted and listed."""
     import cudasymbolicregression_amd as st
 
     rng = np.random.default_rng(seed)
@@ -92,19 +103,15 @@ def test_optimize_matches_oracle_lbfgs(seed, M):
     C_true = helpers.constants_like(ops, rng, scale=1.0)
     Y = lbfgs_oracle.oracle.fused_root(X, ops, ch, C_true, np.zeros((B, N), np.float32), mode=0)["y"]
     keep = np.isfinite(Y).all(1)
+    ops, ch, Y, C_true = ops[keep], ch[keep], Y[keep], C_true[keep]
     C0 = helpers.constants_like(ops, rng, scale=1.0)
-    C_o, y_o, f_o, _ = lbfgs_oracle.optimize(X, Y, ops, ch, C0, max_iter=50)
+    fits = helpers.oracle_fits(X, Y, ops, ch, C0, max_iter=50)
     Cg, Yg, info = st.optimize(_t(X), _t(Y), _t(ops), _t(ch), _t(C0), max_iter=50, return_info=True,
                                raise_errors=False)
-    mse_g = ((Yg.cpu().numpy().astype(np.float64) - Y) ** 2).mean(1)
-    mse_o = ((y_o.astype(np.float64) - Y) ** 2).mean(1)
-    both = keep & np.isfinite(mse_g) & np.isfinite(mse_o) & (info["err"].cpu().numpy() == 0)
-    close = np.abs(mse_g - mse_o) <= 1e-3 * mse_o + 1e-7
-    frac = close[both].mean()
-    print(f"M={M}: {both.sum()} finite problems, {frac:.3f} within tolerance; "
-          f"median |dC| {np.nanmedian(np.abs(Cg.cpu().numpy() - C_o)[ops == 1]):.2e}")
-    assert both.sum() > B // 2
-    assert frac >= 0.9
+    with np.errstate(all="ignore"):
+        mse_g = ((Yg.cpu().numpy().astype(np.float64) - Y) ** 2).mean(1)
+    msg = helpers.compare_fit_to_oracle(mse_g, fits, info["err"].cpu().numpy(), f"M={M}")
+    print(msg, flush=True)
 
 
 def test_optimize_raises_div_by_zero():
@@ -150,8 +157,10 @@ def _bucket_batch(seed):
 def test_optimize_length_buckets_match_oracle():
     """Short runs (3 iterations) keep the two L-BFGS trajectories close, so
     the fitted constants of every bucket are compared directly:
-    |C_gpu - C_oracle| <= 1e-3 * (1 + |C_oracle|) for >= 90% of the problems
-    that end finite in both, and exact agreement on non-constant slots."""
+    |C_gpu - C_oracle| <= 1e-3 * (1 + |C_oracle|) (helpers.compare_constants_to_oracle:
+    against the device-precision oracle, and against the reference-precision
+    one wherever the two precisions agree), and exact agreement on
+    non-constant slots."""
     import cudasymbolicregression_amd as st
 
     ops, ch, rng = _bucket_batch(11)
@@ -165,7 +174,8 @@ def test_optimize_length_buckets_match_oracle():
     Y = lbfgs_oracle.oracle.fused_root(X, ops, ch, C_true, np.zeros((B, N), np.float32), mode=0)["y"]
     C0 = helpers.constants_like(ops, rng, scale=1.0)
     C0[ops != 1] = 0.25  # non-constant slots must come back unchanged
-    C_o, y_o, f_o, _ = lbfgs_oracle.optimize(X, Y, ops, ch, C0, max_iter=3)
+    fits = helpers.oracle_fits(X, Y, ops, ch, C0, max_iter=3)
+    C_o = fits["C32"]
     Cg, Yg, info = st.optimize(_t(X), _t(Y), _t(ops), _t(ch), _t(C0), max_iter=3, return_info=True,
                                raise_errors=False)
     Cg = Cg.cpu().numpy()
@@ -173,12 +183,10 @@ def test_optimize_length_buckets_match_oracle():
     nonc = ops != 1
     np.testing.assert_array_equal(Cg[nonc], C_o[nonc])  # C_init, or NaN where the reference's go NaN
     fin = np.isfinite(C_o).all(1) & np.isfinite(Cg).all(1)
-    close = (np.abs(Cg - C_o) <= 1e-3 * (1 + np.abs(C_o))).all(1)
-    frac = close[fin].mean()
-    print(f"{fin.sum()} finite problems, {frac:.3f} within tolerance; per-problem max |dC|:",
-          np.round(np.abs(Cg - C_o).max(1), 6).tolist())
+    print(helpers.compare_constants_to_oracle(Cg, fits, "buckets, 3 iterations"), flush=True)
+    print("per-problem max |dC| vs the reference-precision oracle:", np.round(np.abs(Cg - C_o).max(1), 6).tolist())
     assert fin.sum() >= B * 3 // 4
-    assert frac >= 0.9
+    close = (np.abs(Cg - C_o) <= 1e-3 * (1 + np.abs(C_o))).all(1)
     # the 64-constant sum: every constant gets the same gradient
     assert close[-1]
 
@@ -264,51 +272,49 @@ def test_optimize_ragged_matches_oracle():
     program family of the shared-X comparison above; the per-problem path
     itself is pinned bit for bit by the test before).
 
-    After 3 iterations the constants agree within 1e-3 (1 + |C|) for >= 90% of
-    the problems that end finite in both.  After 50 iterations the stated
-    optimize tolerance |MSE_gpu - MSE_oracle| <= 1e-3 MSE_oracle + 1e-7,
-    widened by how far the two FP32 evaluators are apart on the problem
+    After 3 iterations the constants agree within 1e-3 (1 + |C|)
+    (helpers.compare_constants_to_oracle).  After 50 iterations the stated
+    optimize tolerance |MSE_gpu - MSE_oracle| <= 1e-3 MSE_oracle + 1e-7
+    holds as in helpers.compare_fit_to_oracle, widened by how far the two
+    FP32 evaluators are apart on the problem
     (e_b = mean_n (y_gpu(C_oracle) - y_oracle(C_oracle))^2 adds
-    2 sqrt(e_b MSE_oracle) + e_b), holds for >= 85%: long trajectories part
-    where the FP64 Wolfe tests and the reference's FP32 ones disagree, e.g. on
-    a flat loss the FP32 Armijo test passes on equality and extrapolates while
-    the FP64 one rejects the step (0.89 measured on this set)."""
+    2 sqrt(e_b MSE_oracle) + e_b)."""
     import cudasymbolicregression_amd as st
     from cudasymbolicregression_amd import fused
 
     ops, ch, n, Xs, Ys, C0 = _ragged_case(32, B=64, buckets=False)
-    B = ops.shape[0]
     keep = np.array([np.isfinite(y).all() for y in Ys])
+    ops, ch, n, C0 = ops[keep], ch[keep], n[keep], C0[keep]
+    Xs = [x for x, k in zip(Xs, keep) if k]
+    Ys = [y for y, k in zip(Ys, keep) if k]
+    B = ops.shape[0]
     Xc, Yc = _t(np.concatenate(Xs)), _t(np.concatenate(Ys))
     C3, info3 = st.optimize_ragged(Xc, Yc, _t(n), _t(ops), _t(ch), _t(C0), max_iter=3, raise_errors=False)
     C3 = C3.cpu().numpy()
     C_r, info = st.optimize_ragged(Xc, Yc, _t(n), _t(ops), _t(ch), _t(C0), max_iter=50, raise_errors=False)
     mse_g = info["loss"].cpu().numpy()
-    mse_o = np.empty(B)
-    floor = np.empty(B)
-    C3_o = np.empty_like(C3)
+    m32, m64, m64x, floor = np.empty(B), np.empty(B), np.empty(B), np.empty(B)
+    C3_32, C3_64, C3_64x = np.empty_like(C3), np.empty_like(C3), np.empty_like(C3)
     for b in range(B):
+        args = (Xs[b], Ys[b][None, :], ops[b:b + 1], ch[b:b + 1], C0[b:b + 1])
         with np.errstate(all="ignore"):
-            C3_o[b] = lbfgs_oracle.optimize(Xs[b], Ys[b][None, :], ops[b:b + 1], ch[b:b + 1], C0[b:b + 1],
-                                            max_iter=3)[0][0]
-            C_o, y_o, _, _ = lbfgs_oracle.optimize(Xs[b], Ys[b][None, :], ops[b:b + 1], ch[b:b + 1], C0[b:b + 1],
-                                                   max_iter=50)
-        y_o = y_o[0].astype(np.float64)
-        mse_o[b] = ((y_o - Ys[b]) ** 2).mean()
+            f3 = helpers.oracle_fits(*args, max_iter=3)
+            f50 = helpers.oracle_fits(*args, max_iter=50)
+        C3_32[b], C3_64[b], C3_64x[b] = f3["C32"][0], f3["C64"][0], f3["C64x"][0]
+        m32[b], m64[b], m64x[b] = f50["mse32"][0], f50["mse64"][0], f50["mse64x"][0]
+        C_o = f50["C32"]
+        y_o = lbfgs_oracle.oracle.fused_root(Xs[b], ops[b:b + 1], ch[b:b + 1], C_o, Ys[b][None, :],
+                                             mode=1)["y"][0].astype(np.float64)
         y_g = fused.fused_root(_t(Xs[b]), _t(ops[b:b + 1]), _t(ch[b:b + 1]), _t(C_o), None, fwd_only=True,
                                raise_errors=False)["y"][0].cpu().numpy().astype(np.float64)
         floor[b] = ((y_g - y_o) ** 2).mean()
-    fin3 = keep & np.isfinite(C3).all(1) & np.isfinite(C3_o).all(1)
-    close3 = (np.abs(C3 - C3_o) <= 1e-3 * (1 + np.abs(C3_o))).all(1)
-    both = keep & np.isfinite(mse_g) & np.isfinite(mse_o) & np.isfinite(floor) & (info["err"].cpu().numpy() == 0)
-    tol = 1e-3 * mse_o + 1e-7 + 2 * np.sqrt(floor * mse_o) + floor
-    close = np.abs(mse_g - mse_o) <= tol
-    print(f"3 iterations: {fin3.sum()} finite, {close3[fin3].mean():.3f} within tolerance; "
-          f"50 iterations: {both.sum()} finite, {close[both].mean():.3f} within tolerance; outside: "
-          f"{[(b, int(n[b]), float(mse_g[b]), float(mse_o[b])) for b in np.nonzero(both & ~close)[0]]}", flush=True)
-    assert fin3.sum() > B // 2 and both.sum() > B // 2
-    assert close3[fin3].mean() >= 0.9
-    assert close[both].mean() >= 0.85
+    print(helpers.compare_constants_to_oracle(C3, dict(C32=C3_32, C64=C3_64, C64x=C3_64x), "ragged, 3 iterations"),
+          flush=True)
+    with np.errstate(invalid="ignore"):
+        extra = np.nan_to_num(2 * np.sqrt(floor * m32) + floor)
+    msg = helpers.compare_fit_to_oracle(mse_g, dict(mse32=m32, mse64=m64, mse64x=m64x), info["err"].cpu().numpy(),
+                                        "ragged, 50 iterations", extra_tol=extra)
+    print(msg, flush=True)
 
 
 def test_optimize_ragged_rejects_bad_sizes():
