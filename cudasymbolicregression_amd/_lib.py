@@ -79,6 +79,9 @@ def hip() -> ctypes.CDLL:
         _declare(lib, "sr_backward_mb_reads_only", I, [I64, I])
         _declare(lib, "sr_multi_backward_mb_workspace_bytes", I64, [I64, I64, I64, I64, I])
         _declare(lib, "sr_fused_workspace_bytes", I64, [I64, I64, I64, I64, I, I])
+        _declare(lib, "sr_fused_noalloc_max_bytes", I64, [])
+        _declare(lib, "sr_fused_root_noalloc_f32", I,
+                 [P, I64, I64, P, P, P, I64, I64, P, I, I, P, P, P, P, P, P, P])
         _declare(lib, "sr_lbfgs_fit", I,
                  [P, P, I64, I64, P, P, P, I64, I64, P, P, P, P, P, P, P, P, P, P, I, I, c_double, I, I, c_double,
                   P, P, I64, P])

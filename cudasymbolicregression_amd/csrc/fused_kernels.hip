@@ -34,6 +34,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <mutex>
+
 using namespace sr;
 
 namespace {
@@ -642,7 +644,7 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
                     int64_t B, const T *dy, int mode, int fwd_only, T *y_out, double *loss_out, T *grad_C,
                     T *grad_X, int32_t *err_b, int32_t *flags, void *workspace, int64_t workspace_bytes,
                     hipStream_t s, const int64_t *x_off = nullptr, const int64_t *y_off = nullptr,
-                    const int32_t *n_pts = nullptr) {
+                    const int32_t *n_pts = nullptr, bool one_stream = false) {
     if (M <= 0 || M > 128 || B < 0 || N <= 0 || n_x < 0 || n_x > MAX_NX_CODE + 1 || (mode != 0 && mode != 1))
         return -1;
     const bool ragged = x_off != nullptr || y_off != nullptr || n_pts != nullptr;
@@ -707,6 +709,14 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
     if (r) return r;
     if (M <= 4) {
         if ((r = fused_launch<SR_F0(T)>(bucket_args(0), L.grid[0], L.mode[0], s))) return r;
+    } else if (one_stream) {
+        // every bucket on the caller's stream, one after the other (no side
+        // streams, so nothing is created on first use)
+        if (L.grid[4] && (r = fused_launch<SR_F4(T)>(bucket_args(4), L.grid[4], L.mode[4], s))) return r;
+        if (L.grid[3] && (r = fused_launch<SR_F3(T)>(bucket_args(3), L.grid[3], L.mode[3], s))) return r;
+        if (L.grid[2] && (r = fused_launch<SR_F2(T)>(bucket_args(2), L.grid[2], L.mode[2], s))) return r;
+        if (L.grid[1] && (r = fused_launch<SR_F1(T)>(bucket_args(1), L.grid[1], L.mode[1], s))) return r;
+        if ((r = fused_launch<SR_F0(T)>(bucket_args(0), L.grid[0], L.mode[0], s))) return r;
     } else {
         // buckets 4+3, 2 and 1 on side streams, bucket 0 on the caller's
         // stream, joined back (also after a failed launch)
@@ -740,9 +750,62 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
     return (int)hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free form: the workspace is a file-scope device array of fixed
+// size instead of caller memory.  Each GPU holds its own copy, whose device
+// address is looked up once per device (hipGetSymbolAddress) and kept; the
+// array's name itself is only a host-side handle and is never passed to a
+// kernel.  Every call on a GPU shares that GPU's copy.
+// ---------------------------------------------------------------------------
+constexpr size_t FUSED_SCRATCH_BYTES = 8u << 20;
+__device__ __align__(256) unsigned char scratch[FUSED_SCRATCH_BYTES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+
+// Device address of this GPU's scratch array, or nullptr when the lookup
+// fails (*err receives its code).  Looked up on the first call per device.
+inline void *scratch_ptr(int *err) {
+    static std::mutex mu;
+    static void *ptrs[SCRATCH_MAX_DEVICES] = {nullptr};
+    int dev = 0;
+    *err = (int)hipGetDevice(&dev);
+    if (*err) return nullptr;
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) {
+        *err = -1;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (ptrs[dev] == nullptr) {
+        void *p = nullptr;
+        *err = (int)hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (*err) return nullptr;
+        ptrs[dev] = p;
+    }
+    return ptrs[dev];
+}
+
 }  // namespace
 
 extern "C" {
+
+int64_t sr_fused_noalloc_max_bytes(void) { return (int64_t)FUSED_SCRATCH_BYTES; }
+
+// sr_fused_root_f32 without a workspace argument (see include/sr_eval.h).
+int sr_fused_root_noalloc_f32(const float *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                              const float *C, int64_t M, int64_t B, const float *dy, int mode, int fwd_only,
+                              float *y_out, double *loss_out, float *grad_C, float *grad_X, int32_t *err_b,
+                              int32_t *flags, void *stream) {
+    if (M <= 0 || M > 128 || B < 0 || N <= 0 || n_x < 0) return -1;
+    const bool want_gx = grad_X != nullptr && !fwd_only;
+    if (fused_layout<float>(N, n_x, B, M, want_gx).total > FUSED_SCRATCH_BYTES) return -2;
+    int e = 0;
+    void *ws = scratch_ptr(&e);
+    if (ws == nullptr) return e ? e : -1;
+    return fused_root_code<float>(X, N, n_x, ops, ch, C, M, B, dy, mode, fwd_only, y_out, loss_out, grad_C, grad_X,
+                                  err_b, flags, ws, (int64_t)FUSED_SCRATCH_BYTES, (hipStream_t)stream, nullptr,
+                                  nullptr, nullptr, true);
+}
+
 
 int64_t sr_fused_workspace_bytes(int64_t N, int64_t n_x, int64_t B, int64_t M, int want_gx, int is_f64) {
     return is_f64 ? (int64_t)fused_layout<double>(N, n_x, B, M, want_gx != 0).total

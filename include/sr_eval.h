@@ -155,6 +155,25 @@ int sr_fused_root_f64(const double *X, int64_t N, int64_t n_x, const int64_t *op
                       double *y_out, double *loss_out, double *grad_C, double *grad_X, int32_t *err_b,
                       int32_t *flags, void *workspace, int64_t workspace_bytes, void *stream);
 
+/* Allocation-free form of sr_fused_root_f32, for callers that cannot
+ * allocate or pass device memory for a workspace (code under graph
+ * capture, tight serving loops).  No counterpart in the reference.  The
+ * workspace is a fixed scratch array inside the library, one copy per GPU
+ * (sr_fused_noalloc_max_bytes() bytes: 8 MiB, enough for about 380k
+ * expressions of at most 32 nodes without grad_X), held until the process
+ * ends.  Calls that need more (longer programs use global tapes, grad_X
+ * needs partial rows) return -2 without launching.  The call allocates
+ * nothing, creates no stream or event, and runs every kernel on `stream`.
+ * Every call on one GPU uses the same scratch: calls that overlap on a GPU
+ * (several streams or host threads at once) race and give wrong results;
+ * serialise them.  Otherwise the arguments and results are those of
+ * sr_fused_root_f32. */
+int64_t sr_fused_noalloc_max_bytes(void);
+int sr_fused_root_noalloc_f32(const float *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                              const float *C, int64_t M, int64_t B, const float *dy, int mode, int fwd_only,
+                              float *y_out, double *loss_out, float *grad_C, float *grad_X, int32_t *err_b,
+                              int32_t *flags, void *stream);
+
 /* Ragged form of sr_fused_root_*: expression b has its own n_pts[b] points
  * at X + x_off[b] (rows of n_x values) and its dy (target in mode 1) and y
  * values at offset y_off[b]; loss_out[b] is the mean over its own points.
