@@ -25,9 +25,13 @@ tuples labelled by the plain L-BFGS, from a disjoint sample of problems.
 Also reported: the roofline of the fitting kernel (algorithmic FP32 slots per
 closure x closures / kernel time, against the FP32 peak), the evaluate
 measurement at 65536 expressions x 1024 points (fused root-only, and the API
-full-cache forward/backward), the uneven-batching fit of 32768 problems with their own
-datasets, and a CPU baseline (the oracle's restatement of the
-reference's CPU algorithm on a bounded sample, single core).
+full-cache forward/backward, every result checked against the FP64
+evaluator), 16384 candidates fitted to one shared noisy target, the
+uneven-batching fit of 32768 problems with their own datasets, the fit from
+the true constants (the ceiling of any warm start), closures-per-problem
+histograms, the measured cost of a transcendental in FMA slots, and a CPU
+baseline: the reference's own OpenMP evaluation units with LBFGS_batch on a
+bounded sample, on every host core this process may use.
 """
 from __future__ import annotations
 
@@ -61,18 +65,28 @@ V -> C [0.25]
 """
 
 # Algorithmic FP32 slots per node per point (forward, backward) with a
-# transcendental counted as T = 4 slots; interpreter overhead earns nothing.
-T_COST = 4
-OPCOST = {
-    0: (0, 0), 1: (0, 1),
-    **{c: (0, 0) for c in range(2, 9)},
-    10: (T_COST + 2, T_COST + 3), 11: (T_COST + 2, T_COST + 3), 12: (T_COST + 1, 1), 13: (T_COST + 1, T_COST + 1),
-    14: (1, 2), 15: (T_COST, T_COST + 2), 16: (T_COST + 2, T_COST + 3), 17: (T_COST + 2, T_COST + 3),
-    18: (T_COST + 2, T_COST + 3), 19: (T_COST + 2, T_COST + 3), 20: (T_COST + 2, T_COST + 3),
-    21: (T_COST + 2, T_COST + 3), 22: (T_COST + 2, T_COST + 3), 23: (1, 0), 24: (1, 0), 25: (T_COST + 1, T_COST + 1),
-    26: (T_COST + 1, T_COST + 1), 27: (1, 1), 28: (T_COST, T_COST + 2), 29: (2, 3), 30: (2, 3), 31: (3, 4),
-    40: (1, 2), 41: (1, 2), 42: (1, 2), 43: (T_COST + 1, T_COST + 3), 44: (2 * T_COST + 2, 2 * T_COST + 4),
-}
+# hardware transcendental counted as T slots; interpreter overhead earns
+# nothing.  T is measured on the GPU at the start of every run
+# (sr_probe_valu_rate: issue rate of FMA / issue rate of v_exp, v_log,
+# v_sin, v_rcp, v_sqrt, v_rsq); 1.8 on MI355X, so the table is rebuilt with
+# the measured value.
+T_COST = 1.8
+
+
+def opcost_table(T):
+    return {
+        0: (0, 0), 1: (0, 1),
+        **{c: (0, 0) for c in range(2, 9)},
+        10: (T + 2, T + 3), 11: (T + 2, T + 3), 12: (T + 1, 1), 13: (T + 1, T + 1),
+        14: (1, 2), 15: (T, T + 2), 16: (T + 2, T + 3), 17: (T + 2, T + 3),
+        18: (T + 2, T + 3), 19: (T + 2, T + 3), 20: (T + 2, T + 3),
+        21: (T + 2, T + 3), 22: (T + 2, T + 3), 23: (1, 0), 24: (1, 0), 25: (T + 1, T + 1),
+        26: (T + 1, T + 1), 27: (1, 1), 28: (T, T + 2), 29: (2, 3), 30: (2, 3), 31: (3, 4),
+        40: (1, 2), 41: (1, 2), 42: (1, 2), 43: (T + 1, T + 3), 44: (2 * T + 2, 2 * T + 4),
+    }
+
+
+OPCOST = opcost_table(T_COST)
 LOSS_SLOTS = 5  # d = y - Y, d*d, accumulate, 2*d, *1/N
 FP32_PEAK_TFLOPS = 157.3
 VALU_LANE_PEAK = 256 * 4 * 16 * 2.4e9  # unpacked FP32 lane-instructions/s (3.93e13)
@@ -85,11 +99,38 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def expression_cost(ops_row) -> int:
+def expression_cost(ops_row) -> float:
     return sum(sum(OPCOST.get(int(o), (0, 1 if int(o) >= 50 else 0))) for o in ops_row if int(o) != 0)
 
 
-def bench_evaluate(dev, steps: int, seed: int = 1, B: int = 65536, N: int = 1024):
+def measure_trans_cost(dev) -> dict:
+    """Issue rates of FP32 FMA and of the hardware transcendentals
+    (sr_probe_valu_rate), and their ratio: the cost of a transcendental in
+    FMA slots."""
+    import torch
+
+    from cudasymbolicregression_amd import _lib
+
+    lib = _lib.hip()
+    blocks, iters = 256 * 32, 4096
+    out = torch.empty(blocks * 256, device=dev)
+    rates = {}
+    for k, name in enumerate(["fma", "exp", "log", "sin", "rcp", "sqrt", "rsq"]):
+        for _ in range(2):  # the second launch is timed
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            _lib.check_status(lib.sr_probe_valu_rate(k, blocks, iters, _lib.ptr(out), _lib.stream_of(dev)), "probe")
+            b.record()
+            torch.cuda.synchronize()
+        ops = blocks * 256 * lib.sr_probe_valu_ops_per_thread(iters)
+        rates[name] = ops / (a.elapsed_time(b) / 1e3)
+    trans = [rates[n] for n in rates if n != "fma"]
+    cost = rates["fma"] / (sum(trans) / len(trans))
+    return dict(trans_cost=round(cost, 2), fma_lane_ops_per_s=round(rates["fma"], -9),
+                trans_lane_ops_per_s={n: round(r, -9) for n, r in rates.items() if n != "fma"})
+
+
+def bench_evaluate(dev, steps: int, seed: int = 1, B: int = 65536, N: int = 1024, fp64_report: bool = True):
     """Plain evaluate forward+backward at scale: B G2 expressions (padded 64,
     n_x = 3), X ~ U(-3,3) (N,3), C ~ N(0,1) on the constant slots, upstream
     dL/dy ~ N(0,1) on the root output; one fused root-only forward+backward
@@ -124,6 +165,37 @@ def bench_evaluate(dev, steps: int, seed: int = 1, B: int = 65536, N: int = 1024
               + B * M * 4 + X.numel() * 4)
     slots = float(sum(expression_cost(row) for row in ops.cpu().tolist())) * N
     t = ms / 1e3
+    fp64 = None
+    if fp64_report:
+        # every result against the FP64 evaluator on the same inputs (untimed)
+        y32, gC32, gX32, e32 = r["y"].clone(), r["grad_C"].clone(), r["grad_X"].clone(), r["err"].clone()
+        del out, r
+        r64 = fused.fused_root(X.double(), ops, ch, C.double(), dy.double(), mode=0, want_grad_X=True,
+                               raise_errors=False)
+        ok = (e32 == 0) & (r64["err"] == 0)
+        # grad_X sums over every expression, so one non-finite contribution
+        # makes a column non-finite: compare it over the expressions whose
+        # outputs and constant gradients are finite in both precisions
+        fin = ok & torch.isfinite(y32).all(1) & torch.isfinite(r64["y"]).all(1) \
+            & torch.isfinite(gC32).all(1) & torch.isfinite(r64["grad_C"]).all(1)
+        sel = fin.nonzero().flatten()
+        gx32 = fused.fused_root(X, ops[sel], ch[sel], C[sel], dy[sel], mode=0, want_grad_X=True,
+                                raise_errors=False)["grad_X"]
+        gx64 = fused.fused_root(X.double(), ops[sel], ch[sel], C[sel].double(), dy[sel].double(), mode=0,
+                                want_grad_X=True, raise_errors=False)["grad_X"]
+        fp64 = dict(error_codes_equal=int((e32 == r64["err"]).sum()), error_codes_fp32=int((e32 != 0).sum()),
+                    error_codes_fp64=int((r64["err"] != 0).sum()),
+                    codes_fp32={int(k): int(v) for k, v in zip(*torch.unique(e32[e32 != 0], return_counts=True))},
+                    y=rel_error_report(y32[ok], r64["y"][ok]),
+                    grad_C=rel_error_report(gC32[ok], r64["grad_C"][ok]),
+                    grad_X_all=rel_error_report(gX32, r64["grad_X"]),
+                    grad_X_finite_expressions=dict(expressions=int(sel.numel()), **rel_error_report(gx32, gx64)),
+                    note="y and grad_C over expressions without an error code in either precision; grad_X_all sums "
+                         "every expression's contribution, grad_X_finite_expressions those whose y and grad_C are "
+                         "finite in both precisions")
+        del r64, y32, gC32
+        r = dict(err=e32)
+        torch.cuda.empty_cache()
     return dict(workload=f"G2 B={B}, padded 64, N={N}, n_x=3, fused fwd+bwd with grad_X", ms=round(ms, 4),
                 hbm=dict(achieved_GBps=round(nbytes / t / 1e9, 1), peak_GBps=HBM_PEAK_GBS,
                          frac=round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4), bytes=nbytes,
@@ -132,7 +204,7 @@ def bench_evaluate(dev, steps: int, seed: int = 1, B: int = 65536, N: int = 1024
                           frac=round(slots / t / 1e12 / FP32_PEAK_TFLOPS, 5),
                           frac_unpacked_lane_peak=round(slots / t / VALU_LANE_PEAK, 4), slots=slots),
                 mean_len=round(float((ops != 0).sum(1).double().mean()), 3),
-                errors=int((r["err"] != 0).sum()))
+                errors=int((r["err"] != 0).sum()), fp64_error=fp64)
 
 
 def bench_full_cache(dev, steps: int, seed: int = 1, B: int = 65536, N: int = 1024):
@@ -268,6 +340,99 @@ def bench_ragged(dev, steps: int, seed: int = 3, B: int = 32768):
                           frac=round(slots / (ms / 1e3) / 1e12 / FP32_PEAK_TFLOPS, 5), slots=slots))
 
 
+def rel_error_report(a, b, tol: float = 1e-5) -> dict:
+    """Error of FP32 results `a` against FP64 results `b` (same shape):
+    where both are finite, quantiles of |a - b| / |b| (|b| > 0) and the
+    share within `tol` relative; where either is not, whether the
+    non-finite masks agree."""
+    import torch
+
+    a, b = a.double().flatten(), b.double().flatten()
+    fa, fb = torch.isfinite(a), torch.isfinite(b)
+    both = fa & fb
+    d = (a[both] - b[both]).abs()
+    ref = b[both].abs()
+    nz = ref > 0
+    rel = torch.sort(d[nz] / ref[nz])[0]
+    q = lambda f: float(rel[min(rel.numel() - 1, int(f * rel.numel()))]) if rel.numel() else 0.0  # noqa: E731
+    return dict(values=int(a.numel()), finite_both=int(both.sum()),
+                within_rel_tol=round(float((d <= tol * ref).double().mean()) if d.numel() else 1.0, 7),
+                rel_tol=tol, rel_p50=q(0.5), rel_p99=q(0.99), rel_p999=q(0.999), rel_max=q(1.0),
+                abs_max=float(d.max()) if d.numel() else 0.0,
+                exact_zero_ref_nonzero_value=int(((ref == 0) & (d > 0)).sum()),
+                nonfinite_mask_mismatch=int((fa != fb).sum()),
+                nan_mask_mismatch=int((torch.isnan(a) != torch.isnan(b)).sum()))
+
+
+def closure_histogram(ncl) -> dict:
+    """Problems per closures-per-problem bin (bin k holds [edge_k, edge_k+1))."""
+    import torch
+
+    edges = [1, 2, 3, 5, 9, 17, 33, 65, 129, 257, 513, 1025]
+    c = ncl.to(torch.int64).cpu()
+    out = {}
+    for lo, hi in zip(edges, edges[1:] + [1 << 40]):
+        n = int(((c >= lo) & (c < hi)).sum())
+        out[f"{lo}-{hi - 1}" if hi < (1 << 40) else f">={lo}"] = n
+    out["0"] = int((c == 0).sum())
+    return out
+
+
+def bench_c3(dev, steps: int, seed: int = 2, B: int = 16384, N: int = 512):
+    """Plain batched L-BFGS, B candidates fitted to one shared noisy target:
+    G2 expressions (padded 64) on N points X ~ U(-3, 3)^3, the target a
+    seed-drawn G2 expression with >= 1 constant, constants ~ N(0, 2^2), plus
+    N(0, 0.01^2) noise; randn starts, history 10, max_iter 100, strong Wolfe,
+    tol_grad 1e-7, every problem run to the reference's own stop rules.  The
+    FP32 loss each fit ends with is compared with the FP64 evaluation of the
+    same constants."""
+    import torch
+
+    import cudasymbolicregression_amd as st
+    from cudasymbolicregression_amd import fused
+    from cudasymbolicregression_amd.optimization import _fit
+
+    gen = torch.Generator().manual_seed(seed)
+    X = (torch.rand(N, 3, generator=gen) * 6 - 3).to(dev)
+    g = st.ProbabilisticContextFreeGrammar(G2, "E", 64, 3, torch.device("cpu"))
+    torch.manual_seed(seed)
+    while True:
+        o, c = g.sample(1)
+        if not (o == 1).any():
+            continue
+        Ct = torch.where(o == 1, torch.randn(o.shape, generator=gen) * 2, torch.zeros(o.shape))
+        y = fused.fused_root(X.double(), o.to(dev), c.to(dev), Ct.to(dev).double(), None, fwd_only=True,
+                             raise_errors=False)["y"][0]
+        if torch.isfinite(y).all() and y.var() > 1e-3 and y.abs().max() < 1e4:
+            break
+    target = (y + 0.01 * torch.randn(N, generator=gen, dtype=torch.float64).to(dev)).float()
+    tstr = g.to_string(g.postfix_to_infix(o, 128))[0]
+    ops, ch = g.sample(B)
+    ops, ch = ops.to(dev), ch.to(dev)
+    C0 = st.create_constants(ops, method="randn")
+    Y = target.unsqueeze(0).expand(B, -1).contiguous()
+    run = lambda: _fit(X, Y, ops, ch, C0, 100, 10, 1e-7, "strong_wolfe")  # noqa: E731
+    ms, (C, loss, iters, err, _, ncl) = timed(run, steps, 1)
+    y64 = fused.fused_root(X.double(), ops, ch, C.double(), None, fwd_only=True, raise_errors=False)["y"]
+    mse64 = ((y64 - Y.double()) ** 2).mean(1)
+    ok = (err == 0) & torch.isfinite(loss) & torch.isfinite(mse64)
+    per_pt = torch.tensor([expression_cost(row) + LOSS_SLOTS for row in ops.cpu().tolist()], dtype=torch.float64)
+    slots = float((per_pt * ncl.cpu().double()).sum()) * N
+    noise_var = 1e-4
+    best = float(mse64[ok].min()) if bool(ok.any()) else float("nan")
+    return dict(workload=f"G2 B={B} candidates (padded 64) fitted to one shared noisy target on N={N} points, n_x=3, "
+                         f"randn starts, max_iter 100, history 10, strong Wolfe, tol_grad 1e-7",
+                target=tstr, ms=round(ms, 3), fits_per_s=round(B / (ms / 1e3), 1),
+                mean_iters=round(float(iters.double().mean()), 2), closures=int(ncl.sum()),
+                problems_without_error=int((err == 0).sum()),
+                fp64_loss_error=rel_error_report(loss[ok], mse64[ok], tol=1e-5),
+                best_mse_fp64=best, noise_variance=noise_var,
+                within_2x_noise=int((mse64[ok] <= 2 * noise_var).sum()),
+                closures_per_problem=closure_histogram(ncl),
+                valu=dict(achieved_TFLOPs=round(slots / (ms / 1e3) / 1e12, 4), peak_TFLOPs=FP32_PEAK_TFLOPS,
+                          frac=round(slots / (ms / 1e3) / 1e12 / FP32_PEAK_TFLOPS, 5), slots=slots))
+
+
 def fit(w, C_start, stop=True):
     from cudasymbolicregression_amd.optimization import _fit
 
@@ -275,33 +440,188 @@ def fit(w, C_start, stop=True):
                 stop_loss=w["stop"] if stop else None)
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or "unknown"
+
+
+def host_threads() -> int:
+    """The host cores this process may use: its affinity set, capped by
+    OMP_NUM_THREADS where that is set (the machine's share for this GPU)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def reference_cpu_fit(X, Y, ops_bm, ch_bm, C0_bm, stop=None, max_iter: int = 100, threads: int = 1):
+    """The reference's CPU algorithm: its OpenMP evaluation units
+    (src/evaluation_kernels_cpu.cpp:8-662, validation, forward over the full
+    (M, N, B) cache, atomic reverse sweep) compiled from its own sources into
+    oracle/_ref/libsrref.so, driven as symbolic_torch/optimization.py:9-112
+    drives them (root-row gather, mean-squared error, dense cache gradient,
+    masked dL/dC) by LBFGS_batch in torch CPU FP32.  stop (B,) float64: a
+    problem whose loss reaches it gets a zero gradient from then on, so the
+    reference's own done test (gradient norm below tol) ends it: the same
+    stop rule as the device step.
+
+    A backward error (204, a divisor exactly 0 under a nonzero adjoint) sets
+    the reference's one error flag, after which its sweep skips every
+    remaining (point, expression) of the batch and its optimize raises.  To
+    keep the batch running as the device path does (errors per problem),
+    the closure then finds the failing problems with one backward call each,
+    zeroes their upstream gradient and their dL/dC from then on (so the
+    done test ends them) and repeats the batch backward.
+    Returns (C (B, M), seconds, closures, failed problems)."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    from cudasymbolicregression_amd.lbfgs_batch import LBFGS_batch
+
+    ref = oracle.ref()
+    torch.set_num_threads(threads)
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(threads)
+    B, M = ops_bm.shape
+    N, n_x = X.shape
+    ops = np.ascontiguousarray(ops_bm.T)
+    ch = np.ascontiguousarray(ch_bm.transpose(1, 0, 2))
+    mask = torch.from_numpy(ops == 1)
+    root = np.maximum((ops != 0).sum(0) - 1, 0)
+    bidx = np.arange(B)
+    Xc = np.ascontiguousarray(X, np.float32)
+    Yt = torch.from_numpy(np.ascontiguousarray(Y, np.float32))
+    stop_t = None if stop is None else torch.from_numpy(np.asarray(stop, np.float64))
+    x = torch.from_numpy(np.ascontiguousarray(C0_bm.T, np.float32)).clone().requires_grad_()
+    opt = LBFGS_batch([x], max_iter=max_iter, history_size=10, tolerance_grad=1e-7, line_search_fn="strong_wolfe")
+    cache = np.empty((M, N, B), np.float32)
+    gcache = np.empty((M, N, B), np.float32)
+    gC = np.empty((M, B), np.float32)
+    gX = np.empty((N, n_x), np.float32)
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    calls = [0]
+    failed = np.zeros(B, bool)
+
+    def find_failing(gup):
+        out = np.zeros(B, bool)
+        for b in np.nonzero(~failed & (gup != 0).any(1))[0]:
+            cb = np.ascontiguousarray(cache[:, :, b:b + 1])
+            gb = np.zeros((M, N, 1), np.float32)
+            gb[root[b], :, 0] = gup[b]
+            ob, cb_ = np.ascontiguousarray(ops[:, b:b + 1]), np.ascontiguousarray(ch[:, b:b + 1])
+            gc1, gx1 = np.zeros((M, 1), np.float32), np.zeros((N, n_x), np.float32)
+            out[b] = ref.ref_backward_f32(P(gb), P(cb), P(ob), P(cb_), M, 1, N, n_x, P(gc1), P(gx1)) != 0
+        return out
+
+    def closure():
+        opt.zero_grad(set_to_none=True)
+        calls[0] += 1
+        C = np.ascontiguousarray(x.detach().numpy())
+        ref.ref_validate(P(ops), P(ch), M, B)
+        ref.ref_forward_f32(P(Xc), N, n_x, P(ops), P(ch), P(C), M, B, P(cache))
+        yr = torch.from_numpy(cache[root, :, bidx].copy()).requires_grad_()
+        loss = ((yr - Yt) ** 2).mean(1)
+        loss.sum().backward()
+        gup = yr.grad.numpy()
+        gup[failed] = 0.0
+        while True:
+            gcache.fill(0.0)
+            gcache[root, :, bidx] = gup
+            gC.fill(0.0)
+            gX.fill(0.0)
+            if ref.ref_backward_f32(P(gcache), P(cache), P(ops), P(ch), M, B, N, n_x, P(gC), P(gX)) == 0:
+                break
+            new = find_failing(gup)
+            if not new.any():
+                raise RuntimeError("reference backward error that no single problem reproduces")
+            failed[new] = True
+            gup[new] = 0.0
+        g = torch.from_numpy(gC.copy())
+        g[~mask] = 0.0
+        g[:, torch.from_numpy(failed)] = 0.0
+        if stop_t is not None:
+            g[:, loss.detach().double() <= stop_t] = 0.0
+        x.grad = g
+        return loss.detach()
+
+    t0 = time.perf_counter()
+    with np.errstate(all="ignore"):
+        opt.step(closure)
+    return x.detach().numpy().T.copy(), time.perf_counter() - t0, calls[0], int(failed.sum())
+
+
 def cpu_baseline(w, sample: int):
-    """The oracle's CPU restatement of the reference algorithm (randn starts,
-    every iteration, no early stop) on `sample` problems."""
+    """The reference's CPU path on a bounded sample of the plain workload
+    (randn starts) on this host's cores, timed twice: with the device step's
+    stop rule (value) and as the reference runs it, to max_iter or its own
+    tolerances.  Without the reference's compiled units (oracle/_ref), the
+    oracle's single-thread restatement (oracle/lbfgs_oracle.py +
+    oracle/eval_oracle.c) is timed instead."""
     import numpy as np
     import torch
 
     from cudasymbolicregression_amd import warmstart
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import lbfgs_oracle
+    import oracle
 
-    X = w["X"].cpu().numpy()
+    Xn = w["X"].cpu().numpy()
     ops = w["ops"][:sample].cpu().numpy()
     ch = w["ch"][:sample].cpu().numpy()
     Y = w["Y"][:sample].cpu().numpy()
     C0 = w["C0"][:sample].cpu().numpy()
+    judge = lambda C: int((warmstart.rel_mse_fp64(  # noqa: E731
+        w["X"], w["Y"][:sample], w["ops"][:sample], w["ch"][:sample],
+        torch.from_numpy(np.ascontiguousarray(C)).to(w["X"].device)) <= 1e-6).sum())
+    where = f"{cpu_model()}"
+    if oracle.ref_available():
+        thr = host_threads()
+        stop = w["stop"][:sample].cpu().numpy()
+        torch_threads = torch.get_num_threads()
+        try:
+            C_s, dt_s, calls_s, fail_s = reference_cpu_fit(Xn, Y, ops, ch, C0, stop=stop, threads=thr)
+            C_f, dt_f, calls_f, fail_f = reference_cpu_fit(Xn, Y, ops, ch, C0, stop=None, threads=thr)
+        finally:
+            torch.set_num_threads(torch_threads)
+        fit_s, fit_f = judge(C_s), judge(C_f)
+        return dict(value=round(fit_s / dt_s, 3), unit="expressions fitted/s", cores=thr, kind="reference",
+                    cpu_model=where,
+                    sample=f"{sample} problems of the same workload (randn starts, max_iter 100, strong Wolfe): the "
+                           f"reference's evaluation units (src/evaluation_kernels_cpu.cpp, OpenMP, {thr} threads) "
+                           f"under LBFGS_batch in torch CPU FP32 ({thr} threads); with the device step's stop rule "
+                           f"{dt_s:.1f} s, {fit_s} fitted, {calls_s} batched closures",
+                    full_fit=dict(value=round(fit_f / dt_f, 3), seconds=round(dt_f, 2), fitted=fit_f,
+                                  closures=calls_f, backward_error_problems=fail_f,
+                                  note="the reference's own stop rules only (max_iter, tol_grad)"),
+                    backward_error_problems=fail_s,
+                    error_note="the reference's optimize raises on a backward error anywhere in the batch; these "
+                               "problems were frozen instead (as the device path reports errors per problem)",
+                    seconds=round(dt_s, 2), fitted=fit_s, problems=sample)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import lbfgs_oracle
+
     t0 = time.perf_counter()
     with np.errstate(all="ignore"):
-        C, _, _, calls = lbfgs_oracle.optimize(X, Y, ops, ch, C0, max_iter=100)
+        C, _, _, calls = lbfgs_oracle.optimize(Xn, Y, ops, ch, C0, max_iter=100)
     dt = time.perf_counter() - t0
-    r = warmstart.rel_mse_fp64(w["X"], w["Y"][:sample], w["ops"][:sample], w["ch"][:sample],
-                               torch.from_numpy(C).to(w["X"].device))
-    fitted = int((r <= 1e-6).sum())
-    return dict(value=fitted / dt, unit="expressions fitted/s", cores=1, kind="port",
+    fitted = judge(C)
+    return dict(value=round(fitted / dt, 3), unit="expressions fitted/s", cores=1, kind="port", cpu_model=where,
                 sample=f"{sample} problems of the same workload, randn starts, full fits (max_iter 100), {dt:.1f} s, "
                        f"{fitted} fitted, {calls} batched closures; oracle/lbfgs_oracle.py + oracle/eval_oracle.c, "
-                       f"single thread")
+                       f"single thread (the reference's compiled CPU units are absent here)")
 
 
 def timed(fn, steps: int, warmup: int):
@@ -330,10 +650,15 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=5, help="test problems (training uses seed - 1)")
     ap.add_argument("--train-tuples", type=int, default=2_000_000)
     ap.add_argument("--epochs", type=int, default=20)
-    ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--cpu-sample", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-evaluate", action="store_true")
     ap.add_argument("--no-ragged", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-ceiling", action="store_true", help="no fit from the true constants")
+    ap.add_argument("--no-probe", action="store_true",
+                    help="no transcendental-rate probe: price a transcendental at the last measured T")
+    ap.add_argument("--no-fp64-report", action="store_true")
     ap.add_argument("--no-warm", action="store_true",
                     help="plain fits only: no MLP training and no warm-start step (value is the plain rate)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -359,6 +684,12 @@ def main() -> int:
     from cudasymbolicregression_amd import warmstart
 
     dev = torch.device("cuda:0")
+    global OPCOST
+    trans = None
+    if not args.no_probe:
+        trans = measure_trans_cost(dev)
+        OPCOST = opcost_table(trans["trans_cost"])
+        log(f"transcendental cost: {trans['trans_cost']} FMA slots ({trans})")
     t0 = time.time()
     g = st.ProbabilisticContextFreeGrammar(G2, "E", 64, 3, torch.device("cpu"))
     gen = torch.Generator().manual_seed(0)
@@ -418,16 +749,36 @@ def main() -> int:
     per_point = torch.tensor([expression_cost(row) + LOSS_SLOTS for row in ops.cpu().tolist()], dtype=torch.float64)
     flops = float((per_point * ncl_r.cpu().double() * X.shape[0]).sum())
     achieved = flops / (fit_ms / 1000.0) / 1e12
-    roofline = dict(bound="mfma", achieved=round(achieved, 4), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+    roofline = dict(bound="valu", achieved=round(achieved, 4), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=round(achieved / FP32_PEAK_TFLOPS, 5), traffic=FIT_TRAFFIC_PROFILED,
                     traffic_source="profiles/fit_hbm_traffic.md: rocprofv3 FETCH_SIZE/WRITE_SIZE of the plain-start "
                                    "fit call (the same kernels on the same problems), not measured live",
                     kernel="sr_lbfgs_fit: the k_fit bucket kernels of one call (concurrent)",
-                    flop_per_launch=flops, frac_unpacked_lane_peak=round(flops / (fit_ms / 1e3) / VALU_LANE_PEAK, 5))
+                    flop_per_launch=flops, frac_unpacked_lane_peak=round(flops / (fit_ms / 1e3) / VALU_LANE_PEAK, 5),
+                    slot_unit="one FP32 VALU lane operation; a transcendental counts trans_cost slots",
+                    trans_cost=trans["trans_cost"] if trans else T_COST,
+                    trans_cost_source="measured in this run (sr_probe_valu_rate)" if trans else "last measurement")
+
+    # ---- ceiling: the same fit from the true constants
+    ceiling = None
+    if not args.no_ceiling:
+        ms_t, (C_t, loss_t, iters_t, err_t, _, ncl_t) = timed(lambda: fit(w, w["Ctrue"]), args.steps, args.warmup)
+        fit_t = int((warmstart.rel_mse_fp64(X, Y, ops, ch, C_t) <= 1e-6).sum())
+        ceiling = {"ms": round(ms_t, 3), "fitted": fit_t, "fitted_per_s": round(fit_t / (ms_t / 1e3), 1),
+                   "over_plain": round(fit_t / (ms_t / 1e3) / plain_value, 3) if plain_value > 0 else None,
+                   "mean_iters": round(float(iters_t.double().mean()), 2), "closures": int(ncl_t.sum()),
+                   "problems_without_error": int((err_t == 0).sum()),
+                   "note": "fit started from the true constants: what a perfect warm start would reach"}
+        log(f"ceiling (true constants): {ms_t:.2f} ms/step, {fit_t} fitted")
+    hist = {"plain": closure_histogram(ncl_p)}
+    if warm is not None:
+        hist["warm"] = closure_histogram(ncl_w)
+    if ceiling is not None:
+        hist["true_constants"] = closure_histogram(ncl_t)
 
     ev = None
     if not args.no_evaluate:
-        ev = bench_evaluate(dev, steps=5)
+        ev = bench_evaluate(dev, steps=5, fp64_report=not args.no_fp64_report)
         log(f"evaluate (65536 x 1024): {ev['ms']:.3f} ms, {ev['hbm']['achieved_GBps']} GB/s, VALU frac {ev['valu']['frac']}")
 
     fc = None
@@ -435,6 +786,12 @@ def main() -> int:
         fc = bench_full_cache(dev, steps=3)
         log(f"full-cache evaluate (65536 x 1024): forward {fc['forward']['ms']:.2f} ms ({fc['forward']['achieved_GBps']} GB/s), "
             f"backward {fc['backward']['ms']:.2f} ms ({fc['backward']['achieved_GBps']} GB/s)")
+
+    c3 = None
+    if not args.no_c3:
+        c3 = bench_c3(dev, steps=2)
+        log(f"C3 (16384 candidates, shared noisy target): {c3['ms']:.1f} ms, {c3['fits_per_s']:.0f} fits/s, "
+            f"best FP64 MSE {c3['best_mse_fp64']:.3g}")
 
     rg = None
     if not args.no_ragged:
@@ -471,11 +828,15 @@ def main() -> int:
                       "problems_without_error": int((err_p == 0).sum())},
             "warm_over_plain": round(value / plain_value, 4) if warm is not None and plain_value > 0 else None,
             "training": tstats,
+            "ceiling": ceiling,
+            "closures_per_problem": hist,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "trans_cost": trans,
         "evaluate": ev,
         "evaluate_full_cache": fc,
+        "shared_target_c3": c3,
         "ragged": rg,
     }
     print(json.dumps(out))

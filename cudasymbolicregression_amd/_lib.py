@@ -80,6 +80,8 @@ def hip() -> ctypes.CDLL:
         _declare(lib, "sr_multi_backward_mb_workspace_bytes", I64, [I64, I64, I64, I64, I])
         _declare(lib, "sr_fused_workspace_bytes", I64, [I64, I64, I64, I64, I, I])
         _declare(lib, "sr_fused_noalloc_max_bytes", I64, [])
+        _declare(lib, "sr_probe_valu_ops_per_thread", I, [I])
+        _declare(lib, "sr_probe_valu_rate", I, [I, I64, I, P, P])
         _declare(lib, "sr_fused_root_noalloc_f32", I,
                  [P, I64, I64, P, P, P, I64, I64, P, I, I, P, P, P, P, P, P, P])
         _declare(lib, "sr_lbfgs_fit", I,

@@ -241,6 +241,16 @@ int sr_fit_num_variants(int bucket);
 int sr_fit_default_variant(int bucket);
 int sr_fit_variant_info(int bucket, int variant, int *waves, int *ppt);
 
+/* Measurement probe (no counterpart in the reference): `blocks` x 256
+ * threads each run 8 independent chains of `iters` FP32 operations of one
+ * kind, 0 FMA, 1 v_exp_f32, 2 v_log_f32, 3 v_sin_f32, 4 v_rcp_f32,
+ * 5 v_sqrt_f32, 6 v_rsq_f32, and write one float each to out
+ * (blocks * 256 floats).  Timing a launch gives the issue rate of the kind;
+ * rate(FMA) / rate(kind) is its cost in FMA slots.
+ * sr_probe_valu_ops_per_thread(iters) = operations per thread. */
+int sr_probe_valu_ops_per_thread(int iters);
+int sr_probe_valu_rate(int kind, int64_t blocks, int iters, float *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
