@@ -13,7 +13,7 @@ import torch  # noqa: F401
 from ._C import Operator, ProbabilisticContextFreeGrammar, get_arity
 from .utils import create_constants
 from . import evaluation
-from .optimization import optimize, optimize_ragged
+from .optimization import optimize, optimize_multistart, optimize_ragged
 
 __all__ = [
     "evaluation",
@@ -21,6 +21,7 @@ __all__ = [
     "ProbabilisticContextFreeGrammar",
     "create_constants",
     "optimize",
+    "optimize_multistart",
     "optimize_ragged",
     "get_arity",
 ]

@@ -87,6 +87,9 @@ def hip() -> ctypes.CDLL:
         _declare(lib, "sr_lbfgs_fit", I,
                  [P, P, I64, I64, P, P, P, I64, I64, P, P, P, P, P, P, P, P, P, P, I, I, c_double, I, I, c_double,
                   P, P, I64, P])
+        _declare(lib, "sr_lbfgs_fit_multi", I,
+                 [P, P, I64, I64, P, P, P, I64, I64, I64, P, P, P, P, P, P, P, P, P, P, I, I, c_double, I, I,
+                  c_double, P, P, I64, P])
         _declare(lib, "sr_lbfgs_workspace_bytes", I64, [I64, I64])
         _declare(lib, "sr_fit_num_buckets", I, [])
         _declare(lib, "sr_fit_num_variants", I, [I])

@@ -183,8 +183,9 @@ def tune_fit(X: torch.Tensor, Y: torch.Tensor, Ops: torch.Tensor, Ch: torch.Tens
     note(best, best_t)
     for _ in range(rounds):
         for b in range(len(best)):
+            start = best[b]  # already timed with the other buckets as they are now
             for v in range(n_var[b]):
-                if v == best[b]:
+                if v == start:
                     continue
                 cfg = list(best)
                 cfg[b] = v

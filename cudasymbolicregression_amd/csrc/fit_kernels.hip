@@ -87,6 +87,7 @@ struct FitArgs {
     int32_t *work_counter;
     float *gtape;            // global tape slots (programs longer than 64 nodes)
     int stationary_exit;     // end a problem once its iteration is an exact fixed point
+    int starts;              // fits per problem: item b = problem * starts + start
 };
 
 // FP64 inner product over the K live slots; lanes past K never contribute.
@@ -382,7 +383,10 @@ void k_fit(FitArgs a) {
     double *rho_l = reinterpret_cast<double *>(Yh + (size_t)hist * SL * WAVE);
     double *part = reinterpret_cast<double *>(hbase + (size_t)groups * hist_bytes(hist, SL));
     int *next_slot = reinterpret_cast<int *>(hbase + (size_t)groups * hist_bytes(hist, SL) + part_bytes(W, SL));
-    const int64_t count = (int64_t)*a.list_count;
+    // every listed problem expands into `starts` consecutive work items, so
+    // the fits of one problem run side by side over the same program,
+    // targets and points
+    const int64_t count = (int64_t)*a.list_count * a.starts;
     const double c1 = 1e-4, c2 = 0.9;
 
     int buf = 0;
@@ -398,8 +402,9 @@ void k_fit(FitArgs a) {
             __syncthreads();
         }
         if (i >= count) break;
-        const int64_t b = (int64_t)a.list[i];
-        const CProg<float> p = load_cprog<float>(a.ops, a.ch, a.C_init, b, a.M, a.n_x);
+        const int64_t pb = (int64_t)a.list[i / a.starts];       // problem
+        const int64_t b = pb * a.starts + (int64_t)(i % a.starts);  // item: its constants and results
+        const CProg<float> p = load_cprog<float>(a.ops, a.ch, a.C_init, pb, a.M, a.n_x, b);
         const bool writer = (wslot == 0);
         if (p.verr) {
             if (writer) {
@@ -427,10 +432,10 @@ void k_fit(FitArgs a) {
             }
             continue;
         }
-        const float *X = a.X + (a.x_off ? a.x_off[b] : 0);
-        const float *Yb = a.Y + (a.y_off ? a.y_off[b] : b * a.N);
-        const int64_t N = a.n_pts ? (int64_t)a.n_pts[b] : a.N;
-        const double stop = a.stop_loss ? a.stop_loss[b] : -1.0;
+        const float *X = a.X + (a.x_off ? a.x_off[pb] : 0);
+        const float *Yb = a.Y + (a.y_off ? a.y_off[pb] : pb * a.N);
+        const int64_t N = a.n_pts ? (int64_t)a.n_pts[pb] : a.N;
+        const double stop = a.stop_loss ? a.stop_loss[pb] : -1.0;
         const int K = p.K;
         // the wave's points form one tile group: load them once per problem
         const bool single = N <= (int64_t)W * WAVE * PPT;
@@ -888,18 +893,23 @@ int sr_fit_variant_info(int bucket, int variant, int *waves, int *ppt) {
     return 0;
 }
 
-// Fits the LEARNABLE_CONSTANT slots of B batch-first expressions to their
-// targets.  See include/sr_eval.h for the argument contract.
-int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
-                 const float *C_init, int64_t M, int64_t B, const int64_t *x_off, const int64_t *y_off,
-                 const int32_t *n_pts, const double *stop_loss, float *C_out, double *loss_out, int32_t *iters_out,
-                 int32_t *nclos_out, int32_t *err_out, int32_t *flags, int max_iter, int history, double tol_grad,
-                 int line_search, int ls_max_iter, double eps, const sr_fit_options *opts, void *workspace,
-                 int64_t ws_bytes, void *stream) {
+}  // extern "C"
+
+namespace {
+
+// One fit launch over B problems with `starts` fits each (see
+// sr_lbfgs_fit_multi in include/sr_eval.h).
+int fit_impl(const float *X, const float *Y, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+             const float *C_init, int64_t M, int64_t B, int64_t starts, const int64_t *x_off, const int64_t *y_off,
+             const int32_t *n_pts, const double *stop_loss, float *C_out, double *loss_out, int32_t *iters_out,
+             int32_t *nclos_out, int32_t *err_out, int32_t *flags, int max_iter, int history, double tol_grad,
+             int line_search, int ls_max_iter, double eps, const sr_fit_options *opts, void *workspace,
+             int64_t ws_bytes, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     if (M <= 0 || M > 128 || B < 0 || n_x < 0 || n_x > MAX_NX_CODE + 1 || history < 1 || history > MHMAX ||
         max_iter < 0 || ls_max_iter < 0 || (line_search != 0 && line_search != 1))
         return -1;
+    if (starts < 1 || B * starts > 0x7fffffff) return -1;
     int variant[NBUCKETS];
     for (int k = 0; k < NBUCKETS; ++k) {
         const int v = opts ? opts->variant[k] : -1;
@@ -928,6 +938,7 @@ int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const i
     a.max_iter = max_iter; a.history = history; a.tol_grad = tol_grad; a.line_search = line_search;
     a.ls_max_iter = ls_max_iter; a.eps = eps;
     a.stationary_exit = stationary_exit;
+    a.starts = (int)starts;
     a.gtape = M > 32 ? (float *)(ws + fit_gt_off(B)) : nullptr;
     auto bucket_args = [&](int k) {
         FitArgs r = a;
@@ -957,6 +968,35 @@ int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const i
     if ((r = launch_bucket(0, variant, bucket_args(0), s))) return fk.finish(r);
     if ((r = fk.finish(0))) return r;
     return (int)hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" {
+
+// Fits the LEARNABLE_CONSTANT slots of B batch-first expressions to their
+// targets.  See include/sr_eval.h for the argument contract.
+int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                 const float *C_init, int64_t M, int64_t B, const int64_t *x_off, const int64_t *y_off,
+                 const int32_t *n_pts, const double *stop_loss, float *C_out, double *loss_out, int32_t *iters_out,
+                 int32_t *nclos_out, int32_t *err_out, int32_t *flags, int max_iter, int history, double tol_grad,
+                 int line_search, int ls_max_iter, double eps, const sr_fit_options *opts, void *workspace,
+                 int64_t ws_bytes, void *stream) {
+    return fit_impl(X, Y, N, n_x, ops, ch, C_init, M, B, 1, x_off, y_off, n_pts, stop_loss, C_out, loss_out,
+                    iters_out, nclos_out, err_out, flags, max_iter, history, tol_grad, line_search, ls_max_iter, eps,
+                    opts, workspace, ws_bytes, stream);
+}
+
+// K fits per problem in one launch.  See include/sr_eval.h.
+int sr_lbfgs_fit_multi(const float *X, const float *Y, int64_t N, int64_t n_x, const int64_t *ops,
+                       const int64_t *ch, const float *C_init, int64_t M, int64_t B, int64_t starts,
+                       const int64_t *x_off, const int64_t *y_off, const int32_t *n_pts, const double *stop_loss,
+                       float *C_out, double *loss_out, int32_t *iters_out, int32_t *nclos_out, int32_t *err_out,
+                       int32_t *flags, int max_iter, int history, double tol_grad, int line_search, int ls_max_iter,
+                       double eps, const sr_fit_options *opts, void *workspace, int64_t ws_bytes, void *stream) {
+    return fit_impl(X, Y, N, n_x, ops, ch, C_init, M, B, starts, x_off, y_off, n_pts, stop_loss, C_out, loss_out,
+                    iters_out, nclos_out, err_out, flags, max_iter, history, tol_grad, line_search, ls_max_iter, eps,
+                    opts, workspace, ws_bytes, stream);
 }
 
 }  // extern "C"

@@ -40,8 +40,11 @@ struct CProg {
 };
 
 template <typename T>
-__device__ inline CProg<T> load_cprog(const int64_t *ops, const int64_t *ch, const T *C, int64_t b, int M, int n_x) {
-    Program<T> p = load_program<T>(ops, ch, C, b, M, 1, M, 1, M, n_x);
+__device__ inline CProg<T> load_cprog(const int64_t *ops, const int64_t *ch, const T *C, int64_t b, int M, int n_x,
+                                      int64_t cb = -1) {
+    // program of row b; constants of row cb (b when cb < 0)
+    Program<T> p = load_program<T>(ops + b * M, ch + b * M * 2, C ? C + (cb < 0 ? b : cb) * M : nullptr, 0, M, 1,
+                                   M, 1, M, n_x);
     CProg<T> f;
     const int lane = lane_id();
     f.cm0 = p.cmask0;

@@ -231,6 +231,25 @@ int sr_lbfgs_fit(const float *X, const float *Y, int64_t N, int64_t n_x, const i
                  int line_search, int ls_max_iter, double eps, const sr_fit_options *opts, void *workspace,
                  int64_t ws_bytes, void *stream);
 
+/* Multi-start form of sr_lbfgs_fit: `starts` (K >= 1) fits of each of the B
+ * problems in one launch, from K start points per problem.  The multi-start
+ * counterpart of the reference's multiple-constant evaluation
+ * (evaluate_multiple_constant_backend, src/symbolic_evaluation.cpp:171-305,
+ * C of shape (M, B, K)), carried through to the fit: the K fits of a problem
+ * share its program, targets, points and stop threshold and run as
+ * neighbouring work items.
+ *   C_init (B,K,M); outputs C_out (B,K,M), loss_out, iters_out, nclos_out,
+ *   err_out (B,K); X, Y, x_off, y_off, n_pts, stop_loss per problem as in
+ *   sr_lbfgs_fit; workspace sr_lbfgs_workspace_bytes(B, M) (problems, not
+ *   fits).  B * K <= 2^31 - 1.  Each fit is the one sr_lbfgs_fit computes
+ *   for that problem and start (bit for bit). */
+int sr_lbfgs_fit_multi(const float *X, const float *Y, int64_t N, int64_t n_x, const int64_t *ops,
+                       const int64_t *ch, const float *C_init, int64_t M, int64_t B, int64_t starts,
+                       const int64_t *x_off, const int64_t *y_off, const int32_t *n_pts, const double *stop_loss,
+                       float *C_out, double *loss_out, int32_t *iters_out, int32_t *nclos_out, int32_t *err_out,
+                       int32_t *flags, int max_iter, int history, double tol_grad, int line_search, int ls_max_iter,
+                       double eps, const sr_fit_options *opts, void *workspace, int64_t ws_bytes, void *stream);
+
 /* Launch-variant table of the fit kernels (constant; no counterpart in the
  * reference; read by the autotuner).  Problems are bucketed by tape rows;
  * each bucket has variants (waves per problem x points per lane per
