@@ -13,7 +13,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import c_double, c_float, c_int, c_int64, c_void_p, POINTER
+from ctypes import c_double, c_float, c_int, c_int64, c_uint64, c_void_p, POINTER
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 HIP_LIB_PATH = os.path.join(_HERE, "libsr_hip.so")
@@ -80,6 +80,10 @@ def hip() -> ctypes.CDLL:
         _declare(lib, "sr_multi_backward_mb_workspace_bytes", I64, [I64, I64, I64, I64, I])
         _declare(lib, "sr_fused_workspace_bytes", I64, [I64, I64, I64, I64, I, I])
         _declare(lib, "sr_fused_noalloc_max_bytes", I64, [])
+        _declare(lib, "sr_pcfg_sample_device", I, [P, P, P, P, I64, I64, I64, I64, P, P, c_uint64, I64, P])
+        _declare(lib, "sr_pcfg_infix_to_postfix_device", I, [P, I64, P, P, P, I64, I64, I64, I64, P])
+        _declare(lib, "sr_pcfg_get_postfix_children_device", I, [P, P, P, I64, I64, P])
+        _declare(lib, "sr_pcfg_postfix_to_infix_device", I, [P, P, P, I64, I64, I64, I64, I64, P])
         _declare(lib, "sr_probe_valu_ops_per_thread", I, [I])
         _declare(lib, "sr_probe_valu_rate", I, [I, I64, I, P, P])
         _declare(lib, "sr_fused_root_noalloc_f32", I,
@@ -121,6 +125,8 @@ def host() -> ctypes.CDLL:
         P, I64, I = c_void_p, c_int64, c_int
         _declare(lib, "sr_pcfg_sort_rules", None, [P, I64, P])
         _declare(lib, "sr_pcfg_sample", None, [P, P, P, P, P, I64, I64, I64, P, P, P, I64])
+        _declare(lib, "sr_pcfg_sample_philox", None, [P, P, P, P, I64, I64, I64, P, P, c_uint64, I64])
+        _declare(lib, "sr_philox4x32_10", None, [P, P, P])
         _declare(lib, "sr_pcfg_infix_to_postfix", None, [P, P, P, P, I64, I64, I64, I64])
         _declare(lib, "sr_pcfg_infix_to_prefix", None, [P, P, P, P, I64, I64, I64, I64])
         _declare(lib, "sr_pcfg_postfix_to_infix", None, [P, P, P, I64, I64, I64, I64, I64])

@@ -24,6 +24,8 @@
 #include <random>
 #include <vector>
 
+#include "sr_philox.h"
+
 namespace {
 
 constexpr int kMaxLen = 128;  // hard cap on stacks and queues (HARD_MAX_LENGTH)
@@ -68,11 +70,12 @@ inline bool functional(int64_t op) {
 }
 
 // ------------------------------------------------------------------ sample
-int64_t sample_row(const int64_t *rhs_ptr, const int64_t *rhs_concat, const int64_t *nt_rule_ptr,
-                   const float *cum, int64_t start_id, int64_t max_length, int64_t max_tries, int64_t *out,
-                   int64_t seed) {
-    std::mt19937 gen(seed);
-    std::uniform_real_distribution<float> unif(0.0f, 1.0f);
+// Leftmost derivation with an explicit symbol stack; `unif` gives the
+// uniform number of each rule choice.
+template <class Uniform>
+int64_t sample_row_with(const int64_t *rhs_ptr, const int64_t *rhs_concat, const int64_t *nt_rule_ptr,
+                        const float *cum, int64_t start_id, int64_t max_length, int64_t max_tries, int64_t *out,
+                        Uniform &unif) {
     for (int64_t attempt = 0; attempt < max_tries; ++attempt) {
         int32_t stack[kMaxLen];
         int32_t sp = 0;
@@ -87,7 +90,7 @@ int64_t sample_row(const int64_t *rhs_ptr, const int64_t *rhs_concat, const int6
                 out[len++] = sym;
                 continue;
             }
-            const float u = unif(gen);
+            const float u = unif();
             int32_t pick = -1;
             for (int32_t r = r0; r < r1; ++r)
                 if (u <= cum[r]) { pick = r; break; }
@@ -105,6 +108,17 @@ int64_t sample_row(const int64_t *rhs_ptr, const int64_t *rhs_concat, const int6
         }
     }
     return E_GEN_MAX_TRIES;
+}
+
+// The reference's stream: std::mt19937 seeded per row with
+// uniform_real_distribution<float>.
+int64_t sample_row(const int64_t *rhs_ptr, const int64_t *rhs_concat, const int64_t *nt_rule_ptr,
+                   const float *cum, int64_t start_id, int64_t max_length, int64_t max_tries, int64_t *out,
+                   int64_t seed) {
+    std::mt19937 gen(seed);
+    std::uniform_real_distribution<float> dist(0.0f, 1.0f);
+    auto unif = [&]() { return dist(gen); };
+    return sample_row_with(rhs_ptr, rhs_concat, nt_rule_ptr, cum, start_id, max_length, max_tries, out, unif);
 }
 
 // ------------------------------------------------- infix -> postfix / prefix
@@ -497,6 +511,21 @@ void sr_pcfg_sample(const int64_t *rule_lhs, const int64_t *rhs_ptr, const int64
         errors[b] = sample_row(rhs_ptr, rhs_concat, nt_rule_ptr, cum_probs, start_id, max_length, max_tries,
                                out + b * max_length, seeds[b]);
 }
+
+// The device sampler's stream (Philox-4x32-10, row b = subsequence b of
+// `seed`), on the host: the same rows sr_pcfg_sample_device draws.
+void sr_pcfg_sample_philox(const int64_t *rhs_ptr, const int64_t *rhs_concat, const int64_t *nt_rule_ptr,
+                           const float *cum_probs, int64_t start_id, int64_t max_length, int64_t max_tries,
+                           int64_t *out, int64_t *errors, uint64_t seed, int64_t B) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t b = 0; b < B; ++b) {
+        sr::PhiloxUniform unif(seed, (uint64_t)b);
+        errors[b] = sample_row_with(rhs_ptr, rhs_concat, nt_rule_ptr, cum_probs, start_id, max_length, max_tries,
+                                    out + b * max_length, unif);
+    }
+}
+
+void sr_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out) { sr::philox4x32_10(ctr, key, out); }
 
 void sr_pcfg_infix_to_postfix(const int64_t *prec, const int64_t *in, int64_t *out, int64_t *errors, int64_t lp,
                               int64_t rp, int64_t B, int64_t M) {

@@ -4,9 +4,16 @@ Mirrors the reference's ProbabilisticContextFreeGrammar class
 (src/pcfg.cpp:44-1061, include/pcfg.h:26-120): the same symbol table and
 ids, grammar-text parser and its error texts, CSR rule tables, and the same
 methods, plus `sample(B) -> (Ops, Ch)` from the README contract
-(README.md:64-69).  The per-row algorithms run in the host engine
-libsr_host.so (one OpenMP iteration per row); results are copied to the
-grammar's device.
+(README.md:64-69).  As the reference dispatches on the device
+(src/pcfg.cpp:572-986), rows on a GPU go through the device kernels of
+libsr_hip.so (sampling on a grammar whose device is a GPU, infix ->
+postfix, the children table and postfix -> infix; one lane per row,
+include/sr_pcfg_device.h), and the rest through the host engine
+libsr_host.so (one OpenMP iteration per row), whose results are copied to
+the input's device.  Both give the same tokens and error codes; sampling
+on a GPU draws its own stream (Philox-4x32-10), as the reference's CUDA
+sampler does (curand there), and `sample_infix(B, engine="host")` keeps
+the reference's CPU stream on any device.
 """
 from __future__ import annotations
 
@@ -110,6 +117,7 @@ class ProbabilisticContextFreeGrammar:
             )
         self._build_symbol_table()
         self._parse_grammar(grammar)
+        self._device_tables: Dict[str, Dict[str, torch.Tensor]] = {}
 
     # ------------------------------------------------------------ symbols
     def _add_symbol(self, s: str, i: int) -> None:
@@ -289,6 +297,21 @@ class ProbabilisticContextFreeGrammar:
         return [s for s, i in self._symbol_to_id.items() if i < self._n_operators]
 
     # ------------------------------------------------------- helpers
+    def _tables_on(self, dev: torch.device) -> Dict[str, torch.Tensor]:
+        """The CSR grammar and precedence table on a GPU (copied once)."""
+        key = str(dev)
+        t = self._device_tables.get(key)
+        if t is None:
+            t = dict(rhs_ptr=self._rhs_ptr.to(dev), rhs_concat=self._rhs_concat.to(dev),
+                     nt_rule_ptr=self._nt_rule_ptr.to(dev), cum=self._cum_probs.to(dev),
+                     prec=self._precedence.to(dev))
+            self._device_tables[key] = t
+        return t
+
+    @staticmethod
+    def _on_gpu(t: torch.Tensor) -> bool:
+        return t.device.type == "cuda"
+
     def _host_i64(self, t: torch.Tensor) -> torch.Tensor:
         return t.detach().to("cpu", torch.int64).contiguous()
 
@@ -328,11 +351,44 @@ class ProbabilisticContextFreeGrammar:
         raise RuntimeError("".join(lines))
 
     # ------------------------------------------------------- sampling
-    def sample_infix(self, B: int) -> torch.Tensor:
-        """B expressions in infix token form, NO_OP padded to padded_maximum_length."""
+    def _sample_errors(self, errs: List[int]) -> None:
+        if any(errs):
+            counts: Dict[int, int] = {}
+            for e in errs:
+                if e:
+                    counts[e] = counts.get(e, 0) + 1
+            msg = "Failed to generate expressions for some batch items. Error counts:\n"
+            for code in sorted(counts):
+                msg += f"  {get_error_message(code)}: {counts[code]} occurrences.\n"
+            raise RuntimeError(msg)
+
+    def sample_infix(self, B: int, engine: str = "auto") -> torch.Tensor:
+        """B expressions in infix token form, NO_OP padded to padded_maximum_length.
+
+        engine "auto": on the device kernels when the grammar's device is a
+        GPU (Philox-4x32-10 rows from one seed drawn from torch's generator
+        of that device), else the host engine; "host": the host engine
+        (std::mt19937 per row, seeds from torch's generator, the
+        reference's CPU stream) on any device."""
         B = int(B)
         L = self._padded_maximum_length
         start_id = self._token_id(self._start_symbol)
+        if engine not in ("auto", "host"):
+            raise ValueError(f"engine must be 'auto' or 'host', got {engine!r}")
+        if engine == "auto" and self._device.type == "cuda":
+            if start_id >= self._nt_rule_ptr.numel() - 1:
+                raise RuntimeError("Start symbol has no rules: " + self._start_symbol)
+            seed = int(torch.randint(0, INT64_MAX, (1,), dtype=torch.int64, device=self._device).item())
+            t = self._tables_on(self._device)
+            out = torch.empty((B, L), dtype=torch.int64, device=self._device)
+            errors = torch.empty(B, dtype=torch.int64, device=self._device)
+            _lib.check_status(_lib.hip().sr_pcfg_sample_device(
+                _lib.ptr(t["rhs_ptr"]), _lib.ptr(t["rhs_concat"]), _lib.ptr(t["nt_rule_ptr"]), _lib.ptr(t["cum"]),
+                self._nt_rule_ptr.numel() - 1, start_id, L, self._max_tries, _lib.ptr(out), _lib.ptr(errors),
+                seed, B, _lib.stream_of(self._device)), "sr_pcfg_sample_device")
+            if B and bool(errors.any()):
+                self._sample_errors(errors.tolist())
+            return out
         seeds = torch.randint(0, INT64_MAX, (B,), dtype=torch.int64, device=self._device).to("cpu")
         out = torch.zeros((B, L), dtype=torch.int64)
         errors = torch.zeros(B, dtype=torch.int64)
@@ -343,16 +399,7 @@ class ProbabilisticContextFreeGrammar:
             _lib.ptr(self._nt_rule_ptr), _lib.ptr(self._cum_probs), start_id, L, self._max_tries,
             _lib.ptr(out), _lib.ptr(errors), _lib.ptr(seeds), B,
         )
-        errs = errors.tolist()
-        if any(errs):
-            counts: Dict[int, int] = {}
-            for e in errs:
-                if e:
-                    counts[e] = counts.get(e, 0) + 1
-            msg = "Failed to generate expressions for some batch items. Error counts:\n"
-            for code in sorted(counts):
-                msg += f"  {get_error_message(code)}: {counts[code]} occurrences.\n"
-            raise RuntimeError(msg)
+        self._sample_errors(errors.tolist())
         return out.to(self._device)
 
     def sample(self, B: int) -> Tuple[torch.Tensor, torch.Tensor]:
@@ -390,6 +437,24 @@ class ProbabilisticContextFreeGrammar:
         return out.to(expressions.device)
 
     def infix_to_postfix(self, expressions: torch.Tensor, verbosity: int = 0) -> torch.Tensor:
+        if self._on_gpu(expressions):
+            e = expressions.detach().to(torch.int64).contiguous()
+            self._check_range(e, self._terminal_limit, "infix_ops", "terminal symbols (less than terminal_limit)")
+            B, M = e.shape
+            if M > HARD_MAX_LENGTH:
+                raise ValueError(
+                    f"M must be less than or equal to HARD_MAX_LENGTH, but got {M} > {HARD_MAX_LENGTH}."
+                )
+            prec = self._tables_on(e.device)["prec"]
+            out = torch.empty((B, M), dtype=torch.int64, device=e.device)
+            errors = torch.empty(B, dtype=torch.int64, device=e.device)
+            if B:
+                _lib.check_status(_lib.hip().sr_pcfg_infix_to_postfix_device(
+                    _lib.ptr(prec), prec.numel(), _lib.ptr(e), _lib.ptr(out), _lib.ptr(errors), self._lparen,
+                    self._rparen, B, M, _lib.stream_of(e.device)), "sr_pcfg_infix_to_postfix_device")
+                if bool(errors.any()):
+                    self._raise_errors(errors.cpu(), e.cpu(), "postfix parsing", verbosity)
+            return out
         return self._parse_infix(expressions, "sr_pcfg_infix_to_postfix", "postfix parsing", verbosity)
 
     def infix_to_prefix(self, expressions: torch.Tensor, verbosity: int = 0) -> torch.Tensor:
@@ -402,10 +467,27 @@ class ProbabilisticContextFreeGrammar:
                 f"Input tensor must be on the same device as the PCFG, but got input on {expressions.device} "
                 f"and PCFG on {self._device}."
             )
+        Mi = int(max_infix_len)
+        if self._on_gpu(expressions) and fn == "sr_pcfg_postfix_to_infix":
+            e = expressions.detach().to(torch.int64).contiguous()
+            self._check_range(e, self._n_operators, "prefix_ops", "valid operator IDs")
+            B, M = e.shape
+            out = torch.empty((B, Mi), dtype=torch.int64, device=e.device)
+            errors = torch.empty(B, dtype=torch.int64, device=e.device)
+            if B and M and Mi > 0:
+                if M > HARD_MAX_LENGTH:
+                    raise ValueError(
+                        f"M must be less than or equal to HARD_MAX_LENGTH, but got {M} > {HARD_MAX_LENGTH}."
+                    )
+                _lib.check_status(_lib.hip().sr_pcfg_postfix_to_infix_device(
+                    _lib.ptr(e), _lib.ptr(out), _lib.ptr(errors), self._lparen, self._rparen, B, M, Mi,
+                    _lib.stream_of(e.device)), "sr_pcfg_postfix_to_infix_device")
+                if bool(errors.any()):
+                    self._raise_errors(errors.cpu(), e.cpu(), context, verbosity)
+                return out
         e = self._host_i64(expressions)
         self._check_range(e, self._n_operators, "prefix_ops", "valid operator IDs")
         B, M = e.shape
-        Mi = int(max_infix_len)
         out = torch.zeros((B, Mi), dtype=torch.int64)
         errors = torch.zeros(B, dtype=torch.int64)
         getattr(_lib.host(), fn)(_lib.ptr(e), _lib.ptr(out), _lib.ptr(errors), self._lparen, self._rparen, B, M, Mi)
@@ -449,6 +531,19 @@ class ProbabilisticContextFreeGrammar:
         return out.to(prefix_ops.device)
 
     def get_postfix_children(self, postfix_ops: torch.Tensor, verbosity: int = 0) -> torch.Tensor:
+        if self._on_gpu(postfix_ops) and postfix_ops.dim() == 2 and 0 < postfix_ops.shape[1] <= HARD_MAX_LENGTH:
+            e = postfix_ops.detach().to(torch.int64).contiguous()
+            self._check_range(e, self._n_operators, "postfix_ops", "valid operator IDs")
+            B, M = e.shape
+            out = torch.empty((B, M, 2), dtype=torch.int64, device=e.device)
+            errors = torch.empty(B, dtype=torch.int64, device=e.device)
+            if B:
+                _lib.check_status(_lib.hip().sr_pcfg_get_postfix_children_device(
+                    _lib.ptr(e), _lib.ptr(out), _lib.ptr(errors), B, M, _lib.stream_of(e.device)),
+                    "sr_pcfg_get_postfix_children_device")
+                if bool(errors.any()):
+                    self._raise_errors(errors.cpu(), e.cpu(), "get postfix children", verbosity)
+            return out
         e = self._host_i64(postfix_ops)
         self._check_range(e, self._n_operators, "postfix_ops", "valid operator IDs")
         B, M = e.shape

@@ -36,6 +36,18 @@ void sr_pcfg_sample(const int64_t *rule_lhs, const int64_t *rhs_ptr, const int64
                     const int64_t *nt_rule_ptr, const float *cum_probs, int64_t start_id, int64_t max_length,
                     int64_t max_tries, int64_t *out, int64_t *errors, const int64_t *seeds, int64_t B);
 
+/* The device sampler's rows on the host (no counterpart in the reference):
+ * sample_infix's derivation with row b drawn from the Philox-4x32-10 stream
+ * (key = seed, counter = (block, b), words (w >> 8) * 2^-24), the rows
+ * sr_pcfg_sample_device (sr_pcfg_device.h) writes for the same seed. */
+void sr_pcfg_sample_philox(const int64_t *rhs_ptr, const int64_t *rhs_concat, const int64_t *nt_rule_ptr,
+                           const float *cum_probs, int64_t start_id, int64_t max_length, int64_t max_tries,
+                           int64_t *out, int64_t *errors, uint64_t seed, int64_t B);
+
+/* One Philox-4x32-10 block: out = philox(ctr[4], key[2]) (Salmon et al.,
+ * SC'11; no counterpart in the reference). */
+void sr_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
+
 /* Shunting-yard infix -> postfix / prefix with the arity validation pass
  * (src/pcfg.cpp:660-755, src/pcfg_cpu.cpp:121-447). */
 void sr_pcfg_infix_to_postfix(const int64_t *prec, const int64_t *in, int64_t *out, int64_t *errors, int64_t lp,

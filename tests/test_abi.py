@@ -8,7 +8,8 @@ import subprocess
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = {"sr_eval.h": "libsr_hip.so", "sr_mlp.h": "libsr_hip.so", "sr_pcfg.h": "libsr_host.so"}
+HEADERS = {"sr_eval.h": "libsr_hip.so", "sr_mlp.h": "libsr_hip.so", "sr_pcfg.h": "libsr_host.so",
+           "sr_pcfg_device.h": "libsr_hip.so"}
 
 
 def _declared(header):

@@ -230,31 +230,66 @@ def _py_sample(rules, start, limit, rnd):
             return out
 
 
-def test_sample_distribution_matches_a_python_sampler():
-    """tests/test_sampling.py:233-287 on the host sampler: 100k samples, the
-    target expressions' frequencies within 20% (relative) of a plain Python
-    sampler of the same grammar."""
-    targets = ["X_0", "X_0 * X_0", "sin ( X_0 )", "cos ( X_0 )", "exp ( X_0 )", "( X_0 ) ^2", "X_0 + X_0",
-               "X_0 - X_0", "X_0 / X_0"]
-    n = 100000
+TARGETS = ["X_0", "X_0 * X_0", "sin ( X_0 )", "cos ( X_0 )", "exp ( X_0 )", "( X_0 ) ^2", "X_0 + X_0",
+           "X_0 - X_0", "X_0 / X_0"]
+N_DIST = 100000
+
+
+@pytest.fixture(scope="module")
+def python_counts():
     rules = _parse(TEST_GRAMMAR)
     rnd = random.Random(0)
     py = defaultdict(int)
-    for _ in range(n):
+    for _ in range(N_DIST):
         s = " ".join(_py_sample(rules, "E", 128, rnd))
-        if s in targets:
+        if s in TARGETS:
             py[s] += 1
+    return py
+
+
+def check_distribution(strings, py, label):
+    """The target expressions' frequencies within 20% (relative) of the
+    Python sampler's (tests/test_sampling.py:233-287)."""
+    got = defaultdict(int)
+    for s in strings:
+        if s in TARGETS:
+            got[s] += 1
+    for t in TARGETS:
+        a, b = py[t] / N_DIST, got[t] / len(strings)
+        print(f"{t}: python {a:.5f} {label} {b:.5f}")
+        assert abs(a - b) / max(a, b) < 0.2, (t, a, b)
+
+
+def test_sample_distribution_matches_a_python_sampler(python_counts):
+    """tests/test_sampling.py:233-287 on the host sampler (the reference's
+    std::mt19937 stream): 100k samples."""
     torch.manual_seed(0)
     g = ProbabilisticContextFreeGrammar(TEST_GRAMMAR, "E", 128, 1, CPU)
-    strings = g.to_string(g.sample_infix(n))
-    cpp = defaultdict(int)
-    for s in strings:
-        if s in targets:
-            cpp[s] += 1
-    for t in targets:
-        a, b = py[t] / n, cpp[t] / n
-        print(f"{t}: python {a:.5f} host {b:.5f}")
-        assert abs(a - b) / max(a, b) < 0.2, (t, a, b)
+    check_distribution(g.to_string(g.sample_infix(N_DIST)), python_counts, "host")
+
+
+def sample_philox_host(g, B, seed):
+    """The device sampler's rows drawn by the host engine
+    (sr_pcfg_sample_philox)."""
+    from cudasymbolicregression_amd import _lib
+
+    L = g.padded_maximum_length
+    out = torch.zeros((B, L), dtype=torch.int64)
+    errors = torch.zeros(B, dtype=torch.int64)
+    _lib.host().sr_pcfg_sample_philox(
+        _lib.ptr(g._rhs_ptr), _lib.ptr(g._rhs_concat), _lib.ptr(g._nt_rule_ptr), _lib.ptr(g._cum_probs),
+        g.get_symbol_id(g._start_symbol), L, g._max_tries, _lib.ptr(out), _lib.ptr(errors), seed, B)
+    return out, errors
+
+
+def test_philox_sample_distribution_matches_a_python_sampler(python_counts):
+    """The same check on the Philox-4x32-10 stream of the device sampler
+    (drawn here by the host engine; the device kernel draws these rows bit
+    for bit, tests/test_pcfg_device_gpu.py)."""
+    g = ProbabilisticContextFreeGrammar(TEST_GRAMMAR, "E", 128, 1, CPU)
+    out, errors = sample_philox_host(g, N_DIST, 0x5EED)
+    assert int(errors.abs().sum()) == 0
+    check_distribution(g.to_string(out), python_counts, "philox")
 
 
 # ------------------------------------------------------------ trivial grammars
