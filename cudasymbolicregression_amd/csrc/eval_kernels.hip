@@ -690,9 +690,7 @@ __global__ void k_reduce_partials(const T *__restrict__ part, int64_t P, int64_t
                                   int accumulate) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= len) return;
-    double s = accumulate ? (double)out[i] : 0.0;
-    for (int64_t r = 0; r < P; ++r) s += (double)part[r * len + i];
-    out[i] = (T)s;
+    out[i] = (T)sum_rows_in_order(part + i, 0, P, len, accumulate ? (double)out[i] : 0.0);
 }
 
 // The same sum in two stages (the single-stage kernel gives each column one
@@ -705,18 +703,14 @@ __global__ void k_reduce_partials_1(const T *__restrict__ part, int64_t P, int64
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= len) return;
     const int64_t r0 = (int64_t)blockIdx.y * RED_ROWS, r1 = r0 + RED_ROWS < P ? r0 + RED_ROWS : P;
-    double s = 0.0;
-    for (int64_t r = r0; r < r1; ++r) s += (double)part[r * len + i];
-    part64[(int64_t)blockIdx.y * len + i] = s;
+    part64[(int64_t)blockIdx.y * len + i] = sum_rows_in_order(part + i, r0, r1, len, 0.0);
 }
 template <typename T>
 __global__ void k_reduce_partials_2(const double *__restrict__ part64, int64_t chunks, int64_t len,
                                     T *__restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= len) return;
-    double s = 0.0;
-    for (int64_t c = 0; c < chunks; ++c) s += part64[c * len + i];
-    out[i] = (T)s;
+    out[i] = (T)sum_rows_in_order(part64 + i, 0, chunks, len, 0.0);
 }
 
 inline int ret_last() { return (int)hipGetLastError(); }

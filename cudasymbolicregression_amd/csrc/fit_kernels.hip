@@ -236,30 +236,15 @@ __device__ inline void closure_code(const Code &c, const Plan &pl, int nc, const
             const uint32_t f = pl.fw<LT>(i), sw = pl.sw<LT>(i);
             const int op = pf_op(f);
             const bool binary = pf_bin(f);
-            float g[PPT], a0[PPT], a1[PPT], go0[PPT], go1[PPT];
-            bool add0[PPT], add1[PPT];
-            int32_t ee[PPT];
+            float g[PPT], a0[PPT], a1[PPT], ge0[PPT], ge1[PPT];
             const int grow = ps_grow(sw);
 #pragma unroll
             for (int p = 0; p < PPT; ++p) g[p] = t.get(grow, p);
             fetch(f, i, a0, a1);
-            if (op == OP_COPY) {
-#pragma unroll
-                for (int p = 0; p < PPT; ++p) {
-                    go0[p] = g[p];
-                    add0[p] = true;
-                    go1[p] = 0.0f;
-                    add1[p] = false;
-                    ee[p] = 0;
-                }
-            } else {
-                bwd_ops<float, PPT>(op, g, a0, a1, go0, add0, go1, add1, ee);
-            }
-#pragma unroll
-            for (int p = 0; p < PPT; ++p)
-                if (act[p] && ee[p] > e) e = ee[p];
-            // scatter to the operands
-            auto scatter = [&](int k, const float (&go)[PPT], const bool (&add)[PPT]) {
+            bwd_masked<float, PPT>(op, g, a0, a1, act, ge0, ge1, e);
+            // scatter to the operands; ge is the amount added to an operand's
+            // adjoint (0 where the reference adds nothing or past the end)
+            auto scatter = [&](int k, const float (&ge)[PPT]) {
                 const uint32_t mode = ps_mode(sw, k);
                 const int tg = ps_tgt(sw, k);
                 if (mode - 1u < 3u) {
@@ -273,22 +258,22 @@ __device__ inline void closure_code(const Code &c, const Plan &pl, int nc, const
 #pragma unroll
                         for (int p = 0; p < PPT; ++p) o[p] = t.get(tg, p);
 #pragma unroll
-                    for (int p = 0; p < PPT; ++p) t.put(tg, p, o[p] + ((act[p] && add[p]) ? go[p] : 0.0f));
+                    for (int p = 0; p < PPT; ++p) t.put(tg, p, o[p] + ge[p]);
                 } else if (mode == PM_CONST) {  // a constant used several times: wave sum per use
                     double gv = 0.0;
 #pragma unroll
-                    for (int p = 0; p < PPT; ++p) gv += (act[p] && add[p]) ? (double)go[p] : 0.0;
+                    for (int p = 0; p < PPT; ++p) gv += (double)ge[p];
                     const double s = wave_sum_rows(gv, 4);
                     if (tg < WAVE) { if (lane == tg) gc[0] += s; }
                     else if (lane == tg - WAVE) gc[SL - 1] += s;
                 } else if (mode == PM_NOOP) {
 #pragma unroll
                     for (int p = 0; p < PPT; ++p)
-                        if (act[p] && add[p] && go[p] != 0.0f) e = e > ERR_GRAD_NO_OP ? e : ERR_GRAD_NO_OP;
+                        if (ge[p] != 0.0f) e = e > ERR_GRAD_NO_OP ? e : ERR_GRAD_NO_OP;
                 }
             };
-            scatter(0, go0, add0);
-            if (binary) scatter(1, go1, add1);
+            scatter(0, ge0);
+            if (binary) scatter(1, ge1);
         }
         // ---- adjoints of the constants used once, from their rows, in FP64
 #pragma unroll

@@ -41,7 +41,16 @@ using namespace sr;
 namespace {
 
 constexpr int F_NCREG = 8;  // constants accumulated in per-lane FP64 registers
-constexpr int F_NX = 4;     // variables held in registers per point
+constexpr int F_NX = 4;     // variables held in tape rows per point (NXR_ROWS)
+static_assert(F_NX == NXR_ROWS, "the variables of a point are the tape's variable rows");
+
+// With want_gx, the adjoints of the variable rows accumulate in NXR_ROWS more
+// tape rows, LT + NXR_ROWS + v, when their numbers fit the 8-bit scatter
+// target (every bucket but the longest); the tape grows by those rows.
+__host__ __device__ constexpr bool f_var_rows(int LT) { return LT + 2 * NXR_ROWS <= 256; }
+__host__ __device__ inline size_t f_tape_elems(int LT, int PPT) {
+    return (size_t)(LT + (f_var_rows(LT) ? 2 : 1) * NXR_ROWS) * PPT * WAVE;
+}
 constexpr int F_NBUCKETS = 5;
 // Wave-private grad_X rows of a block are kept in LDS only when they are
 // small: larger rows would cut the waves per CU (LDS rows of 12 KB per wave
@@ -105,18 +114,20 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
     const size_t gx_bytes = gx_lds ? (((size_t)WPB * gx_len * sizeof(T) + 15) / 16 * 16) : 0;
     Tape<T, PPT> t;
     {
-        T *slot = MEM ? (T *)a.gtape + ((size_t)blockIdx.x * WPB + wave) * tape_elems(LT, PPT)
-                      : reinterpret_cast<T *>(smem + gx_bytes) + (size_t)wave * tape_elems(LT, PPT);
+        T *slot = MEM ? (T *)a.gtape + ((size_t)blockIdx.x * WPB + wave) * f_tape_elems(LT, PPT)
+                      : reinterpret_cast<T *>(smem + gx_bytes) + (size_t)wave * f_tape_elems(LT, PPT);
         t.v = slot;
     }
-    // this wave's grad_X partial row; each element is only ever touched by
-    // the lane that owns its point (n = lane mod 64), zeroing included
+    // this wave's grad_X partial row, variable-major (element (n, v) at
+    // v * N + n, so a wave's 64 points are 64 consecutive words); each element
+    // is only ever touched by the lane that owns its point (n = lane mod 64),
+    // zeroing included
     T *gxrow = nullptr;
     if (want_gx) {
         gxrow = gx_lds ? reinterpret_cast<T *>(smem) + (size_t)wave * gx_len
                       : (T *)a.gx_partial + (a.gx_row0 + (int64_t)blockIdx.x * WPB + wave) * gx_len;
-        for (int64_t nz = lane; nz < N; nz += WAVE)
-            for (int v = 0; v < n_x; ++v) gxrow[nz * n_x + v] = T(0);
+        for (int v = 0; v < n_x; ++v)
+            for (int64_t nz = lane; nz < N; nz += WAVE) gxrow[(int64_t)v * N + nz] = T(0);
     }
     const int64_t count = (int64_t)*a.list_count;
     const T *const X0 = X;
@@ -152,7 +163,22 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
         }
         const Code c = compile_code(p, a.M);
         constexpr int NVR = GX ? 0 : NXR_ROWS;  // variables held in tape rows LT + v
-        const Plan pl = compile_plan(p, a.M, c, LT, NVR);
+        // their adjoints in rows LT + NXR_ROWS + v (VROWS), or in gxr registers
+        constexpr bool VROWS = !GX && f_var_rows(LT);
+        const int var_adj_row0 = (VROWS && want_gx) ? LT + NXR_ROWS : -1;
+        // the variables the program reads (wave-uniform): only their grad_X
+        // rows are touched
+        int vmask = 0;
+        if (!GX && want_gx) {
+#pragma unroll
+            for (int v = 0; v < NXR_ROWS; ++v) {
+                const int vop = VAR_START_ID + v;
+                const bool u0 = lane < a.M && node_op(p.w0) == vop;
+                const bool u1 = lane + WAVE < a.M && node_op(p.w1) == vop;
+                vmask |= (v < n_x && (__ballot(u0) | __ballot(u1))) ? 1 << v : 0;
+            }
+        }
+        const Plan pl = compile_plan(p, a.M, c, LT, NVR, var_adj_row0);
         const int NI = c.NI;
         if (NI + pl.nextra > LT) {  // the bucketing rule and the compiler disagree: never expected
             if (lane == 0) {
@@ -220,13 +246,24 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
         for (int64_t base = 0; base < Nb; base += (int64_t)WAVE * PPT, take_next()) {
             bool act[PPT];
             int64_t n[PPT];
-            T gxr[PPT][F_NX];
+            T gxr[PPT][VROWS ? 1 : F_NX];
 #pragma unroll
             for (int q = 0; q < PPT; ++q) {
                 n[q] = base + q * WAVE + lane;
                 act[q] = n[q] < Nb;
 #pragma unroll
-                for (int v = 0; v < F_NX; ++v) gxr[q][v] = T(0);
+                for (int v = 0; v < (VROWS ? 1 : F_NX); ++v) gxr[q][v] = T(0);
+            }
+            // this group's grad_X partial sums, read now and added to after
+            // the reverse sweep, so the read's latency hides behind the
+            // group's evaluation (the row is this wave's alone)
+            T gxp[PPT][F_NX];
+            if (!GX && want_gx) {
+#pragma unroll
+                for (int q = 0; q < PPT; ++q)
+#pragma unroll
+                    for (int v = 0; v < F_NX; ++v)
+                        gxp[q][v] = (((vmask >> v) & 1) && act[q]) ? gxrow[(int64_t)v * N + n[q]] : T(0);
             }
             load_group(base + (int64_t)WAVE * PPT, xn, dn);
             // variable rows, and zeroed extra adjoint rows
@@ -238,6 +275,12 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
             for (int r = NI; r < NI + pl.nextra; ++r)
 #pragma unroll
                 for (int q = 0; q < PPT; ++q) t.put(r, q, T(0));
+            if (var_adj_row0 >= 0)
+#pragma unroll
+                for (int v = 0; v < NVR; ++v)
+                    if ((vmask >> v) & 1)
+#pragma unroll
+                        for (int q = 0; q < PPT; ++q) t.put(var_adj_row0 + v, q, T(0));
             // operands of instruction ii: tape rows or scalars, both rows read
             // before either is used (a unary instruction reads row 0 and
             // discards it); with GX, variables read from X
@@ -312,29 +355,15 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
                 const uint32_t f = pl.fw<LT>(ii), sw = pl.sw<LT>(ii);
                 const int op = pf_op(f);
                 const bool binary = pf_bin(f);
-                T g[PPT], a0[PPT], a1[PPT], go0[PPT], go1[PPT];
-                bool add0[PPT], add1[PPT];
-                int32_t ee[PPT];
+                T g[PPT], a0[PPT], a1[PPT], ge0[PPT], ge1[PPT];
                 const int grow = ps_grow(sw);
 #pragma unroll
                 for (int q = 0; q < PPT; ++q) g[q] = t.get(grow, q);
                 fetch(f, ii, a0, a1);
-                if (op == OP_COPY) {
-#pragma unroll
-                    for (int q = 0; q < PPT; ++q) {
-                        go0[q] = g[q];
-                        add0[q] = true;
-                        go1[q] = T(0);
-                        add1[q] = false;
-                        ee[q] = 0;
-                    }
-                } else {
-                    bwd_ops<T, PPT>(op, g, a0, a1, go0, add0, go1, add1, ee);
-                }
-#pragma unroll
-                for (int q = 0; q < PPT; ++q)
-                    if (act[q] && ee[q] > e) e = ee[q];
-                auto scatter = [&](int k, const T (&go)[PPT], const bool (&add)[PPT]) {
+                bwd_masked<T, PPT>(op, g, a0, a1, act, ge0, ge1, e);
+                // ge: the amount added to the operand's adjoint (0 where the
+                // reference adds nothing or the point is past the end)
+                auto scatter = [&](int k, const T (&ge)[PPT]) {
                     const uint32_t mode = ps_mode(sw, k);
                     const int tg = ps_tgt(sw, k);
                     if (mode - 1u < 3u) {
@@ -349,40 +378,40 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
 #pragma unroll
                             for (int q = 0; q < PPT; ++q) o[q] = t.get(tg, q);
 #pragma unroll
-                        for (int q = 0; q < PPT; ++q) t.put(tg, q, o[q] + ((act[q] && add[q]) ? go[q] : T(0)));
+                        for (int q = 0; q < PPT; ++q) t.put(tg, q, o[q] + ge[q]);
                     } else if (mode == PM_CONST) {  // a constant used several times: wave sum per use
                         double gv = 0.0;
 #pragma unroll
-                        for (int q = 0; q < PPT; ++q) gv += (act[q] && add[q]) ? (double)go[q] : 0.0;
+                        for (int q = 0; q < PPT; ++q) gv += (double)ge[q];
                         const double s = wave_sum_rows(gv, 4);
                         if (tg < WAVE) { if (lane == tg) gc[0] += s; }
                         else if (lane == tg - WAVE) gc[SL - 1] += s;
                     } else if (mode == PM_VAR) {
                         if (want_gx) {
                             if constexpr (!GX) {
+                                if constexpr (!VROWS) {
 #pragma unroll
-                                for (int q = 0; q < PPT; ++q) {
-                                    const T gvx = (act[q] && add[q]) ? go[q] : T(0);
+                                    for (int q = 0; q < PPT; ++q)
 #pragma unroll
-                                    for (int v = 0; v < F_NX; ++v)
-                                        if (v == tg) gxr[q][v] += gvx;
+                                        for (int v = 0; v < F_NX; ++v)
+                                            if (v == tg) gxr[q][v] += ge[q];
                                 }
                             } else {
                                 const uint32_t w = c.word_lt<LT>(ii);
                                 const int v = desc_idx(k == 0 ? instr_d0(w) : instr_d1(w));
 #pragma unroll
                                 for (int q = 0; q < PPT; ++q)
-                                    if (act[q] && add[q] && go[q] != T(0)) gxrow[n[q] * n_x + v] += go[q];
+                                    if (ge[q] != T(0)) gxrow[(int64_t)v * N + n[q]] += ge[q];
                             }
                         }
                     } else if (mode == PM_NOOP) {
 #pragma unroll
                         for (int q = 0; q < PPT; ++q)
-                            if (act[q] && add[q] && go[q] != T(0)) e = e > ERR_GRAD_NO_OP ? e : ERR_GRAD_NO_OP;
+                            if (ge[q] != T(0)) e = e > ERR_GRAD_NO_OP ? e : ERR_GRAD_NO_OP;
                     }
                 };
-                scatter(0, go0, add0);
-                if (binary) scatter(1, go1, add1);
+                scatter(0, ge0);
+                if (binary) scatter(1, ge1);
             }
             // ---- adjoints of the constants used once, from their rows, in FP64
 #pragma unroll
@@ -412,12 +441,17 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
 #pragma unroll
                 for (int q = 0; q < PPT; ++q) {
                     if (!act[q]) continue;
-                    for (int v = 0; v < n_x; ++v) {
-                        T gxv = T(0);
 #pragma unroll
-                        for (int u = 0; u < F_NX; ++u) gxv = (u == v) ? gxr[q][u] : gxv;
-                        if (gxv == T(0)) continue;
-                        gxrow[n[q] * n_x + v] += gxv;
+                    for (int v = 0; v < F_NX; ++v) {
+                        if (!((vmask >> v) & 1)) continue;
+                        T gxv = T(0);
+                        if constexpr (VROWS) {
+                            gxv = t.get(LT + NXR_ROWS + v, q);
+                        } else {
+                            gxv = gxr[q][v];
+                        }
+                        // a zero contribution is not added (the sum keeps a -0)
+                        gxrow[(int64_t)v * N + n[q]] = gxv == T(0) ? gxp[q][v] : gxp[q][v] + gxv;
                     }
                 }
             }
@@ -507,17 +541,17 @@ __global__ void k_sum_rows_1(const T *__restrict__ part, int64_t P, int64_t len,
     if (i >= len) return;
     const int64_t r0 = (int64_t)blockIdx.y * ROW_CHUNK;
     const int64_t r1 = r0 + ROW_CHUNK < P ? r0 + ROW_CHUNK : P;
-    double s = 0.0;
-    for (int64_t r = r0; r < r1; ++r) s += (double)part[r * len + i];
-    part64[(int64_t)blockIdx.y * len + i] = s;
+    part64[(int64_t)blockIdx.y * len + i] = sum_rows_in_order(part + i, r0, r1, len, 0.0);
 }
+// The rows are variable-major (v * N + n); grad_X is (N, n_x).
 template <typename T>
-__global__ void k_sum_rows_2(const double *__restrict__ part64, int64_t chunks, int64_t len, T *__restrict__ out) {
+__global__ void k_sum_rows_2(const double *__restrict__ part64, int64_t chunks, int64_t len, int64_t N, int n_x,
+                             T *__restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= len) return;
-    double s = 0.0;
-    for (int64_t c = 0; c < chunks; ++c) s += part64[c * len + i];
-    out[i] = (T)s;
+    const double s = sum_rows_in_order(part64 + i, 0, chunks, len, 0.0);
+    const int64_t v = i / N, n = i - v * N;
+    out[n * n_x + v] = (T)s;
 }
 
 // Bucket kernels <LT, SL, PPT, WPB, MEM>.
@@ -540,7 +574,7 @@ int gx_mode(bool want_gx, int64_t gx_len) {
 template <typename T, int LT, int SL, int PPT, int WPB, bool MEM>
 size_t fused_lds(int mode, int64_t gx_len) {
     return (mode == 1 ? (((size_t)WPB * gx_len * sizeof(T) + 15) / 16 * 16) : 0) +
-           (MEM ? 0 : (size_t)WPB * tape_elems(LT, PPT) * sizeof(T));
+           (MEM ? 0 : (size_t)WPB * f_tape_elems(LT, PPT) * sizeof(T));
 }
 
 // Persistent grid of one bucket: as many blocks as the CUs hold at once; with
@@ -634,7 +668,7 @@ FWs fused_layout(int64_t N, int64_t n_x, int64_t B, int64_t M, bool want_gx) {
     if (want_gx) off += (size_t)((w.rows + ROW_CHUNK - 1) / ROW_CHUNK) * gx_len * sizeof(double);
     off = (off + 255) / 256 * 256;
     w.gt_off = off;
-    if (w.grid[4]) off += (size_t)w.grid[4] * 2 * tape_elems(256, 1) * sizeof(T);
+    if (w.grid[4]) off += (size_t)w.grid[4] * 2 * f_tape_elems(256, 1) * sizeof(T);
     w.total = off + 256;
     return w;
 }
@@ -745,7 +779,7 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
         hipLaunchKernelGGL(k_sum_rows_1<T>, dim3(nb, (unsigned)chunks), dim3(256), 0, s, (const T *)(ws + L.gx_off),
                            rows, gx_len, (double *)(ws + L.g64_off));
         hipLaunchKernelGGL(k_sum_rows_2<T>, dim3(nb), dim3(256), 0, s, (const double *)(ws + L.g64_off), chunks,
-                           gx_len, grad_X);
+                           gx_len, N, (int)n_x, grad_X);
     }
     return (int)hipGetLastError();
 }

@@ -103,7 +103,6 @@ __device__ inline SVec<T, SL> compact_start(const CProg<T> &p) {
 // (2-bit kind, 11-bit index).
 constexpr int K_TAPE = 0, K_VAR = 1, K_CONST = 2, K_IMM = 3;
 constexpr int IMM_NOOP = 8;  // a NO_OP node used as an operand: NaN, and 201 if it receives a gradient
-constexpr int OP_COPY = 63;
 constexpr int MAX_NX_CODE = 2047;
 
 __host__ __device__ inline uint32_t instr_d0(uint32_t w) { return (w >> 6) & 0x1FFFu; }
@@ -342,14 +341,18 @@ __device__ inline int count_rows_wave(const int64_t *ops_row, const int64_t *ch_
 
 // Operand record of a node (used by its parent): scalar 0 | row 1-9 |
 // xg 10 | scatter mode 11-13 | scatter target 14-21 | constant used once 22.
+// var_adj_row0 >= 0: a variable v < nvr accumulates its adjoint in row
+// var_adj_row0 + v (PM_ACC) instead of PM_VAR.
 __device__ inline uint32_t operand_rec(uint32_t w, bool is_op, uint32_t ix, bool extra, uint32_t erow, bool once,
-                                       int var_row0, int nvr) {
+                                       int var_row0, int nvr, int var_adj_row0) {
     if (is_op) return (ix << 1) | ((extra ? PM_ACC : PM_WRITE) << 11) | ((extra ? erow : ix) << 14);
     const uint32_t d = operand_desc(w, 0u, false);
     const int kind = desc_kind(d);
     const uint32_t idx = (uint32_t)desc_idx(d);
     if (kind == K_VAR) {
         const uint32_t t = (idx > 255u ? 255u : idx) << 14;
+        if ((int)idx < nvr && var_adj_row0 >= 0)
+            return (((uint32_t)var_row0 + idx) << 1) | (PM_ACC << 11) | (((uint32_t)var_adj_row0 + idx) << 14);
         return (int)idx < nvr ? (((uint32_t)var_row0 + idx) << 1) | (PM_VAR << 11) | t : (1u << 10) | (PM_VAR << 11) | t;
     }
     if (kind == K_CONST) return 1u | (PM_CONST << 11) | (idx << 14) | ((once ? 1u : 0u) << 22);
@@ -357,9 +360,12 @@ __device__ inline uint32_t operand_rec(uint32_t w, bool is_op, uint32_t ix, bool
 }
 
 // Builds the plan of a compiled program (c = compile_code(p, M)).  Variables
-// v < nvr are tape rows var_row0 + v; the others are read from X.
+// v < nvr are tape rows var_row0 + v; the others are read from X.  With
+// var_adj_row0 >= 0 the adjoints of the variables v < nvr accumulate in rows
+// var_adj_row0 + v (at most 255), which the caller zeroes per tile group.
 template <typename T>
-__device__ inline Plan compile_plan(const CProg<T> &p, int M, const Code &c, int var_row0, int nvr) {
+__device__ inline Plan compile_plan(const CProg<T> &p, int M, const Code &c, int var_row0, int nvr,
+                                    int var_adj_row0 = -1) {
     const int lane = lane_id();
     const int root = p.root;
     const int NI = c.NI;
@@ -380,8 +386,8 @@ __device__ inline Plan compile_plan(const CProg<T> &p, int M, const Code &c, int
     // second-constant rows follow the extra instruction rows
     const uint32_t dr0 = (uint32_t)(NI + ne + __popcll(dm0 & below));
     const uint32_t dr1 = (uint32_t)(NI + ne + __popcll(dm0) + __popcll(dm1 & below));
-    const uint32_t r0 = operand_rec(w0, op0, ix0, ex0, er0, (cs0 >> lane) & 1ull, var_row0, nvr);
-    const uint32_t r1 = operand_rec(w1, op1, ix1, ex1, er1, (cs1 >> lane) & 1ull, var_row0, nvr);
+    const uint32_t r0 = operand_rec(w0, op0, ix0, ex0, er0, (cs0 >> lane) & 1ull, var_row0, nvr, var_adj_row0);
+    const uint32_t r1 = operand_rec(w1, op1, ix1, ex1, er1, (cs1 >> lane) & 1ull, var_row0, nvr, var_adj_row0);
     auto child_rec = [&](int ch) -> uint32_t {  // every lane takes part in both shuffles
         const int cc = ch & 127;
         const uint32_t lo = (uint32_t)__shfl((int)r0, cc & (WAVE - 1), WAVE);

@@ -279,6 +279,54 @@ __device__ inline void bwd_ops(int op, const T *g, const T *a0, const T *a1, T *
     }
 }
 
+// Opcode of the copy instruction of a leaf root (no reference operator has
+// this id).
+constexpr int OP_COPY = 63;
+
+// Backward of one instruction for P points with the reference's skip rule
+// and the tile group's active points folded in: ge0[p] / ge1[p] are the
+// amounts added to the operands' adjoints, 0 where the reference adds
+// nothing (or the point is past the end).  Only DIV raises an error (204,
+// a zero divisor under a nonzero adjoint at an active point), so only its
+// branch touches err; OP_COPY (a leaf root) passes the adjoint through.
+// Same values as bwd_nodes with the masks applied.
+template <typename T, int P>
+__device__ inline void bwd_masked(int op, const T *g, const T *a0, const T *a1, const bool *act, T *ge0, T *ge1,
+                                  int32_t &err) {
+    if (op >= ADD && op <= MUL) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const bool on = act[p] && !(g[p] == T(0));
+            const T m0 = g[p] * a1[p], m1 = g[p] * a0[p];
+            const T d1 = op == SUB ? -g[p] : g[p];
+            ge0[p] = on ? (op == MUL ? m0 : g[p]) : T(0);
+            ge1[p] = on ? (op == MUL ? m1 : d1) : T(0);
+        }
+    } else if (op == OP_COPY) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            ge0[p] = act[p] ? g[p] : T(0);
+            ge1[p] = T(0);
+        }
+    } else {
+        T go0[P], go1[P];
+        bool add0[P], add1[P];
+        int32_t e[P];
+        if (op == DIV) {
+            bwd_nodes<T, P>(DIV, g, a0, a1, go0, add0, go1, add1, e);
+#pragma unroll
+            for (int p = 0; p < P; ++p) err = (act[p] && e[p] > err) ? e[p] : err;
+        } else {
+            bwd_nodes<T, P>(op, g, a0, a1, go0, add0, go1, add1, e);
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            ge0[p] = (act[p] && add0[p]) ? go0[p] : T(0);
+            ge1[p] = (act[p] && add1[p]) ? go1[p] : T(0);
+        }
+    }
+}
+
 template <typename T>
 __device__ inline int32_t bwd_node(int op, T g, T a0, T a1, T &go0, bool &add0, T &go1, bool &add1) {
     int32_t e;

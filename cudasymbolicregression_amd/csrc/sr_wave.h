@@ -379,4 +379,23 @@ __device__ inline void finish_constants(const Program<T> &p, const GradAcc<T> &a
     }
 }
 
+// s + src[r0 * stride] + src[(r0 + 1) * stride] + ... + src[(r1 - 1) * stride],
+// added in FP64 in row order; the loads go out 16 at a time, so a thread
+// pays one memory latency per 16 rows instead of one per row.
+template <typename T>
+__device__ inline double sum_rows_in_order(const T *__restrict__ src, int64_t r0, int64_t r1, int64_t stride,
+                                           double s) {
+    constexpr int U = 16;
+    int64_t r = r0;
+    for (; r + U <= r1; r += U) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = src[(r + u) * stride];
+#pragma unroll
+        for (int u = 0; u < U; ++u) s += (double)v[u];
+    }
+    for (; r < r1; ++r) s += (double)src[r * stride];
+    return s;
+}
+
 }  // namespace sr
