@@ -79,6 +79,11 @@ def hip() -> ctypes.CDLL:
         _declare(lib, "sr_backward_mb_reads_only", I, [I64, I])
         _declare(lib, "sr_multi_backward_mb_workspace_bytes", I64, [I64, I64, I64, I64, I])
         _declare(lib, "sr_fused_workspace_bytes", I64, [I64, I64, I64, I64, I, I])
+        _declare(lib, "sr_fused_num_variants", I, [I, I])
+        _declare(lib, "sr_fused_variant_info", I, [I, I, I, POINTER(c_int), POINTER(c_int)])
+        for sfx in ("f32", "f64"):
+            _declare(lib, f"sr_fused_root_opts_{sfx}", I,
+                     [P, I64, I64, P, P, P, I64, I64, P, I, I, P, P, P, P, P, P, P, P, I64, P])
         _declare(lib, "sr_fused_noalloc_max_bytes", I64, [])
         _declare(lib, "sr_pcfg_sample_device", I, [P, P, P, P, I64, I64, I64, I64, P, P, c_uint64, I64, P])
         _declare(lib, "sr_pcfg_infix_to_postfix_device", I, [P, I64, P, P, P, I64, I64, I64, I64, P])

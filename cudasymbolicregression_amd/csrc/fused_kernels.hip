@@ -30,6 +30,7 @@
 #include "sr_device_info.h"
 #include "sr_queue.h"
 #include "sr_streams.h"
+#include "sr_eval.h"
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -613,12 +614,46 @@ int fused_launch(const FArgs &a, int grid, int mode, hipStream_t s) {
     return (int)hipGetLastError();
 }
 
-#define SR_F0(T) T, 8, 1, FCfg<T>::B0_PPT, 4, false
-#define SR_F1(T) T, 16, 1, FCfg<T>::B0_PPT, 4, false
-#define SR_F2(T) T, 32, 1, 1, 2, false
-#define SR_F3(T) T, 64, 1, 1, 2, false
-#define SR_F4(T) T, 256, 2, 1, 2, true
-constexpr int F_WPB[F_NBUCKETS] = {4, 4, 2, 2, 2};
+// Launch variants of the bucket kernels (waves per block x points per lane
+// per dispatch), chosen per call (sr_fused_options); variant 0 is the
+// default of each bucket (resolve_variants below picks among the first two
+// by N).  Only the FP32 kernels have alternatives.
+struct FVar {
+    int wpb, ppt;
+    int (*grid)(int mode, int64_t gx_len, int n_x);
+    int (*launch)(const FArgs &a, int grid, int mode, hipStream_t s);
+    int (*gxmode)(bool want_gx, int64_t gx_len);
+};
+template <typename T, int LT, int SL, int PPT, int WPB, bool MEM>
+constexpr FVar fvar() {
+    return FVar{WPB, PPT, &fused_grid<T, LT, SL, PPT, WPB, MEM>, &fused_launch<T, LT, SL, PPT, WPB, MEM>,
+                &gx_mode<T, WPB>};
+}
+constexpr int F_MAXVAR = 4;
+template <typename T> struct FTab;
+template <> struct FTab<float> {
+    static constexpr int n[F_NBUCKETS] = {4, 3, 3, 2, 1};
+    static constexpr FVar v[F_NBUCKETS][F_MAXVAR] = {
+        {fvar<float, 8, 1, 2, 4, false>(), fvar<float, 8, 1, 1, 4, false>(), fvar<float, 8, 1, 4, 4, false>(),
+         fvar<float, 8, 1, 2, 2, false>()},
+        {fvar<float, 16, 1, 2, 4, false>(), fvar<float, 16, 1, 1, 4, false>(), fvar<float, 16, 1, 2, 2, false>(),
+         FVar{}},
+        {fvar<float, 32, 1, 1, 2, false>(), fvar<float, 32, 1, 2, 2, false>(), fvar<float, 32, 1, 1, 4, false>(),
+         FVar{}},
+        {fvar<float, 64, 1, 1, 2, false>(), fvar<float, 64, 1, 1, 4, false>(), FVar{}, FVar{}},
+        {fvar<float, 256, 2, 1, 2, true>(), FVar{}, FVar{}, FVar{}},
+    };
+};
+template <> struct FTab<double> {
+    static constexpr int n[F_NBUCKETS] = {1, 1, 1, 1, 1};
+    static constexpr FVar v[F_NBUCKETS][F_MAXVAR] = {
+        {fvar<double, 8, 1, 1, 4, false>(), FVar{}, FVar{}, FVar{}},
+        {fvar<double, 16, 1, 1, 4, false>(), FVar{}, FVar{}, FVar{}},
+        {fvar<double, 32, 1, 1, 2, false>(), FVar{}, FVar{}, FVar{}},
+        {fvar<double, 64, 1, 1, 2, false>(), FVar{}, FVar{}, FVar{}},
+        {fvar<double, 256, 2, 1, 2, true>(), FVar{}, FVar{}, FVar{}},
+    };
+};
 
 // Workspace: lists [5][B] int32, counters [5]+[5] | queue | segment starts
 // and segment queue workspace (grad_X wanted) | grad_X partial rows | FP64
@@ -631,27 +666,44 @@ struct FWs {
     int64_t rows;           // partial rows in all
     int64_t W[F_NBUCKETS];  // waves per bucket launch (segments)
 };
+// var: each bucket's variant; nullptr = the largest workspace over every
+// variant of every bucket (what sr_fused_workspace_bytes reports, so one
+// workspace serves any choice).
 template <typename T>
-FWs fused_layout(int64_t N, int64_t n_x, int64_t B, int64_t M, bool want_gx) {
+FWs fused_layout(int64_t N, int64_t n_x, int64_t B, int64_t M, bool want_gx, const int *var = nullptr) {
     FWs w;
     const int64_t gx_len = N * n_x;
     const int ni = (int)n_x;
-    w.mode[0] = gx_mode<T, 4>(want_gx, gx_len);
-    w.mode[1] = gx_mode<T, 4>(want_gx, gx_len);
-    w.mode[2] = gx_mode<T, 2>(want_gx, gx_len);
-    w.mode[3] = gx_mode<T, 2>(want_gx, gx_len);
-    w.mode[4] = gx_mode<T, 2>(want_gx, gx_len);
-    w.grid[0] = fused_grid<SR_F0(T)>(w.mode[0], gx_len, ni);
     // a program of M nodes needs at most 2 M - 1 tape rows
-    w.grid[1] = M > 4 ? fused_grid<SR_F1(T)>(w.mode[1], gx_len, ni) : 0;
-    w.grid[2] = M > 8 ? fused_grid<SR_F2(T)>(w.mode[2], gx_len, ni) : 0;
-    w.grid[3] = M > 16 ? fused_grid<SR_F3(T)>(w.mode[3], gx_len, ni) : 0;
-    w.grid[4] = M > 32 ? fused_grid<SR_F4(T)>(w.mode[4], gx_len, ni) : 0;
+    const bool used[F_NBUCKETS] = {true, M > 4, M > 8, M > 16, M > 32};
+    int64_t rows_k[F_NBUCKETS];
+    for (int k = 0; k < F_NBUCKETS; ++k) {
+        w.grid[k] = 0;
+        w.mode[k] = 0;
+        w.W[k] = 0;
+        rows_k[k] = 0;
+        if (!used[k]) continue;
+        const int v0 = var ? var[k] : 0, v1 = var ? var[k] + 1 : FTab<T>::n[k];
+        for (int v = v0; v < v1; ++v) {
+            const FVar &f = FTab<T>::v[k][v];
+            const int mode = f.gxmode(want_gx, gx_len);
+            const int grid = f.grid(mode, gx_len, ni);
+            const int64_t W = (int64_t)grid * f.wpb;
+            const int64_t rows = mode == 1 ? grid : (mode == 2 ? W : 0);
+            if (var) {
+                w.grid[k] = grid;
+                w.mode[k] = mode;
+            } else if (grid > w.grid[k]) {
+                w.grid[k] = grid;
+            }
+            w.W[k] = W > w.W[k] ? W : w.W[k];
+            rows_k[k] = rows > rows_k[k] ? rows : rows_k[k];
+        }
+    }
     w.rows = 0;
     for (int k = 0; k < F_NBUCKETS; ++k) {
-        w.W[k] = (int64_t)w.grid[k] * F_WPB[k];
         w.row0[k] = w.rows;
-        w.rows += w.mode[k] == 1 ? w.grid[k] : (w.mode[k] == 2 ? w.W[k] : 0);
+        w.rows += rows_k[k];
     }
     size_t off = (((size_t)F_NBUCKETS * B * 4 + 64) + 255) / 256 * 256;
     w.q_off = off;
@@ -668,9 +720,33 @@ FWs fused_layout(int64_t N, int64_t n_x, int64_t B, int64_t M, bool want_gx) {
     if (want_gx) off += (size_t)((w.rows + ROW_CHUNK - 1) / ROW_CHUNK) * gx_len * sizeof(double);
     off = (off + 255) / 256 * 256;
     w.gt_off = off;
-    if (w.grid[4]) off += (size_t)w.grid[4] * 2 * f_tape_elems(256, 1) * sizeof(T);
+    if (w.grid[4]) off += (size_t)w.grid[4] * FTab<T>::v[4][0].wpb * f_tape_elems(256, 1) * sizeof(T);
     w.total = off + 256;
     return w;
+}
+
+// The variant of each bucket: the caller's choice, or (no options, or -1)
+// the automatic one: variant 0, except that a bucket whose variant 0 takes
+// two points per lane and variant 1 one switches to variant 1 when N padded
+// to 64-point tiles is at most 80% of N padded to 128-point ones (at N = 64
+// half of every 128-point tile is padding; two points per lane gain about
+// 15% at equal work).  False for a choice out of range.
+template <typename T>
+bool resolve_variants(const int *variant, int64_t N, int var[F_NBUCKETS]) {
+    for (int k = 0; k < F_NBUCKETS; ++k) {
+        int v = variant ? variant[k] : -1;
+        if (v >= FTab<T>::n[k]) return false;
+        if (v < 0) {
+            v = 0;
+            if (FTab<T>::n[k] > 1 && FTab<T>::v[k][0].ppt == 2 && FTab<T>::v[k][1].ppt == 1 &&
+                FTab<T>::v[k][1].wpb == FTab<T>::v[k][0].wpb) {
+                const int64_t pad1 = (N + 63) / 64 * 64, pad2 = (N + 127) / 128 * 128;
+                if (5 * pad1 <= 4 * pad2) v = 1;
+            }
+        }
+        var[k] = v;
+    }
+    return true;
 }
 
 template <typename T>
@@ -678,7 +754,7 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
                     int64_t B, const T *dy, int mode, int fwd_only, T *y_out, double *loss_out, T *grad_C,
                     T *grad_X, int32_t *err_b, int32_t *flags, void *workspace, int64_t workspace_bytes,
                     hipStream_t s, const int64_t *x_off = nullptr, const int64_t *y_off = nullptr,
-                    const int32_t *n_pts = nullptr, bool one_stream = false) {
+                    const int32_t *n_pts = nullptr, bool one_stream = false, const int *variant = nullptr) {
     if (M <= 0 || M > 128 || B < 0 || N <= 0 || n_x < 0 || n_x > MAX_NX_CODE + 1 || (mode != 0 && mode != 1))
         return -1;
     const bool ragged = x_off != nullptr || y_off != nullptr || n_pts != nullptr;
@@ -687,7 +763,9 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
     if (B > 0 && dy == nullptr && (mode == 1 || !fwd_only)) return -1;  // the kernel reads dy in these cases
     if (B > 0 && grad_C == nullptr && !fwd_only) return -1;
     const bool want_gx = grad_X != nullptr && !fwd_only;
-    const FWs L = fused_layout<T>(N, n_x, B, M, want_gx);
+    int var[F_NBUCKETS];
+    if (!resolve_variants<T>(variant, N, var)) return -1;
+    const FWs L = fused_layout<T>(N, n_x, B, M, want_gx, var);
     if (workspace_bytes < (int64_t)L.total) return -2;
     hipLaunchKernelGGL(k_fill_i32_e, dim3(1), dim3(64), 0, s, flags, 0, (int64_t)2);
     const int64_t gx_len = N * n_x;
@@ -741,16 +819,19 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
     };
     int r = (int)hipGetLastError();
     if (r) return r;
+    auto launch = [&](int k, hipStream_t st) {
+        return FTab<T>::v[k][var[k]].launch(bucket_args(k), L.grid[k], L.mode[k], st);
+    };
     if (M <= 4) {
-        if ((r = fused_launch<SR_F0(T)>(bucket_args(0), L.grid[0], L.mode[0], s))) return r;
+        if ((r = launch(0, s))) return r;
     } else if (one_stream) {
         // every bucket on the caller's stream, one after the other (no side
         // streams, so nothing is created on first use)
-        if (L.grid[4] && (r = fused_launch<SR_F4(T)>(bucket_args(4), L.grid[4], L.mode[4], s))) return r;
-        if (L.grid[3] && (r = fused_launch<SR_F3(T)>(bucket_args(3), L.grid[3], L.mode[3], s))) return r;
-        if (L.grid[2] && (r = fused_launch<SR_F2(T)>(bucket_args(2), L.grid[2], L.mode[2], s))) return r;
-        if (L.grid[1] && (r = fused_launch<SR_F1(T)>(bucket_args(1), L.grid[1], L.mode[1], s))) return r;
-        if ((r = fused_launch<SR_F0(T)>(bucket_args(0), L.grid[0], L.mode[0], s))) return r;
+        if (L.grid[4] && (r = launch(4, s))) return r;
+        if (L.grid[3] && (r = launch(3, s))) return r;
+        if (L.grid[2] && (r = launch(2, s))) return r;
+        if (L.grid[1] && (r = launch(1, s))) return r;
+        if ((r = launch(0, s))) return r;
     } else {
         // buckets 4+3, 2 and 1 on side streams, bucket 0 on the caller's
         // stream, joined back (also after a failed launch)
@@ -758,17 +839,12 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
         if ((r = fk.fork())) return fk.finish(r);
         // buckets 4 and 3 share a stream; with M <= 64 bucket 4 only takes
         // programs with extra adjoint rows (rare) and goes second
-        if (M > 64 && (r = fused_launch<SR_F4(T)>(bucket_args(4), L.grid[4], L.mode[4], fk.side(0))))
-            return fk.finish(r);
-        if (L.grid[3] && (r = fused_launch<SR_F3(T)>(bucket_args(3), L.grid[3], L.mode[3], fk.side(0))))
-            return fk.finish(r);
-        if (L.grid[4] && M <= 64 && (r = fused_launch<SR_F4(T)>(bucket_args(4), L.grid[4], L.mode[4], fk.side(0))))
-            return fk.finish(r);
-        if (L.grid[2] && (r = fused_launch<SR_F2(T)>(bucket_args(2), L.grid[2], L.mode[2], fk.side(1))))
-            return fk.finish(r);
-        if (L.grid[1] && (r = fused_launch<SR_F1(T)>(bucket_args(1), L.grid[1], L.mode[1], fk.side(2))))
-            return fk.finish(r);
-        if ((r = fused_launch<SR_F0(T)>(bucket_args(0), L.grid[0], L.mode[0], s))) return fk.finish(r);
+        if (M > 64 && (r = launch(4, fk.side(0)))) return fk.finish(r);
+        if (L.grid[3] && (r = launch(3, fk.side(0)))) return fk.finish(r);
+        if (L.grid[4] && M <= 64 && (r = launch(4, fk.side(0)))) return fk.finish(r);
+        if (L.grid[2] && (r = launch(2, fk.side(1)))) return fk.finish(r);
+        if (L.grid[1] && (r = launch(1, fk.side(2)))) return fk.finish(r);
+        if ((r = launch(0, s))) return fk.finish(r);
         if ((r = fk.finish(0))) return r;
     }
     if (want_gx) {
@@ -831,7 +907,9 @@ int sr_fused_root_noalloc_f32(const float *X, int64_t N, int64_t n_x, const int6
                               int32_t *flags, void *stream) {
     if (M <= 0 || M > 128 || B < 0 || N <= 0 || n_x < 0) return -1;
     const bool want_gx = grad_X != nullptr && !fwd_only;
-    if (fused_layout<float>(N, n_x, B, M, want_gx).total > FUSED_SCRATCH_BYTES) return -2;
+    int var[F_NBUCKETS];
+    if (!resolve_variants<float>(nullptr, N, var)) return -1;
+    if (fused_layout<float>(N, n_x, B, M, want_gx, var).total > FUSED_SCRATCH_BYTES) return -2;
     int e = 0;
     void *ws = scratch_ptr(&e);
     if (ws == nullptr) return e ? e : -1;
@@ -859,6 +937,38 @@ int sr_fused_root_f64(const double *X, int64_t N, int64_t n_x, const int64_t *op
                       int32_t *flags, void *workspace, int64_t workspace_bytes, void *stream) {
     return fused_root_code<double>(X, N, n_x, ops, ch, C, M, B, dy, mode, fwd_only, y_out, loss_out, grad_C, grad_X,
                                    err_b, flags, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+// Launch-variant table of the fused evaluator and its per-call form.
+int sr_fused_num_variants(int bucket, int is_f64) {
+    if (bucket < 0 || bucket >= F_NBUCKETS) return -1;
+    return is_f64 ? FTab<double>::n[bucket] : FTab<float>::n[bucket];
+}
+int sr_fused_variant_info(int bucket, int variant, int is_f64, int *waves_per_block, int *ppt) {
+    if (bucket < 0 || bucket >= F_NBUCKETS || variant < 0 || variant >= sr_fused_num_variants(bucket, is_f64))
+        return -1;
+    const FVar &f = is_f64 ? FTab<double>::v[bucket][variant] : FTab<float>::v[bucket][variant];
+    if (waves_per_block) *waves_per_block = f.wpb;
+    if (ppt) *ppt = f.ppt;
+    return 0;
+}
+int sr_fused_root_opts_f32(const float *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                           const float *C, int64_t M, int64_t B, const float *dy, int mode, int fwd_only,
+                           float *y_out, double *loss_out, float *grad_C, float *grad_X, int32_t *err_b,
+                           int32_t *flags, const sr_fused_options *opts, void *workspace, int64_t workspace_bytes,
+                           void *stream) {
+    return fused_root_code<float>(X, N, n_x, ops, ch, C, M, B, dy, mode, fwd_only, y_out, loss_out, grad_C, grad_X,
+                                  err_b, flags, workspace, workspace_bytes, (hipStream_t)stream, nullptr, nullptr,
+                                  nullptr, false, opts ? opts->variant : nullptr);
+}
+int sr_fused_root_opts_f64(const double *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                           const double *C, int64_t M, int64_t B, const double *dy, int mode, int fwd_only,
+                           double *y_out, double *loss_out, double *grad_C, double *grad_X, int32_t *err_b,
+                           int32_t *flags, const sr_fused_options *opts, void *workspace, int64_t workspace_bytes,
+                           void *stream) {
+    return fused_root_code<double>(X, N, n_x, ops, ch, C, M, B, dy, mode, fwd_only, y_out, loss_out, grad_C, grad_X,
+                                   err_b, flags, workspace, workspace_bytes, (hipStream_t)stream, nullptr, nullptr,
+                                   nullptr, false, opts ? opts->variant : nullptr);
 }
 
 // Ragged: expression b has n_pts[b] points at X + x_off[b] (row-major,

@@ -17,11 +17,12 @@ Inputs are batch-first: X (N, n_x), Ops (B, M), Ch (B, M, 2), C (B, M).
 """
 from __future__ import annotations
 
-from typing import Dict, Optional
+import ctypes
+from typing import Dict, Optional, Sequence
 
 import torch
 
-from . import _lib
+from . import _lib, autotune
 from ._C import _require_gpu, get_error_message
 
 
@@ -36,7 +37,13 @@ def fused_root(
     fwd_only: bool = False,
     raise_errors: bool = True,
     out: Optional[Dict[str, torch.Tensor]] = None,
+    config: Optional[Sequence[int]] = None,
 ) -> Dict[str, torch.Tensor]:
+    """config: launch variant per tape-row bucket (autotune.fused_variants);
+    None = the tuned configuration of this workload if `autotune.tune_fused`
+    stored one, else the built-in defaults.  Every variant gives the same y
+    and error codes bit for bit; loss, grad_C and grad_X differ only by the
+    order of their sums."""
     _require_gpu(X, "X")
     dev = X.device
     dt = X.dtype
@@ -80,12 +87,20 @@ def fused_root(
     lib = _lib.hip()
     is64 = dt == torch.float64
     ws_bytes = lib.sr_fused_workspace_bytes(N, n_x, B, M, int(gX is not None), int(is64))
-    ws = torch.empty(max(ws_bytes, 256), dtype=torch.uint8, device=dev)
-    fn = lib.sr_fused_root_f64 if is64 else lib.sr_fused_root_f32
+    ws = o.get("_ws")
+    if ws is None or ws.numel() < ws_bytes or ws.device != dev:
+        ws = torch.empty(max(ws_bytes, 256), dtype=torch.uint8, device=dev)
+        if out is not None:
+            o["_ws"] = ws
+    if config is None:
+        config = autotune.fused_lookup(dev, dt, M, N, n_x, gX is not None, bool(fwd_only))
+    opts = autotune.make_fused_options(config, is64) if config is not None else None
+    fn = lib.sr_fused_root_opts_f64 if is64 else lib.sr_fused_root_opts_f32
     _lib.check_status(
         fn(_lib.ptr(X), N, n_x, _lib.ptr(Ops), _lib.ptr(Ch), _lib.ptr(C), M, B, _lib.ptr(dyc), int(mode),
            int(bool(fwd_only)), _lib.ptr(y), _lib.ptr(loss), _lib.ptr(gC), _lib.ptr(gX), _lib.ptr(err),
-           _lib.ptr(flags), _lib.ptr(ws), ws.numel(), _lib.stream_of(dev)),
+           _lib.ptr(flags), ctypes.byref(opts) if opts is not None else None, _lib.ptr(ws), ws.numel(),
+           _lib.stream_of(dev)),
         "fused_root",
     )
     res = dict(y=y, err=err, flags=flags)

@@ -155,6 +155,37 @@ int sr_fused_root_f64(const double *X, int64_t N, int64_t n_x, const int64_t *op
                       double *y_out, double *loss_out, double *grad_C, double *grad_X, int32_t *err_b,
                       int32_t *flags, void *workspace, int64_t workspace_bytes, void *stream);
 
+/* Per-call launch choice of the fused evaluator (HOST memory; no
+ * counterpart in the reference, whose evaluator has one launch shape).
+ * variant[k]: launch variant of tape-row bucket k (k = 0..4 for programs of
+ * at most 8, 16, 32, 64 and 256 tape rows), or -1 = the automatic choice
+ * that sr_fused_root_* (no options) makes: variant 0, except that buckets 0
+ * and 1 take their one-point-per-lane variant 1 when N padded to 64-point
+ * tiles is at most 80% of N padded to 128-point ones (N <= 64 or
+ * 129 <= N <= 192).
+ * Variants differ in waves per block and points per lane; all give the
+ * same y and error codes bit for bit, and loss, grad_C and grad_X up to the
+ * order of their sums (over points in FP64; over expressions in FP32
+ * partial rows, then FP64).  The workspace of sr_fused_workspace_bytes
+ * serves every variant. */
+typedef struct sr_fused_options {
+    int variant[5];
+} sr_fused_options;
+/* Number of variants of bucket k (FP32: is_f64 = 0), or -1. */
+int sr_fused_num_variants(int bucket, int is_f64);
+/* Waves per block and points per lane of one variant; 0 or -1. */
+int sr_fused_variant_info(int bucket, int variant, int is_f64, int *waves_per_block, int *points_per_lane);
+int sr_fused_root_opts_f32(const float *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                           const float *C, int64_t M, int64_t B, const float *dy, int mode, int fwd_only,
+                           float *y_out, double *loss_out, float *grad_C, float *grad_X, int32_t *err_b,
+                           int32_t *flags, const sr_fused_options *opts, void *workspace, int64_t workspace_bytes,
+                           void *stream);
+int sr_fused_root_opts_f64(const double *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
+                           const double *C, int64_t M, int64_t B, const double *dy, int mode, int fwd_only,
+                           double *y_out, double *loss_out, double *grad_C, double *grad_X, int32_t *err_b,
+                           int32_t *flags, const sr_fused_options *opts, void *workspace, int64_t workspace_bytes,
+                           void *stream);
+
 /* Allocation-free form of sr_fused_root_f32, for callers that cannot
  * allocate or pass device memory for a workspace (code under graph
  * capture, tight serving loops).  No counterpart in the reference.  The
