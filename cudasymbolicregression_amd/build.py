@@ -23,6 +23,9 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-I" + INCLUDE,
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # a wave-uniform atomic (the work-queue fetch) reduces its lanes with DPP
+    # row operations instead of a 64-iteration scalar loop over the lanes
+    "-mllvm", "-amdgpu-atomic-optimizer-strategy=DPP",
     "-Wno-unused-command-line-argument",
 ]
 

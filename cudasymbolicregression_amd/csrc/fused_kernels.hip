@@ -80,6 +80,7 @@ struct FArgs {
     void *gx_partial;   // [rows][N*n_x] partial rows (grad_X wanted), or nullptr
     int64_t gx_row0;    // first partial row of this launch
     int gx_lds;         // 1: wave rows in LDS, one partial row per block; 0: one per wave
+    int64_t gx_rows;    // k_fused_wide: partial rows of this bucket (list chunks)
     const int32_t *seg; // static segments of this bucket's list (grad_X wanted), else nullptr
     int32_t *err_b;     // (B) or nullptr
     int32_t *flags;     // [0] validation (max), [1] backward (max)
@@ -93,6 +94,11 @@ struct FArgs {
     const int64_t *x_off;   // ragged: element offset of expression b's X rows (nullable)
     const int64_t *y_off;   // ragged: offset of expression b's dy / y values
     const int32_t *n_pts;   // ragged: number of points of expression b
+    // wide evaluator (k_fused_wide) only
+    const uint32_t *plan;   // compiled programs, PREC_WORDS words each (k_plan_fused)
+    const float *xt;        // X transposed: variable v's N values at xt + v * n_pad, zero-padded
+    int64_t n_pad;          // N rounded up to a multiple of WIDE_PAD
+    int vec;                // dy and y may be accessed 16 bytes at a time (N % 4 == 0, aligned)
 };
 
 // LT: most instructions of an expression in the bucket; SL: slot registers
@@ -502,6 +508,469 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Planned programs and the wide evaluator (FP32, shared X of at most
+// NXR_ROWS variables, programs of at most PLAN_MAX_ROWS tape rows).
+//
+// k_plan_fused compiles every short program once, one wave per program at
+// full occupancy, into a record that the bucket kernels load with two
+// instructions instead of compiling it again (the compilation is a chain of
+// dependent wave-uniform steps, slow at the bucket kernels' occupancy).  It
+// also validates and classifies every program, so Ops and Ch are read once.
+//
+// Record (PREC_WORDS 32-bit words): 0-15 the fetch words of instructions
+// 0..15, 16-31 their scatter words, 32-47 and 48-63 their two scalar
+// operands (float bits), 64-67 the tape row of constants 0..15 when used
+// once (one byte each, NO_CROW otherwise), 68 = NI | nextra << 8 | nc << 16
+// | used-variable mask << 24, 69-70 and 71-72 the constant-node masks of
+// nodes 0-63 and 64-127.  Tape rows: instructions and extra rows in
+// [0, LT), variable v in row LT + v (the block's copy of X, or the wave's
+// rows); variable adjoints go to registers (PM_VAR).
+//
+// k_fused_wide evaluates a tile group of 64 x PPT points per dispatch with
+// each lane's PPT points held as PPT / 4 float4 quads (four consecutive
+// points), so a tape row is read and written 16 bytes per lane.
+// ---------------------------------------------------------------------------
+constexpr int PREC_WORDS = 80;
+constexpr int PLAN_MAX_ROWS = 16;
+constexpr int PLAN_MAX_NC = 16;
+constexpr int WIDE_PAD = 1024;  // n_pad granularity: a multiple of every tile group (64 x PPT, PPT <= 16)
+
+// One wave per program.  Programs that fail validation get their code
+// (flags[0], err_b) and no bucket (-1).  With `plan`, programs of at most
+// PLAN_MAX_ROWS rows and PLAN_MAX_NC constants are compiled into their
+// record and go to bucket 0 (<= 8 rows, LT 8) or 1 (LT 16); the others
+// follow the rule of k_classify_fused over buckets 2-4.
+__global__ __launch_bounds__(256) void k_plan_fused(const int64_t *__restrict__ ops, const int64_t *__restrict__ ch,
+                                                    const float *__restrict__ C, int M, int64_t B, int n_x,
+                                                    int8_t *bucket_of, int32_t *cost_of, uint32_t *plan,
+                                                    int32_t *err_b, int32_t *flags) {
+    const int64_t b = (int64_t)blockIdx.x * 4 + uniform_i32((int)(threadIdx.x / WAVE));
+    if (b >= B) return;
+    const int lane = lane_id();
+    const CProg<float> p = load_cprog<float>(ops, ch, C, b, M, n_x);
+    if (p.verr) {
+        if (lane == 0) {
+            bucket_of[b] = (int8_t)-1;
+            if (cost_of) cost_of[b] = 1;
+            atomicMax(&flags[0], p.verr);
+            if (err_b) err_b[b] = p.verr;
+        }
+        return;
+    }
+    const Code c = compile_code(p, M);
+    const int rows = c.NI + extra_rows(p, M);
+    const bool planned = rows <= PLAN_MAX_ROWS && p.nc <= PLAN_MAX_NC;
+    const int bucket = planned ? (rows <= 8 ? 0 : 1) : (p.nc > 64 ? 4 : (rows <= 32 ? 2 : (rows <= 64 ? 3 : 4)));
+    if (lane == 0) {
+        bucket_of[b] = (int8_t)bucket;
+        if (cost_of) cost_of[b] = rows + 2;
+    }
+    if (!planned) return;
+    const int LT = bucket == 0 ? 8 : 16;
+    const Plan pl = compile_plan(p, M, c, LT, NXR_ROWS, -1);
+    const SVec<float, 1> xc = compact_start<float, 1>(p);
+    float sv0[1], sv1[1];
+    resolve_scalars<float, 1, PLAN_MAX_ROWS>(c, xc, sv0, sv1);
+    int vmask = 0;
+#pragma unroll
+    for (int v = 0; v < NXR_ROWS; ++v) {
+        const int vop = VAR_START_ID + v;
+        const bool u0 = lane < M && node_op(p.w0) == vop;
+        const bool u1 = lane + WAVE < M && node_op(p.w1) == vop;
+        vmask |= (v < n_x && (__ballot(u0) | __ballot(u1))) ? 1 << v : 0;
+    }
+    uint32_t *rec = plan + (size_t)b * PREC_WORDS;
+    const int src = lane & 15, part = lane >> 4;
+    const uint32_t fw = (uint32_t)__shfl((int)pl.f0, src, WAVE);
+    const uint32_t sw = (uint32_t)__shfl((int)pl.s0, src, WAVE);
+    const uint32_t s0 = __float_as_uint(__shfl(sv0[0], src, WAVE));
+    const uint32_t s1 = __float_as_uint(__shfl(sv1[0], src, WAVE));
+    rec[lane] = part == 0 ? fw : (part == 1 ? sw : (part == 2 ? s0 : s1));
+    // constant j's row is byte j of words 64..67
+    const uint32_t cr = pl.crow0 & 255u;
+    uint32_t cw = 0u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) cw |= ((uint32_t)__shfl((int)cr, (4 * lane + k) & (WAVE - 1), WAVE) & 255u) << (8 * k);
+    const uint32_t h0 = (uint32_t)c.NI | ((uint32_t)pl.nextra << 8) | ((uint32_t)p.nc << 16) | ((uint32_t)vmask << 24);
+    uint32_t hv = cw;
+    hv = lane == 4 ? h0 : hv;
+    hv = lane == 5 ? (uint32_t)p.cm0 : hv;
+    hv = lane == 6 ? (uint32_t)(p.cm0 >> 32) : hv;
+    hv = lane == 7 ? (uint32_t)p.cm1 : hv;
+    hv = lane == 8 ? (uint32_t)(p.cm1 >> 32) : hv;
+    if (lane < 9) rec[64 + lane] = hv;
+}
+
+// X (N, n_x) -> xt [NXR_ROWS][n_pad], variable-major, zero past N and n_x.
+__global__ void k_xt_f32(const float *__restrict__ X, int64_t N, int n_x, int64_t n_pad, float *__restrict__ xt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NXR_ROWS * n_pad) return;
+    const int64_t v = i / n_pad, n = i - v * n_pad;
+    xt[i] = (v < n_x && n < N) ? X[n * n_x + v] : 0.0f;
+}
+
+__device__ inline float4 f4_zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
+__device__ inline float f4_get(const float4 &v, int e) { return e == 0 ? v.x : (e == 1 ? v.y : (e == 2 ? v.z : v.w)); }
+
+// Four values of a row of n floats starting at point n0 (zero past n): one
+// 16-byte load when the caller allows it and the quad is whole.
+__device__ inline float4 load_quad(const float *__restrict__ row, int64_t n0, int64_t n, bool vec) {
+    if (vec && n0 + 3 < n) return *reinterpret_cast<const float4 *>(row + n0);
+    float4 r;
+    r.x = n0 < n ? row[n0] : 0.f;
+    r.y = n0 + 1 < n ? row[n0 + 1] : 0.f;
+    r.z = n0 + 2 < n ? row[n0 + 2] : 0.f;
+    r.w = n0 + 3 < n ? row[n0 + 3] : 0.f;
+    return r;
+}
+__device__ inline void store_quad(float *__restrict__ row, int64_t n0, int64_t n, bool vec, float4 v) {
+    if (vec && n0 + 3 < n) {
+        *reinterpret_cast<float4 *>(row + n0) = v;
+        return;
+    }
+    if (n0 < n) row[n0] = v.x;
+    if (n0 + 1 < n) row[n0 + 1] = v.y;
+    if (n0 + 2 < n) row[n0 + 2] = v.z;
+    if (n0 + 3 < n) row[n0 + 3] = v.w;
+}
+
+// LDS of the wide evaluator.  X for all points, variable-major ([v][n_pad],
+// one copy per block shared by its waves) when it takes at most
+// WIDE_XBLK_MAX bytes; otherwise each wave keeps its tile group's variables
+// in tape rows LT + v.  Then each wave's tape: its rows of 64 x PPT points.
+constexpr int64_t WIDE_XBLK_MAX = 32 * 1024;
+__host__ __device__ inline bool wide_xblk(int n_x, int64_t n_pad) {
+    return (int64_t)n_x * n_pad * (int64_t)sizeof(float) <= WIDE_XBLK_MAX;
+}
+__host__ __device__ inline int wide_rows(int LT, int n_x, int64_t n_pad) { return LT + (wide_xblk(n_x, n_pad) ? 0 : n_x); }
+__host__ __device__ inline size_t wide_xblk_bytes(int n_x, int64_t n_pad) {
+    return wide_xblk(n_x, n_pad) ? (size_t)n_x * n_pad * sizeof(float) : 0;
+}
+
+// LT: tape rows of the bucket (8 or 16); PPT: points per lane per dispatch
+// (a multiple of 4); WPB: waves per block.  Same per-point arithmetic and
+// reference rules as k_fused_code (fwd_ops, bwd_masked, the scatter modes),
+// so y and the error codes are the same bit for bit; the FP64 sums (loss,
+// grad_C) and the grad_X partial rows add the same terms in another order.
+// Variable adjoints of a tile group accumulate in registers (PM_VAR).
+template <int LT, int PPT, int WPB>
+__global__ __launch_bounds__(WPB * 64) void k_fused_wide(FArgs a) {
+    static_assert(PPT % 4 == 0, "points per lane come in quads");
+    constexpr int Q = PPT / 4;
+    constexpr int NCW = 4;  // constants summed in per-lane FP64 registers (the others per tile group)
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int wave = uniform_i32((int)(threadIdx.x / WAVE));
+    const int lane = lane_id();
+    const int n_x = a.n_x;
+    const int64_t N = a.N, NP = a.n_pad;
+    const bool want_gx = a.want_gx != 0 && !a.fwd_only;
+    const bool vec = a.vec != 0;
+    const bool xblk = wide_xblk(n_x, NP);
+    const int rows = wide_rows(LT, n_x, NP);
+    float4 *const xb4 = reinterpret_cast<float4 *>(smem);
+    float4 *const tape = reinterpret_cast<float4 *>(smem + wide_xblk_bytes(n_x, NP)) + (size_t)wave * rows * Q * WAVE;
+    const float *const dy = (const float *)a.dy;
+    float *const y_out = (float *)a.y_out;
+    float *const grad_C = (float *)a.grad_C;
+    const int64_t gx_len = N * n_x;
+    const int64_t NP4 = NP / 4;
+    if (xblk) {  // the block's copy of X
+        const float4 *src = reinterpret_cast<const float4 *>(a.xt);
+        for (int64_t z = threadIdx.x; z < n_x * NP4; z += WPB * WAVE) xb4[z] = src[(z / NP4) * NP4 + z % NP4];
+        __syncthreads();
+    }
+    // this wave's variables of one tile group in rows LT + v (no block copy)
+    const bool one_group = N <= (int64_t)WAVE * PPT;
+    auto put_var_rows = [&](int64_t base, int mask) {
+        for (int v = 0; v < n_x; ++v) {
+            if (!((mask >> v) & 1)) continue;
+            const float4 *src = reinterpret_cast<const float4 *>(a.xt + (int64_t)v * NP + base);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) tape[((LT + v) * Q + q) * WAVE + lane] = src[q * WAVE + lane];
+        }
+    };
+    if (!xblk && one_group) put_var_rows(0, (1 << n_x) - 1);
+    const int64_t count = (int64_t)*a.list_count;
+    // Work: with grad_X, the list is cut into gx_rows chunks of kc
+    // consecutive positions (trailing chunks may be empty); a wave takes a
+    // chunk from the shared counter and sums its programs' variable
+    // adjoints into the chunk's own partial row, variable-major (v * N + n),
+    // in list order.  Which wave runs a chunk does not change its row, so
+    // grad_X is the same on every call while the chunks balance the load.
+    // Without grad_X, waves take CH positions at a time.
+    const int64_t kc = want_gx ? (count + a.gx_rows - 1) / a.gx_rows : 0;
+    constexpr int CH = fetch_chunk(LT);
+    int64_t pos = 0, pos_end = 0;
+    float *gxrow = nullptr;
+    const bool need_dy = a.mode == 1 || !a.fwd_only;
+    const float inv_n = 1.0f / (float)N;
+    for (;;) {
+        if (pos >= pos_end) {
+            if (want_gx) {
+                const int64_t c = wave_fetch_add(a.work_counter);
+                if (kc == 0 || c * kc >= count) break;
+                pos = c * kc;
+                pos_end = pos + kc < count ? pos + kc : count;
+                gxrow = (float *)a.gx_partial + (a.gx_row0 + c) * gx_len;
+                if (vec) {
+                    for (int64_t z = lane; z < gx_len / 4; z += WAVE) reinterpret_cast<float4 *>(gxrow)[z] = f4_zero();
+                } else {
+                    for (int64_t z = lane; z < gx_len; z += WAVE) gxrow[z] = 0.0f;
+                }
+            } else {
+                const int64_t c = wave_fetch_add_n(a.work_counter, CH);
+                if (c >= count) break;
+                pos = c;
+                pos_end = c + CH < count ? c + CH : count;
+            }
+        }
+        const int64_t i = pos++;
+        const int64_t b = (int64_t)a.list[i];
+        const uint32_t *rec = a.plan + (size_t)b * PREC_WORDS;
+        const uint32_t rw = rec[lane];
+        const uint32_t hw = rec[64 + (lane & 15)];
+        const uint32_t h0 = rl_u32(hw, 4);
+        const int NI = (int)(h0 & 255u), nextra = (int)((h0 >> 8) & 255u), nc = (int)((h0 >> 16) & 255u);
+        const int vmask = (int)(h0 >> 24);
+        const uint64_t cm0 = (uint64_t)rl_u32(hw, 5) | ((uint64_t)rl_u32(hw, 6) << 32);
+        const uint64_t cm1 = (uint64_t)rl_u32(hw, 7) | ((uint64_t)rl_u32(hw, 8) << 32);
+        auto crow = [&](int j) -> int { return (int)((rl_u32(hw, j >> 2) >> (8 * (j & 3))) & 255u); };
+        const int64_t yo = b * N;
+        double lacc = 0.0;
+        double cacc[NCW];
+#pragma unroll
+        for (int q = 0; q < NCW; ++q) cacc[q] = 0.0;
+        double gc = 0.0;  // lane j: FP64 gradient of constant j (nc <= 16)
+        int32_t e = 0;
+        for (int64_t base = 0; base < N; base += (int64_t)WAVE * PPT) {
+            // row r of this tile group: the wave's tape, or the block's X
+            auto row4 = [&](int r) -> float4 * {
+                return (xblk && r >= LT) ? xb4 + (int64_t)(r - LT) * NP4 + base / 4 : tape + (size_t)r * Q * WAVE;
+            };
+            // operands of instruction ii (a unary instruction reads no second row)
+            auto fetch = [&](uint32_t f, int ii, float (&a0)[PPT], float (&a1)[PPT]) {
+                const bool bin = pf_bin(f);
+                const bool c0 = pf_sc(f, 0), c1 = pf_sc(f, 1) || !bin;
+                const float4 *p0 = row4(pf_row(f, 0)), *p1 = row4(pf_row(f, 1));
+                const float s0 = __uint_as_float(rl_u32(rw, 32 + ii));
+                const float s1 = bin ? __uint_as_float(rl_u32(rw, 48 + ii)) : 0.0f;
+                float4 t0[Q], t1[Q];
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    t0[q] = c0 ? f4_zero() : p0[q * WAVE + lane];
+                    t1[q] = c1 ? f4_zero() : p1[q * WAVE + lane];
+                }
+#pragma unroll
+                for (int q = 0; q < Q; ++q)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        a0[4 * q + k] = c0 ? s0 : f4_get(t0[q], k);
+                        a1[4 * q + k] = c1 ? s1 : f4_get(t1[q], k);
+                    }
+            };
+            bool act[PPT];
+            int64_t n0[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                n0[q] = base + (int64_t)q * (WAVE * 4) + (int64_t)lane * 4;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) act[4 * q + k] = n0[q] + k < N;
+            }
+            // upstream values (MODE 0) or targets (MODE 1), and this group's
+            // grad_X partial values of the used variables: loaded now, used
+            // after the forward pass and the reverse sweep
+            float4 dq[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) dq[q] = need_dy ? load_quad(dy + yo, n0[q], N, vec) : f4_zero();
+            float4 gxp[NXR_ROWS][Q];
+            float gxr[NXR_ROWS][PPT];
+#pragma unroll
+            for (int v = 0; v < NXR_ROWS; ++v) {
+#pragma unroll
+                for (int q = 0; q < Q; ++q)
+                    gxp[v][q] = (want_gx && v < n_x && ((vmask >> v) & 1)) ? load_quad(gxrow + (int64_t)v * N, n0[q], N, vec)
+                                                                          : f4_zero();
+#pragma unroll
+                for (int k = 0; k < PPT; ++k) gxr[v][k] = 0.0f;
+            }
+            if (!xblk && !one_group) put_var_rows(base, vmask);
+            for (int r = NI; r < NI + nextra; ++r) {
+                float4 *pr = row4(r);
+#pragma unroll
+                for (int q = 0; q < Q; ++q) pr[q * WAVE + lane] = f4_zero();
+            }
+            // ---- forward
+            for (int ii = 0; ii < NI; ++ii) {
+                const uint32_t f = rl_u32(rw, ii);
+                const int op = pf_op(f);
+                float a0[PPT], a1[PPT], r[PPT];
+                fetch(f, ii, a0, a1);
+                if (op == OP_COPY) {
+#pragma unroll
+                    for (int k = 0; k < PPT; ++k) r[k] = a0[k];
+                } else {
+                    fwd_ops<float, PPT>(op, a0, a1, r);
+                }
+                float4 *pw = row4(ii);
+#pragma unroll
+                for (int q = 0; q < Q; ++q)
+                    pw[q * WAVE + lane] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
+            }
+            // ---- root value, loss and upstream gradient; the root's adjoint
+            //      replaces its value
+            {
+                float4 *pr = row4(NI - 1);
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    const float4 yq = pr[q * WAVE + lane];
+                    if (y_out) store_quad(y_out + yo, n0[q], N, vec, yq);
+                    float gq[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const float yh = f4_get(yq, k), dvk = f4_get(dq[q], k);
+                        float g0 = 0.0f;
+                        if (a.mode == 1) {
+                            const float d = yh - dvk;
+                            if (act[4 * q + k]) lacc += (double)d * (double)d;
+                            g0 = (2.0f * d) * inv_n;
+                        } else if (!a.fwd_only) {
+                            g0 = dvk;
+                        }
+                        gq[k] = act[4 * q + k] ? g0 : 0.0f;
+                    }
+                    pr[q * WAVE + lane] = make_float4(gq[0], gq[1], gq[2], gq[3]);
+                }
+            }
+            if (a.fwd_only) continue;
+            // ---- reverse sweep
+            for (int ii = NI - 1; ii >= 0; --ii) {
+                const uint32_t f = rl_u32(rw, ii), sw = rl_u32(rw, 16 + ii);
+                const int op = pf_op(f);
+                const bool binary = pf_bin(f);
+                float g[PPT], a0[PPT], a1[PPT], ge0[PPT], ge1[PPT];
+                {
+                    const float4 *pg = row4(ps_grow(sw));
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) {
+                        const float4 t = pg[q * WAVE + lane];
+                        g[4 * q] = t.x;
+                        g[4 * q + 1] = t.y;
+                        g[4 * q + 2] = t.z;
+                        g[4 * q + 3] = t.w;
+                    }
+                }
+                fetch(f, ii, a0, a1);
+                bwd_masked<float, PPT>(op, g, a0, a1, act, ge0, ge1, e);
+                auto scatter = [&](int k, const float (&ge)[PPT]) {
+                    const uint32_t mode = ps_mode(sw, k);
+                    const int tg = ps_tgt(sw, k);
+                    if (mode - 1u < 3u) {  // PM_WRITE, PM_CROW, PM_ACC: a tape row
+                        float4 *pt = row4(tg);
+#pragma unroll
+                        for (int q = 0; q < Q; ++q) {
+                            const float4 o = mode == PM_ACC ? pt[q * WAVE + lane] : f4_zero();
+                            pt[q * WAVE + lane] = make_float4(o.x + ge[4 * q], o.y + ge[4 * q + 1], o.z + ge[4 * q + 2],
+                                                              o.w + ge[4 * q + 3]);
+                        }
+                    } else if (mode == PM_VAR) {  // a variable: its adjoint for grad_X
+                        if (want_gx) {
+#pragma unroll
+                            for (int v = 0; v < NXR_ROWS; ++v)
+                                if (v == tg)
+#pragma unroll
+                                    for (int k2 = 0; k2 < PPT; ++k2) gxr[v][k2] += ge[k2];
+                        }
+                    } else if (mode == PM_CONST) {  // a constant used several times: wave sum per use
+                        double gv = 0.0;
+#pragma unroll
+                        for (int k2 = 0; k2 < PPT; ++k2) gv += (double)ge[k2];
+                        const double s = wave_sum_rows(gv, 4);
+                        if (lane == tg) gc += s;
+                    } else if (mode == PM_NOOP) {
+#pragma unroll
+                        for (int k2 = 0; k2 < PPT; ++k2)
+                            if (ge[k2] != 0.0f) e = e > ERR_GRAD_NO_OP ? e : ERR_GRAD_NO_OP;
+                    }  // PM_NONE adds nothing
+                };
+                scatter(0, ge0);
+                if (binary) scatter(1, ge1);
+            }
+            // ---- adjoints of the constants used once, from their rows, in FP64
+            for (int j = 0; j < nc; ++j) {
+                const int row = crow(j);
+                if (row == (int)NO_CROW) continue;
+                const float4 *pc = row4(row);
+                double gv = 0.0;
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    const float4 t = pc[q * WAVE + lane];
+                    gv += (double)t.x;
+                    gv += (double)t.y;
+                    gv += (double)t.z;
+                    gv += (double)t.w;
+                }
+                if (j < NCW) {
+#pragma unroll
+                    for (int r = 0; r < NCW; ++r)
+                        if (r == j) cacc[r] += gv;
+                } else {
+                    const double s = wave_sum_rows(gv, 4);
+                    if (lane == j) gc += s;
+                }
+            }
+            // ---- variable adjoints of this tile group into the partial row
+            //      (a zero contribution is not added: the sum keeps a -0)
+            if (want_gx) {
+#pragma unroll
+                for (int v = 0; v < NXR_ROWS; ++v) {
+                    if (v >= n_x || !((vmask >> v) & 1)) continue;
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) {
+                        const float4 o = gxp[v][q];
+                        const float t0 = gxr[v][4 * q], t1 = gxr[v][4 * q + 1], t2 = gxr[v][4 * q + 2],
+                                    t3 = gxr[v][4 * q + 3];
+                        const float4 sm = make_float4(t0 == 0.f ? o.x : o.x + t0, t1 == 0.f ? o.y : o.y + t1,
+                                                      t2 == 0.f ? o.z : o.z + t2, t3 == 0.f ? o.w : o.w + t3);
+                        store_quad(gxrow + (int64_t)v * N, n0[q], N, vec, sm);
+                    }
+                }
+            }
+        }
+        if (a.mode == 1 && a.loss_out) {
+            const double s = wave_sum_rows(lacc, 4);
+            if (lane == 0) a.loss_out[b] = s / (double)N;
+        }
+        if (a.fwd_only) {
+            if (lane == 0 && a.err_b) a.err_b[b] = 0;
+            continue;
+        }
+        const int ncr = nc < NCW ? nc : NCW;
+#pragma unroll
+        for (int r = 0; r < NCW; ++r) {
+            if (r < ncr) {
+                const double s = wave_sum_rows(cacc[r], 4);
+                if (lane == r) gc += s;
+            }
+        }
+        // dL/dC back to node slots: constant node k takes its compact lane
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k = lane + h * WAVE;
+            const uint64_t cm = h == 0 ? cm0 : cm1;
+            const uint64_t below = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
+            const int o = (h == 0 ? 0 : __popcll(cm0)) + __popcll(cm & below);
+            const double v0 = __shfl(gc, o & (WAVE - 1), WAVE);
+            if (k < a.M) grad_C[b * a.M + k] = ((cm >> lane) & 1ull) ? (float)v0 : 0.0f;
+        }
+        e = wave_max_err(e);
+        if (lane == 0) {
+            if (e) atomicMax(&a.flags[1], e);
+            if (a.err_b) a.err_b[b] = e;
+        }
+    }
+}
+
 // Buckets by tape rows (count_rows_wave) and constant count nc: rows <= 8,
 // <= 16, <= 32, <= 64 (all with nc <= 64), the rest.  One wave per
 // expression; the run counts of the deterministic bucket lists follow in
@@ -536,13 +1005,37 @@ __global__ void k_fill_u64_e(uint64_t *p, uint64_t v, int64_t n) {
 // order -> out[i].
 constexpr int ROW_CHUNK = 32;
 
+// The partial rows of each bucket launch: [row0, row0 + cap).  A launch of
+// k_fused_wide with grad_X fills only the first rows of its range (one per
+// non-empty list chunk); the other launches fill all of theirs.
+struct RowSel {
+    int64_t row0[F_NBUCKETS], cap[F_NBUCKETS];
+    int chunked[F_NBUCKETS];
+    const int32_t *counts;  // bucket list lengths (device)
+};
+__device__ inline int64_t rows_used(const RowSel &rs, int k) {
+    if (!rs.chunked[k]) return rs.cap[k];
+    const int64_t c = rs.counts[k], R = rs.cap[k];
+    if (c <= 0 || R <= 0) return 0;
+    const int64_t kc = (c + R - 1) / R;
+    return (c + kc - 1) / kc;
+}
+
 template <typename T>
-__global__ void k_sum_rows_1(const T *__restrict__ part, int64_t P, int64_t len, double *__restrict__ part64) {
+__global__ void k_sum_rows_1(const T *__restrict__ part, int64_t P, int64_t len, double *__restrict__ part64,
+                             RowSel rs) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= len) return;
     const int64_t r0 = (int64_t)blockIdx.y * ROW_CHUNK;
     const int64_t r1 = r0 + ROW_CHUNK < P ? r0 + ROW_CHUNK : P;
-    part64[(int64_t)blockIdx.y * len + i] = sum_rows_in_order(part + i, r0, r1, len, 0.0);
+    double s = 0.0;
+    for (int k = 0; k < F_NBUCKETS; ++k) {  // the used rows in row order
+        const int64_t lo = r0 > rs.row0[k] ? r0 : rs.row0[k];
+        const int64_t e = rs.row0[k] + rows_used(rs, k);
+        const int64_t hi = r1 < e ? r1 : e;
+        if (lo < hi) s = sum_rows_in_order(part + i, lo, hi, len, s);
+    }
+    part64[(int64_t)blockIdx.y * len + i] = s;
 }
 // The rows are variable-major (v * N + n); grad_X is (N, n_x).
 template <typename T>
@@ -581,7 +1074,7 @@ size_t fused_lds(int mode, int64_t gx_len) {
 // Persistent grid of one bucket: as many blocks as the CUs hold at once; with
 // wave rows in the workspace, few enough that the rows stay within budget.
 template <typename T, int LT, int SL, int PPT, int WPB, bool MEM>
-int fused_grid(int mode, int64_t gx_len, int n_x) {
+int fused_grid(int mode, int64_t gx_len, int n_x, int64_t) {
     int per_cu = 0;
     if (MEM) {
         per_cu = 2;
@@ -620,38 +1113,79 @@ int fused_launch(const FArgs &a, int grid, int mode, hipStream_t s) {
 // by N).  Only the FP32 kernels have alternatives.
 struct FVar {
     int wpb, ppt;
-    int (*grid)(int mode, int64_t gx_len, int n_x);
+    int (*grid)(int mode, int64_t gx_len, int n_x, int64_t N);
     int (*launch)(const FArgs &a, int grid, int mode, hipStream_t s);
     int (*gxmode)(bool want_gx, int64_t gx_len);
+    bool wide;  // k_fused_wide: FP32, n_x <= NXR_ROWS, shared X, programs planned by k_plan_fused
 };
 template <typename T, int LT, int SL, int PPT, int WPB, bool MEM>
 constexpr FVar fvar() {
     return FVar{WPB, PPT, &fused_grid<T, LT, SL, PPT, WPB, MEM>, &fused_launch<T, LT, SL, PPT, WPB, MEM>,
-                &gx_mode<T, WPB>};
+                &gx_mode<T, WPB>, false};
 }
-constexpr int F_MAXVAR = 4;
+
+// The wide evaluator's LDS (the block's X copy and WPB tapes).  With grad_X
+// wanted its partial rows are in the workspace (row mode 2), one per list
+// chunk.
+inline int64_t wide_npad(int64_t N) { return (N + WIDE_PAD - 1) / WIDE_PAD * WIDE_PAD; }
+template <int LT, int PPT, int WPB>
+size_t wide_lds(int n_x, int64_t N) {
+    const int64_t np = wide_npad(N);
+    return wide_xblk_bytes(n_x, np) + (size_t)WPB * wide_rows(LT, n_x, np) * PPT * WAVE * sizeof(float);
+}
+inline int wide_gx_mode(bool want_gx, int64_t) { return want_gx ? 2 : 0; }
+template <int LT, int PPT, int WPB>
+int wide_grid(int mode, int64_t gx_len, int n_x, int64_t N) {
+    int per_cu = 0;
+    const size_t lds = wide_lds<LT, PPT, WPB>(n_x, N);
+    if (lds > lds_limit_device() ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_fused_wide<LT, PPT, WPB>, WPB * 64,
+                                                      lds) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    int64_t grid = (int64_t)num_cus() * per_cu;
+    if (mode == 2) {
+        const int64_t row_bytes = gx_len * (int64_t)sizeof(float) * WPB;
+        const int64_t cap = (int64_t)GX_GLOBAL_BUDGET / (row_bytes > 0 ? row_bytes : 1);
+        grid = grid < cap ? grid : (cap > 1 ? cap : 1);
+    }
+    return (int)grid;
+}
+template <int LT, int PPT, int WPB>
+int wide_launch(const FArgs &a, int grid, int mode, hipStream_t s) {
+    const size_t lds = wide_lds<LT, PPT, WPB>(a.n_x, a.N);
+    if (lds > lds_limit_device()) return -3;
+    hipLaunchKernelGGL((k_fused_wide<LT, PPT, WPB>), dim3((unsigned)grid), dim3(WPB * 64), lds, s, a);
+    return (int)hipGetLastError();
+}
+template <int LT, int PPT, int WPB>
+constexpr FVar fvar_wide() {
+    return FVar{WPB, PPT, &wide_grid<LT, PPT, WPB>, &wide_launch<LT, PPT, WPB>, &wide_gx_mode, true};
+}
+
+constexpr int F_MAXVAR = 6;
 template <typename T> struct FTab;
 template <> struct FTab<float> {
-    static constexpr int n[F_NBUCKETS] = {4, 3, 3, 2, 1};
+    static constexpr int n[F_NBUCKETS] = {6, 5, 3, 2, 1};
     static constexpr FVar v[F_NBUCKETS][F_MAXVAR] = {
-        {fvar<float, 8, 1, 2, 4, false>(), fvar<float, 8, 1, 1, 4, false>(), fvar<float, 8, 1, 4, 4, false>(),
-         fvar<float, 8, 1, 2, 2, false>()},
-        {fvar<float, 16, 1, 2, 4, false>(), fvar<float, 16, 1, 1, 4, false>(), fvar<float, 16, 1, 2, 2, false>(),
-         FVar{}},
+        {fvar_wide<8, 4, 4>(), fvar_wide<8, 8, 2>(), fvar<float, 8, 1, 2, 4, false>(),
+         fvar<float, 8, 1, 1, 4, false>(), fvar<float, 8, 1, 4, 4, false>(), fvar<float, 8, 1, 2, 2, false>()},
+        {fvar_wide<16, 4, 2>(), fvar_wide<16, 4, 4>(), fvar<float, 16, 1, 2, 4, false>(),
+         fvar<float, 16, 1, 1, 4, false>(), fvar<float, 16, 1, 2, 2, false>(), FVar{}},
         {fvar<float, 32, 1, 1, 2, false>(), fvar<float, 32, 1, 2, 2, false>(), fvar<float, 32, 1, 1, 4, false>(),
-         FVar{}},
-        {fvar<float, 64, 1, 1, 2, false>(), fvar<float, 64, 1, 1, 4, false>(), FVar{}, FVar{}},
-        {fvar<float, 256, 2, 1, 2, true>(), FVar{}, FVar{}, FVar{}},
+         FVar{}, FVar{}, FVar{}},
+        {fvar<float, 64, 1, 1, 2, false>(), fvar<float, 64, 1, 1, 4, false>(), FVar{}, FVar{}, FVar{}, FVar{}},
+        {fvar<float, 256, 2, 1, 2, true>(), FVar{}, FVar{}, FVar{}, FVar{}, FVar{}},
     };
 };
 template <> struct FTab<double> {
     static constexpr int n[F_NBUCKETS] = {1, 1, 1, 1, 1};
     static constexpr FVar v[F_NBUCKETS][F_MAXVAR] = {
-        {fvar<double, 8, 1, 1, 4, false>(), FVar{}, FVar{}, FVar{}},
-        {fvar<double, 16, 1, 1, 4, false>(), FVar{}, FVar{}, FVar{}},
-        {fvar<double, 32, 1, 1, 2, false>(), FVar{}, FVar{}, FVar{}},
-        {fvar<double, 64, 1, 1, 2, false>(), FVar{}, FVar{}, FVar{}},
-        {fvar<double, 256, 2, 1, 2, true>(), FVar{}, FVar{}, FVar{}},
+        {fvar<double, 8, 1, 1, 4, false>(), FVar{}, FVar{}, FVar{}, FVar{}, FVar{}},
+        {fvar<double, 16, 1, 1, 4, false>(), FVar{}, FVar{}, FVar{}, FVar{}, FVar{}},
+        {fvar<double, 32, 1, 1, 2, false>(), FVar{}, FVar{}, FVar{}, FVar{}, FVar{}},
+        {fvar<double, 64, 1, 1, 2, false>(), FVar{}, FVar{}, FVar{}, FVar{}, FVar{}},
+        {fvar<double, 256, 2, 1, 2, true>(), FVar{}, FVar{}, FVar{}, FVar{}, FVar{}},
     };
 };
 
@@ -659,7 +1193,10 @@ template <> struct FTab<double> {
 // and segment queue workspace (grad_X wanted) | grad_X partial rows | FP64
 // chunk sums of the rows | global tape (bucket 4).
 struct FWs {
-    size_t q_off, seg_off, sq_off, gx_off, g64_off, gt_off, total;
+    size_t q_off, seg_off, sq_off, gx_off, g64_off, gt_off, plan_off, xt_off, total;
+    int64_t nrows[F_NBUCKETS];  // partial rows per bucket
+    int chunked[F_NBUCKETS];    // k_fused_wide with grad_X: rows per list chunk
+    bool wide;              // some bucket runs k_fused_wide: plans and xt are laid out
     int grid[F_NBUCKETS];   // blocks per bucket launch (0: not launched)
     int mode[F_NBUCKETS];   // grad_X row mode per bucket
     int64_t row0[F_NBUCKETS];  // first partial row per bucket
@@ -681,18 +1218,25 @@ FWs fused_layout(int64_t N, int64_t n_x, int64_t B, int64_t M, bool want_gx, con
         w.grid[k] = 0;
         w.mode[k] = 0;
         w.W[k] = 0;
+        w.chunked[k] = 0;
         rows_k[k] = 0;
         if (!used[k]) continue;
         const int v0 = var ? var[k] : 0, v1 = var ? var[k] + 1 : FTab<T>::n[k];
         for (int v = v0; v < v1; ++v) {
             const FVar &f = FTab<T>::v[k][v];
             const int mode = f.gxmode(want_gx, gx_len);
-            const int grid = f.grid(mode, gx_len, ni);
+            const int grid = f.grid(mode, gx_len, ni, N);
             const int64_t W = (int64_t)grid * f.wpb;
-            const int64_t rows = mode == 1 ? grid : (mode == 2 ? W : 0);
+            // k_fused_wide: two list chunks per wave, each with its own row
+            int64_t rows = mode == 1 ? grid : (mode == 2 ? W : 0);
+            if (f.wide && mode == 2) {
+                const int64_t cap = (int64_t)GX_GLOBAL_BUDGET / (gx_len * (int64_t)sizeof(T) > 0 ? gx_len * (int64_t)sizeof(T) : 1);
+                rows = 2 * W < cap ? 2 * W : (cap > W ? cap : W);
+            }
             if (var) {
                 w.grid[k] = grid;
                 w.mode[k] = mode;
+                w.chunked[k] = f.wide && mode == 2;
             } else if (grid > w.grid[k]) {
                 w.grid[k] = grid;
             }
@@ -703,6 +1247,7 @@ FWs fused_layout(int64_t N, int64_t n_x, int64_t B, int64_t M, bool want_gx, con
     w.rows = 0;
     for (int k = 0; k < F_NBUCKETS; ++k) {
         w.row0[k] = w.rows;
+        w.nrows[k] = rows_k[k];
         w.rows += rows_k[k];
     }
     size_t off = (((size_t)F_NBUCKETS * B * 4 + 64) + 255) / 256 * 256;
@@ -721,6 +1266,20 @@ FWs fused_layout(int64_t N, int64_t n_x, int64_t B, int64_t M, bool want_gx, con
     off = (off + 255) / 256 * 256;
     w.gt_off = off;
     if (w.grid[4]) off += (size_t)w.grid[4] * FTab<T>::v[4][0].wpb * f_tape_elems(256, 1) * sizeof(T);
+    off = (off + 255) / 256 * 256;
+    // plan records and the transposed X of the wide evaluator
+    w.wide = false;
+    for (int k = 0; k < 2; ++k)
+        for (int v = var ? var[k] : 0; v < (var ? var[k] + 1 : FTab<T>::n[k]); ++v)
+            w.wide = w.wide || FTab<T>::v[k][v].wide;
+    w.plan_off = off;
+    w.xt_off = off;
+    if (w.wide) {
+        off += (size_t)B * PREC_WORDS * sizeof(uint32_t);
+        off = (off + 255) / 256 * 256;
+        w.xt_off = off;
+        off += (size_t)NXR_ROWS * wide_npad(N) * sizeof(float);
+    }
     w.total = off + 256;
     return w;
 }
@@ -731,17 +1290,24 @@ FWs fused_layout(int64_t N, int64_t n_x, int64_t B, int64_t M, bool want_gx, con
 // to 64-point tiles is at most 80% of N padded to 128-point ones (at N = 64
 // half of every 128-point tile is padding; two points per lane gain about
 // 15% at equal work).  False for a choice out of range.
+// wide_ok: the call may run k_fused_wide (FP32, at most NXR_ROWS variables,
+// shared X); otherwise a wide choice counts as automatic, and the automatic
+// choice skips the wide variants.
 template <typename T>
-bool resolve_variants(const int *variant, int64_t N, int var[F_NBUCKETS]) {
+bool resolve_variants(const int *variant, int64_t N, bool wide_ok, int var[F_NBUCKETS]) {
     for (int k = 0; k < F_NBUCKETS; ++k) {
         int v = variant ? variant[k] : -1;
         if (v >= FTab<T>::n[k]) return false;
+        if (v >= 0 && FTab<T>::v[k][v].wide && !wide_ok) v = -1;
         if (v < 0) {
-            v = 0;
-            if (FTab<T>::n[k] > 1 && FTab<T>::v[k][0].ppt == 2 && FTab<T>::v[k][1].ppt == 1 &&
-                FTab<T>::v[k][1].wpb == FTab<T>::v[k][0].wpb) {
+            int v0 = 0;
+            while (!wide_ok && v0 + 1 < FTab<T>::n[k] && FTab<T>::v[k][v0].wide) ++v0;
+            v = v0;
+            const FVar &f0 = FTab<T>::v[k][v0];
+            if (!f0.wide && v0 + 1 < FTab<T>::n[k] && f0.ppt == 2 && FTab<T>::v[k][v0 + 1].ppt == 1 &&
+                FTab<T>::v[k][v0 + 1].wpb == f0.wpb) {
                 const int64_t pad1 = (N + 63) / 64 * 64, pad2 = (N + 127) / 128 * 128;
-                if (5 * pad1 <= 4 * pad2) v = 1;
+                if (5 * pad1 <= 4 * pad2) v = v0 + 1;
             }
         }
         var[k] = v;
@@ -763,8 +1329,9 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
     if (B > 0 && dy == nullptr && (mode == 1 || !fwd_only)) return -1;  // the kernel reads dy in these cases
     if (B > 0 && grad_C == nullptr && !fwd_only) return -1;
     const bool want_gx = grad_X != nullptr && !fwd_only;
+    const bool wide_ok = sizeof(T) == 4 && n_x <= NXR_ROWS && !ragged;
     int var[F_NBUCKETS];
-    if (!resolve_variants<T>(variant, N, var)) return -1;
+    if (!resolve_variants<T>(variant, N, wide_ok, var)) return -1;
     const FWs L = fused_layout<T>(N, n_x, B, M, want_gx, var);
     if (workspace_bytes < (int64_t)L.total) return -2;
     hipLaunchKernelGGL(k_fill_i32_e, dim3(1), dim3(64), 0, s, flags, 0, (int64_t)2);
@@ -795,8 +1362,21 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
         }
     }
     void *sqws = want_gx ? (void *)(ws + L.sq_off) : nullptr;
-    hipLaunchKernelGGL(k_classify_fused, dim3((unsigned)((B + 3) / 4)), dim3(256), 0, s, ops, ch, (int)M, B,
-                       (int)n_x, queue_bucket_of(qws), want_gx ? queue_cost_of(sqws) : (int32_t *)nullptr);
+    const int64_t n_pad = wide_npad(N);
+    if (L.wide) {
+        // programs compiled once into plan records (validation and
+        // classification included); X transposed for the variable rows
+        const int64_t nxt = (int64_t)NXR_ROWS * n_pad;
+        hipLaunchKernelGGL(k_xt_f32, dim3((unsigned)((nxt + 255) / 256)), dim3(256), 0, s, (const float *)X, N,
+                           (int)n_x, n_pad, (float *)(ws + L.xt_off));
+        hipLaunchKernelGGL(k_plan_fused, dim3((unsigned)((B + 3) / 4)), dim3(256), 0, s, ops, ch, (const float *)C,
+                           (int)M, B, (int)n_x, queue_bucket_of(qws),
+                           want_gx ? queue_cost_of(sqws) : (int32_t *)nullptr, (uint32_t *)(ws + L.plan_off), err_b,
+                           flags);
+    } else {
+        hipLaunchKernelGGL(k_classify_fused, dim3((unsigned)((B + 3) / 4)), dim3(256), 0, s, ops, ch, (int)M, B,
+                           (int)n_x, queue_bucket_of(qws), want_gx ? queue_cost_of(sqws) : (int32_t *)nullptr);
+    }
     queue_finish(qws, B, F_NBUCKETS, lists, counters, s, want_gx ? &segs : nullptr, sqws);
     FArgs a;
     memset(&a, 0, sizeof(a));
@@ -807,12 +1387,19 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
     a.gtape = L.grid[4] ? (void *)(ws + L.gt_off) : nullptr;
     a.mode = mode; a.fwd_only = fwd_only; a.want_gx = want_gx ? 1 : 0;
     a.x_off = x_off; a.y_off = y_off; a.n_pts = n_pts;
+    if (L.wide) {
+        a.plan = (const uint32_t *)(ws + L.plan_off);
+        a.xt = (const float *)(ws + L.xt_off);
+        a.n_pad = n_pad;
+        a.vec = (N % 4 == 0 && ((uintptr_t)dy & 15) == 0 && ((uintptr_t)y_out & 15) == 0) ? 1 : 0;
+    }
     auto bucket_args = [&](int k) {
         FArgs r = a;
         r.list = lists + (size_t)k * B;
         r.list_count = counters + k;
         r.work_counter = counters + F_NBUCKETS + k;
         r.gx_row0 = L.row0[k];
+        r.gx_rows = L.nrows[k];
         r.gx_lds = L.mode[k] == 1 ? 1 : 0;
         r.seg = want_gx ? segs.seg[k] : nullptr;
         return r;
@@ -852,8 +1439,15 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
         const int64_t chunks = (rows + ROW_CHUNK - 1) / ROW_CHUNK;
         const unsigned nb = (unsigned)((gx_len + 255) / 256);
         if (chunks > 65535) return -1;
+        RowSel rs;
+        for (int k = 0; k < F_NBUCKETS; ++k) {
+            rs.row0[k] = L.row0[k];
+            rs.cap[k] = L.nrows[k];
+            rs.chunked[k] = L.chunked[k];
+        }
+        rs.counts = counters;
         hipLaunchKernelGGL(k_sum_rows_1<T>, dim3(nb, (unsigned)chunks), dim3(256), 0, s, (const T *)(ws + L.gx_off),
-                           rows, gx_len, (double *)(ws + L.g64_off));
+                           rows, gx_len, (double *)(ws + L.g64_off), rs);
         hipLaunchKernelGGL(k_sum_rows_2<T>, dim3(nb), dim3(256), 0, s, (const double *)(ws + L.g64_off), chunks,
                            gx_len, N, (int)n_x, grad_X);
     }
@@ -867,7 +1461,7 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
 // array's name itself is only a host-side handle and is never passed to a
 // kernel.  Every call on a GPU shares that GPU's copy.
 // ---------------------------------------------------------------------------
-constexpr size_t FUSED_SCRATCH_BYTES = 8u << 20;
+constexpr size_t FUSED_SCRATCH_BYTES = 128u << 20;
 __device__ __align__(256) unsigned char scratch[FUSED_SCRATCH_BYTES];
 
 constexpr int SCRATCH_MAX_DEVICES = 64;
@@ -908,7 +1502,7 @@ int sr_fused_root_noalloc_f32(const float *X, int64_t N, int64_t n_x, const int6
     if (M <= 0 || M > 128 || B < 0 || N <= 0 || n_x < 0) return -1;
     const bool want_gx = grad_X != nullptr && !fwd_only;
     int var[F_NBUCKETS];
-    if (!resolve_variants<float>(nullptr, N, var)) return -1;
+    if (!resolve_variants<float>(nullptr, N, n_x <= NXR_ROWS, var)) return -1;
     if (fused_layout<float>(N, n_x, B, M, want_gx, var).total > FUSED_SCRATCH_BYTES) return -2;
     int e = 0;
     void *ws = scratch_ptr(&e);

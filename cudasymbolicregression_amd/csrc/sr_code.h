@@ -339,6 +339,20 @@ __device__ inline int count_rows_wave(const int64_t *ops_row, const int64_t *ch_
     return ni + __popcll(em0) + __popcll(em1) + __popcll(dm0) + __popcll(dm1);
 }
 
+// Extra tape rows of a loaded program (the rows beyond its instructions, as
+// compile_plan assigns them); NI + extra_rows is the program's row count.
+template <typename T>
+__device__ inline int extra_rows(const CProg<T> &p, int M) {
+    const int lane = lane_id();
+    const int root = p.root;
+    const uint64_t om0 = __ballot(lane < M && lane <= root && arity_of(node_op(p.w0)) > 0);
+    const uint64_t om1 = __ballot(lane + WAVE < M && lane + WAVE <= root && arity_of(node_op(p.w1)) > 0);
+    uint64_t em0, em1, cs0, cs1, dm0, dm1;
+    node_uses(p.w0, p.w1, om0, om1, root, M, em0, em1, cs0, cs1);
+    second_const_rows(p.w0, p.w1, om0, om1, cs0, cs1, dm0, dm1);
+    return __popcll(em0) + __popcll(em1) + __popcll(dm0) + __popcll(dm1);
+}
+
 // Operand record of a node (used by its parent): scalar 0 | row 1-9 |
 // xg 10 | scatter mode 11-13 | scatter target 14-21 | constant used once 22.
 // var_adj_row0 >= 0: a variable v < nvr accumulates its adjoint in row

@@ -159,13 +159,18 @@ int sr_fused_root_f64(const double *X, int64_t N, int64_t n_x, const int64_t *op
  * counterpart in the reference, whose evaluator has one launch shape).
  * variant[k]: launch variant of tape-row bucket k (k = 0..4 for programs of
  * at most 8, 16, 32, 64 and 256 tape rows), or -1 = the automatic choice
- * that sr_fused_root_* (no options) makes: variant 0, except that buckets 0
- * and 1 take their one-point-per-lane variant 1 when N padded to 64-point
- * tiles is at most 80% of N padded to 128-point ones (N <= 64 or
- * 129 <= N <= 192).
- * Variants differ in waves per block and points per lane; all give the
- * same y and error codes bit for bit, and loss, grad_C and grad_X up to the
- * order of their sums (over points in FP64; over expressions in FP32
+ * that sr_fused_root_* (no options) makes: variant 0.  In FP32 with at most
+ * 4 variables and one shared X, variants 0 and 1 of buckets 0 and 1 are the
+ * wide evaluator (programs compiled once by a pre-pass into plan records,
+ * four consecutive points per lane per 16-byte tape access); the other
+ * variants, and every call that cannot use the wide evaluator (FP64, more
+ * variables, the ragged form), run one wave per program with its own
+ * compilation, where the automatic choice takes the one-point-per-lane
+ * variant when N padded to 64-point tiles is at most 80% of N padded to
+ * 128-point ones (N <= 64 or 129 <= N <= 192).
+ * Variants differ in kernel, waves per block and points per lane; all give
+ * the same y and error codes bit for bit, and loss, grad_C and grad_X up to
+ * the order of their sums (over points in FP64; over expressions in FP32
  * partial rows, then FP64).  The workspace of sr_fused_workspace_bytes
  * serves every variant. */
 typedef struct sr_fused_options {
@@ -190,10 +195,11 @@ int sr_fused_root_opts_f64(const double *X, int64_t N, int64_t n_x, const int64_
  * allocate or pass device memory for a workspace (code under graph
  * capture, tight serving loops).  No counterpart in the reference.  The
  * workspace is a fixed scratch array inside the library, one copy per GPU
- * (sr_fused_noalloc_max_bytes() bytes: 8 MiB, enough for about 380k
+ * (sr_fused_noalloc_max_bytes() bytes: 128 MiB, enough for about 380k
  * expressions of at most 32 nodes without grad_X), held until the process
- * ends.  Calls that need more (longer programs use global tapes, grad_X
- * needs partial rows) return -2 without launching.  The call allocates
+ * ends.  Calls that need more (each expression takes a 320-byte plan
+ * record, longer programs use global tapes, grad_X needs partial rows)
+ * return -2 without launching.  The call allocates
  * nothing, creates no stream or event, and runs every kernel on `stream`.
  * Every call on one GPU uses the same scratch: calls that overlap on a GPU
  * (several streams or host threads at once) race and give wrong results;

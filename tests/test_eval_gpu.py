@@ -307,8 +307,11 @@ def test_multiple_constants_match_oracle(K):
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_fused_root_ragged_equals_per_problem_calls(mode, dtype):
     """Each expression on its own points (N_b in [1, 300]) in one ragged call
-    gives, bit for bit, what a dense call on that expression alone gives:
-    y, loss (mode 1), grad_C and error codes."""
+    gives, bit for bit, what a dense call on that expression alone gives with
+    the same bucket kernels: y, loss (mode 1), grad_C and error codes.  (The
+    ragged form runs the one-wave-per-program kernels, variant 2 of buckets 0
+    and 1; a default dense FP32 call runs the planned wide kernels, whose
+    FP64 sums group the points differently.)"""
     from cudasymbolicregression_amd import fused
 
     rng = np.random.default_rng(50 + mode)
@@ -327,9 +330,10 @@ def test_fused_root_ragged_equals_per_problem_calls(mode, dtype):
     start = np.concatenate([[0], np.cumsum(n)[:-1]])
     lengths = (ops != 0).sum(1)
     print("program lengths:", sorted(lengths.tolist()), flush=True)
+    same_kernels = [2, 2, 0, 0, 0] if dtype == np.float32 else None
     for b in range(B):
         d = fused.fused_root(td(Xs[b]), td(ops[b:b + 1]), td(ch[b:b + 1]), td(C[b:b + 1]), td(ds[b][None, :]),
-                             mode=mode, raise_errors=False)
+                             mode=mode, raise_errors=False, config=same_kernels)
         seg = r["y"][start[b]:start[b] + n[b]]
         assert torch.equal(bits(seg), bits(d["y"][0])), b
         assert torch.equal(bits(r["grad_C"][b]), bits(d["grad_C"][0])), b
