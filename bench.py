@@ -252,10 +252,15 @@ def bench_full_cache(dev, steps: int, seed: int = 1, B: int = 65536, N: int = 10
     bwd_direct = lambda: _lib.check_status(lib.sr_backward_mb_f32(  # noqa: E731
         _lib.ptr(gup), _lib.ptr(cache), _lib.ptr(ops_mb), _lib.ptr(ch_mb), M, B, N, 3, _lib.ptr(gC), _lib.ptr(gX),
         _lib.ptr(ws), ws.numel(), _lib.ptr(err), s), "backward")
-    bwd = lambda: _lib.check_status(lib.sr_backward_mb_rows_f32(  # noqa: E731
+    bwd_read = lambda: _lib.check_status(lib.sr_backward_mb_rows_f32(  # noqa: E731
         _lib.ptr(gup), _lib.ptr(cache), _lib.ptr(ops_mb), _lib.ptr(ch_mb), M, B, N, 3, _lib.ptr(gC), _lib.ptr(gX),
         _lib.ptr(ws), ws.numel(), _lib.ptr(rws), rws.numel(), _lib.ptr(err), s), "backward")
+    bwd = lambda: _lib.check_status(lib.sr_backward_mb_rows_recompute_f32(  # noqa: E731
+        _lib.ptr(gup), _lib.ptr(cache), _lib.ptr(X), _lib.ptr(C_mb), _lib.ptr(ops_mb), _lib.ptr(ch_mb), M, B, N, 3,
+        _lib.ptr(gC), _lib.ptr(gX), _lib.ptr(ws), ws.numel(), _lib.ptr(rws), rws.numel(), _lib.ptr(err), s),
+        "backward")
     ms_bd, _ = timed(bwd_direct, steps, 1)
+    ms_br, _ = timed(bwd_read, steps, 1)
     ms_b, _ = timed(bwd, steps, 1)  # the API's path
     del rws
     L = float((ops != 0).sum(1).double().mean())
@@ -271,7 +276,10 @@ def bench_full_cache(dev, steps: int, seed: int = 1, B: int = 65536, N: int = 10
                              ms_without_workspace=round(ms_fd, 3)),
                 backward=dict(ms=round(ms_b, 3), bytes=bb, achieved_GBps=gbs(bb, ms_b),
                               frac=round(gbs(bb, ms_b) / HBM_PEAK_GBS, 4),
-                              note="gathered rows, then the reverse sweep (sr_backward_mb_rows_f32, the API's path)",
+                              note="gathered gradient rows, then the reverse sweep recomputing the forward values "
+                                   "(sr_backward_mb_rows_recompute_f32, the API's path); bytes are the reference's "
+                                   "(upstream gradient and cached values read)",
+                              ms_reading_cached_values=round(ms_br, 3),
                               ms_without_row_workspace=round(ms_bd, 3)))
 
 

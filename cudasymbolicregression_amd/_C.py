@@ -231,10 +231,12 @@ class _SymbolicEvaluation(torch.autograd.Function):
         rws, rws_n = _speed_workspace(lib.sr_backward_mb_rows_workspace_bytes(M, N, B, 1, int(is64)) if grouped else 0,
                                       X.device, cache.numel() * cache.element_size())
         err = torch.empty(1, dtype=torch.int32, device=X.device)
+        # the cache is this function's forward output (saved, unmodified):
+        # the backward recomputes its values from X and C where it gathers rows
         _lib.check_status(
-            getattr(lib, f"sr_backward_mb_rows_{'f64' if is64 else 'f32'}")(
-                _lib.ptr(g), _lib.ptr(cache), _lib.ptr(Ops), _lib.ptr(Ch), M, B, N, n_x, _lib.ptr(grad_C),
-                _lib.ptr(grad_X), _lib.ptr(ws), ws.numel(), _lib.ptr(rws), rws_n,
+            getattr(lib, f"sr_backward_mb_rows_recompute_{'f64' if is64 else 'f32'}")(
+                _lib.ptr(g), _lib.ptr(cache), _lib.ptr(X), _lib.ptr(C), _lib.ptr(Ops), _lib.ptr(Ch), M, B, N, n_x,
+                _lib.ptr(grad_C), _lib.ptr(grad_X), _lib.ptr(ws), ws.numel(), _lib.ptr(rws), rws_n,
                 _lib.ptr(err), _lib.stream_of(X.device),
             ),
             "symbolic backward",
@@ -348,9 +350,9 @@ class _SymbolicEvaluationMultiple(torch.autograd.Function):
                                       X.device, cache.numel() * cache.element_size())
         err = torch.empty(1, dtype=torch.int32, device=X.device)
         _lib.check_status(
-            getattr(lib, f"sr_multi_backward_mb_rows_{'f64' if is64 else 'f32'}")(
-                _lib.ptr(g), _lib.ptr(cache), _lib.ptr(Ops), _lib.ptr(Ch), M, B, N, n_x, K, _lib.ptr(grad_C),
-                _lib.ptr(grad_X), _lib.ptr(ws), ws.numel(), _lib.ptr(rws), rws_n,
+            getattr(lib, f"sr_multi_backward_mb_rows_recompute_{'f64' if is64 else 'f32'}")(
+                _lib.ptr(g), _lib.ptr(cache), _lib.ptr(X), _lib.ptr(C), _lib.ptr(Ops), _lib.ptr(Ch), M, B, N, n_x,
+                K, _lib.ptr(grad_C), _lib.ptr(grad_X), _lib.ptr(ws), ws.numel(), _lib.ptr(rws), rws_n,
                 _lib.ptr(err), _lib.stream_of(X.device),
             ),
             "symbolic multiple backward",

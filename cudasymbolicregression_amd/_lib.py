@@ -66,6 +66,10 @@ def hip() -> ctypes.CDLL:
             _declare(lib, f"sr_multi_backward_mb_rows_{sfx}", I,
                      [P, P, P, P, I64, I64, I64, I64, I64, P, P, P, I64, P, I64, P, P])
             _declare(lib, f"sr_backward_mb_{sfx}", I, [P, P, P, P, I64, I64, I64, I64, P, P, P, I64, P, P])
+            _declare(lib, f"sr_backward_mb_rows_recompute_{sfx}", I,
+                     [P, P, P, P, P, P, I64, I64, I64, I64, P, P, P, I64, P, I64, P, P])
+            _declare(lib, f"sr_multi_backward_mb_rows_recompute_{sfx}", I,
+                     [P, P, P, P, P, P, I64, I64, I64, I64, I64, P, P, P, I64, P, I64, P, P])
             _declare(lib, f"sr_fused_root_{sfx}", I,
                      [P, I64, I64, P, P, P, I64, I64, P, I, I, P, P, P, P, P, P, P, I64, P])
             _declare(lib, f"sr_fused_root_ragged_{sfx}", I,

@@ -17,6 +17,7 @@
 // node-first tensors Ops (M,B), Ch (M,B,2), C (M,B), cache (M,N,B).
 #include "sr_wave.h"
 #include "sr_device_info.h"
+#include "sr_streams.h"
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -271,7 +272,7 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_fwd_rows(const T *__restrict_
             } else {
                 forward_tile_at<T>(p, pc, [&](int k) -> T & { return t[(int64_t)(k < L ? k : L - 1) * N]; }, L);
             }
-        } else {
+        } else if (cache != nullptr) {  // (no cache: the rows only)
             T *t = cache + n * E + e;
             const int64_t rs = N * E;
             forward_tile_at<T>(p, pc, [&](int k) -> T & { return t[(int64_t)(k < L ? k : L - 1) * rs]; }, L);
@@ -459,6 +460,7 @@ __global__ __launch_bounds__(BWD_WAVES * 64) void k_full_backward(
 // copy of the upstream gradient and is only read.  Outputs and reductions
 // as in k_full_backward.
 constexpr int BWD_BATCH = 16;
+constexpr int64_t BWD_CHUNKS = 1;  // pipeline chunks of the gathered backward
 constexpr int BWR_WPE = 8;  // waves per SIMD the FP32 row-tape sweep's registers are sized for (FP64 would spill)
 
 // Backward gather (with a row-buffer workspace).  The grouped backward
@@ -472,19 +474,21 @@ constexpr int BWR_WPE = 8;  // waves per SIMD the FP32 row-tape sweep's register
 // NO_OP padding row (k >= L_e) that is not zero is reported as
 // ERR_GRAD_NO_OP, as the grouped backward does for the groups it reads itself
 // (those without room in the buffer).
+// VALS = false: only the gradient is gathered (the reverse sweep recomputes
+// the forward values), so the cache is not read and vrows not written.
 constexpr int GA_WAVES = 8;
-template <typename T>
+template <typename T, bool VALS>
 __global__ __launch_bounds__(GA_WAVES * 64) void k_bwd_gather(const T *__restrict__ grad_cache,
                                                              const T *__restrict__ cache, int M, int64_t N, int64_t E,
-                                                             int64_t G, FwdRows fr, T *__restrict__ vrows,
-                                                             int32_t *err_flag) {
+                                                             int64_t g_begin, int64_t g_count, FwdRows fr,
+                                                             T *__restrict__ vrows, int32_t *err_flag) {
     __shared__ T gtile[FG][WAVE + 1];
-    __shared__ T vtile[FG][WAVE + 1];
+    __shared__ T vtile[VALS ? FG : 1][VALS ? WAVE + 1 : 1];
     __shared__ int64_t base_s[FG];
     __shared__ int L_s[FG];
     const int wave = uniform_i32((int)(threadIdx.x / WAVE));
     const int lane = lane_id();
-    const int64_t g = (int64_t)blockIdx.x % G, tl = (int64_t)blockIdx.x / G;
+    const int64_t g = g_begin + (int64_t)blockIdx.x % g_count, tl = (int64_t)blockIdx.x / g_count;
     if (fr.goff[g + 1] > fr.cap) return;  // read by the grouped backward itself
     const int64_t e = g * FG + lane, n0 = tl * WAVE;
     if (wave == 0) {
@@ -508,14 +512,14 @@ __global__ __launch_bounds__(GA_WAVES * 64) void k_bwd_gather(const T *__restric
             const bool ok = n < N && e < E;
             const int64_t off = ok ? ((int64_t)k * N + n) * E + e : 0;
             gv[i] = ok ? grad_cache[off] : T(0);
-            vv[i] = (ok && k < myL) ? cache[off] : T(0);
+            if constexpr (VALS) vv[i] = (ok && k < myL) ? cache[off] : T(0);
         }
 #pragma unroll
         for (int i = 0; i < PW; ++i) {
             const int pp = wave + i * GA_WAVES;
             if (k < myL) {
                 gtile[lane][pp] = gv[i];
-                vtile[lane][pp] = vv[i];
+                if constexpr (VALS) vtile[lane][pp] = vv[i];
             } else if (gv[i] != T(0)) {
                 lerr = ERR_GRAD_NO_OP;
             }
@@ -527,7 +531,7 @@ __global__ __launch_bounds__(GA_WAVES * 64) void k_bwd_gather(const T *__restric
                 if (k < L_s[q] && n < N) {
                     const int64_t o = base_s[q] + (int64_t)k * N + n;
                     grows[o] = gtile[q][lane];
-                    vrows[o] = vtile[q][lane];
+                    if constexpr (VALS) vrows[o] = vtile[q][lane];
                 }
             }
             __syncthreads();
@@ -549,7 +553,7 @@ k_full_backward_grp(
     const T *__restrict__ grad_cache, const T *__restrict__ cache, const int64_t *__restrict__ ops,
     const int64_t *__restrict__ ch, int M, int64_t B, int64_t K, int64_t N, int n_x, T *__restrict__ grad_C,
     T *__restrict__ gx_partial, int32_t *err_flag, FwdRows fr,
-    const T *__restrict__ vrows) {
+    const T *__restrict__ vrows, int64_t blk_begin) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int Ls[BWD_WAVES];
     const int wave = uniform_i32((int)(threadIdx.x / WAVE));
@@ -563,7 +567,7 @@ k_full_backward_grp(
                                    (size_t)BWD_WAVES * M);  // [4][64][NXREG]
     const int64_t E = B * K;
     // XCD-remapped so that neighbouring expression groups share one XCD's L2
-    const int64_t blk = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
+    const int64_t blk = blk_begin + xcd_remap(blockIdx.x, (int64_t)gridDim.x);
     const int64_t e0 = blk * BWD_WAVES;
     // the block's expressions lie in one row-buffer group (BWD_WAVES divides FG)
     const int64_t grp = e0 / FG;
@@ -776,11 +780,17 @@ int64_t backward_mb_ws_bytes(int64_t B, int64_t N, int64_t n_x, int64_t K = 1) {
     return part + chunks * N * n_x * 8 + 256;
 }
 
+// X and C (both or neither): the gathered groups recompute their forward
+// values from them instead of reading the cache (which must then be their
+// forward output); the other groups still read the cache.
 template <typename T>
 int backward_mb(T *grad_out, const T *cache, const int64_t *ops, const int64_t *ch, int64_t M, int64_t B,
                 int64_t K, int64_t N, int64_t n_x, T *grad_C, T *grad_X, void *workspace, int64_t ws_bytes,
-                int32_t *d_err, hipStream_t s, void *rows_ws = nullptr, int64_t rows_bytes = 0) {
+                int32_t *d_err, hipStream_t s, void *rows_ws = nullptr, int64_t rows_bytes = 0,
+                const T *X = nullptr, const T *C = nullptr) {
     if (M <= 0 || M > 128 || N < 0 || B < 0 || n_x < 0 || K <= 0 || rows_bytes < 0) return -1;
+    if ((X == nullptr) != (C == nullptr)) return -1;
+    const bool recomp = X != nullptr;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, d_err, 0, (int64_t)1);
     int64_t nblk = (B * K + BWD_WAVES - 1) / BWD_WAVES;
     if (grad_X) {
@@ -818,6 +828,7 @@ int backward_mb(T *grad_out, const T *cache, const int64_t *ops, const int64_t *
         const int64_t n_tiles = (N + TILE - 1) / TILE;
         FwdRows fr{nullptr, nullptr, nullptr, nullptr, -1};
         T *vrows = nullptr;
+        const bool manyx = n_x > NXREG;
         if (rows_ws != nullptr && rows_bytes >= ib + 512 && G * n_tiles <= 0x7fffffffLL) {
             unsigned char *w = static_cast<unsigned char *>(rows_ws);
             const int64_t o1 = align256(8 * (G + 1)), o2 = o1 + align256(8 * E);
@@ -832,19 +843,52 @@ int backward_mb(T *grad_out, const T *cache, const int64_t *ops, const int64_t *
                                const_cast<int64_t *>(fr.goff), const_cast<int64_t *>(fr.eoff),
                                const_cast<uint8_t *>(fr.Lw));
             hipLaunchKernelGGL(k_fwd_scan, dim3(1), dim3(1024), 0, s, const_cast<int64_t *>(fr.goff), G);
-            hipLaunchKernelGGL(k_bwd_gather<T>, dim3((unsigned)(G * n_tiles)), dim3(GA_WAVES * 64), 0, s, grad_out,
-                               cache, (int)M, N, E, G, fr, vrows, d_err);
+            // The gather (and, recomputing, the forward's row pass that
+            // writes the value rows) on two side streams at once, then the
+            // reverse sweep on the caller's stream, chunk by chunk of groups.
+            // One chunk is fastest at 65536 x 1024 (10.8 ms): with 8 chunks,
+            // the sweep of one chunk under the gather of the next, the kernels
+            // slowed each other down (14.3 ms).
+            const bool rec_ok = recomp && (E * ((n_tiles + FR_TILES - 1) / FR_TILES) + FR_WAVES - 1) / FR_WAVES <=
+                                              0x7fffffffLL;
+            const int64_t nch = G < BWD_CHUNKS ? G : BWD_CHUNKS;
+            const int64_t gpc = (G + nch - 1) / nch;      // groups per chunk
+            constexpr int64_t BPG = FG / BWD_WAVES;       // sweep blocks per group
+            ForkScope fk(s);
+            int r = fk.fork();
+            if (r) return fk.finish(r);
+            for (int64_t g0 = 0; g0 < G; g0 += gpc) {
+                const int64_t gc = g0 + gpc < G ? gpc : G - g0;
+                if (rec_ok) {
+                    hipLaunchKernelGGL((k_bwd_gather<T, false>), dim3((unsigned)(gc * n_tiles)), dim3(GA_WAVES * 64),
+                                       0, fk.side(0), grad_out, cache, (int)M, N, E, g0, gc, fr, vrows, d_err);
+                    FwdRows frv = fr;
+                    frv.rows = vrows;
+                    const int64_t e0 = g0 * FG, ec = (g0 + gc) * FG < E ? gc * FG : E - e0;
+                    const int64_t nrb = (ec * ((n_tiles + FR_TILES - 1) / FR_TILES) + FR_WAVES - 1) / FR_WAVES;
+                    hipLaunchKernelGGL(k_fwd_rows<T>, dim3((unsigned)nrb), dim3(FR_WAVES * 64), 0, fk.side(1), X, N,
+                                       (int)n_x, ops, ch, C, (int)M, B, K, n_tiles, e0, ec, frv, (T *)nullptr);
+                } else {
+                    hipLaunchKernelGGL((k_bwd_gather<T, true>), dim3((unsigned)(gc * n_tiles)), dim3(GA_WAVES * 64),
+                                       0, fk.side(0), grad_out, cache, (int)M, N, E, g0, gc, fr, vrows, d_err);
+                }
+                if ((r = (int)hipGetLastError())) return fk.finish(r);
+                if ((r = fk.handoff(0))) return fk.finish(r);
+                if (rec_ok && (r = fk.handoff(1))) return fk.finish(r);
+                const int64_t b0 = g0 * BPG, b1 = (g0 + gc) * BPG < nblk ? (g0 + gc) * BPG : nblk;
+                hipLaunchKernelGGL(manyx ? (k_full_backward_grp<T, true, true>) : (k_full_backward_grp<T, true, false>),
+                                   dim3((unsigned)(b1 - b0)), dim3(BWD_WAVES * 64), backward_grp_lds<T>(M, true), s,
+                                   grad_out, cache, ops, ch, (int)M, B, K, N, (int)n_x, grad_C, gx_partial, d_err, fr,
+                                   (const T *)vrows, b0);
+                if ((r = (int)hipGetLastError())) return fk.finish(r);
+            }
+            if ((r = fk.finish(0))) return r;
         }
-        const bool manyx = n_x > NXREG;
-        if (fr.cap >= 0)  // the gathered groups
-            hipLaunchKernelGGL(manyx ? (k_full_backward_grp<T, true, true>) : (k_full_backward_grp<T, true, false>),
-                               dim3((unsigned)nblk), dim3(BWD_WAVES * 64), backward_grp_lds<T>(M, true), s, grad_out,
-                               cache, ops, ch, (int)M, B, K, N, (int)n_x, grad_C, gx_partial, d_err, fr,
-                               (const T *)vrows);
         // the others (all of them without a row workspace)
         hipLaunchKernelGGL(manyx ? (k_full_backward_grp<T, false, true>) : (k_full_backward_grp<T, false, false>),
                            dim3((unsigned)nblk), dim3(BWD_WAVES * 64), backward_grp_lds<T>(M), s, grad_out, cache,
-                           ops, ch, (int)M, B, K, N, (int)n_x, grad_C, gx_partial, d_err, fr, (const T *)vrows);
+                           ops, ch, (int)M, B, K, N, (int)n_x, grad_C, gx_partial, d_err, fr, (const T *)vrows,
+                           (int64_t)0);
     } else {
         size_t lds = (size_t)BWD_WAVES * M * sizeof(double) + (size_t)BWD_WAVES * WAVE * NXREG * sizeof(T);
         if (lds > lds_limit()) return -3;
@@ -953,6 +997,25 @@ int sr_backward_mb_rows_f64(double *grad_out, const double *cache, const int64_t
                                (hipStream_t)stream, rows_ws, rows_bytes);
 }
 
+// The gathered groups recompute their forward values from X and C (the
+// inputs of the forward that wrote `cache`) instead of reading the cache.
+int sr_backward_mb_rows_recompute_f32(float *grad_out, const float *cache, const float *X, const float *C,
+                                      const int64_t *ops, const int64_t *ch, int64_t M, int64_t B, int64_t N,
+                                      int64_t n_x, float *grad_C, float *grad_X, void *workspace, int64_t ws_bytes,
+                                      void *rows_ws, int64_t rows_bytes, int32_t *d_err, void *stream) {
+    if (X == nullptr || C == nullptr) return -1;
+    return backward_mb<float>(grad_out, cache, ops, ch, M, B, 1, N, n_x, grad_C, grad_X, workspace, ws_bytes, d_err,
+                              (hipStream_t)stream, rows_ws, rows_bytes, X, C);
+}
+int sr_backward_mb_rows_recompute_f64(double *grad_out, const double *cache, const double *X, const double *C,
+                                      const int64_t *ops, const int64_t *ch, int64_t M, int64_t B, int64_t N,
+                                      int64_t n_x, double *grad_C, double *grad_X, void *workspace, int64_t ws_bytes,
+                                      void *rows_ws, int64_t rows_bytes, int32_t *d_err, void *stream) {
+    if (X == nullptr || C == nullptr) return -1;
+    return backward_mb<double>(grad_out, cache, ops, ch, M, B, 1, N, n_x, grad_C, grad_X, workspace, ws_bytes, d_err,
+                               (hipStream_t)stream, rows_ws, rows_bytes, X, C);
+}
+
 int sr_multi_forward_mb_f32(const float *X, int64_t N, int64_t n_x, const int64_t *ops, const int64_t *ch,
                             const float *C, int64_t M, int64_t B, int64_t K, float *cache, void *stream) {
     return forward_mb<float>(X, N, n_x, ops, ch, C, M, B, K, cache, nullptr, 0, (hipStream_t)stream);
@@ -999,6 +1062,24 @@ int sr_multi_backward_mb_rows_f64(double *grad_out, const double *cache, const i
                                   int64_t rows_bytes, int32_t *d_err, void *stream) {
     return backward_mb<double>(grad_out, cache, ops, ch, M, B, K, N, n_x, grad_C, grad_X, workspace, ws_bytes, d_err,
                                (hipStream_t)stream, rows_ws, rows_bytes);
+}
+int sr_multi_backward_mb_rows_recompute_f32(float *grad_out, const float *cache, const float *X, const float *C,
+                                            const int64_t *ops, const int64_t *ch, int64_t M, int64_t B, int64_t N,
+                                            int64_t n_x, int64_t K, float *grad_C, float *grad_X, void *workspace,
+                                            int64_t ws_bytes, void *rows_ws, int64_t rows_bytes, int32_t *d_err,
+                                            void *stream) {
+    if (X == nullptr || C == nullptr) return -1;
+    return backward_mb<float>(grad_out, cache, ops, ch, M, B, K, N, n_x, grad_C, grad_X, workspace, ws_bytes, d_err,
+                              (hipStream_t)stream, rows_ws, rows_bytes, X, C);
+}
+int sr_multi_backward_mb_rows_recompute_f64(double *grad_out, const double *cache, const double *X, const double *C,
+                                            const int64_t *ops, const int64_t *ch, int64_t M, int64_t B, int64_t N,
+                                            int64_t n_x, int64_t K, double *grad_C, double *grad_X, void *workspace,
+                                            int64_t ws_bytes, void *rows_ws, int64_t rows_bytes, int32_t *d_err,
+                                            void *stream) {
+    if (X == nullptr || C == nullptr) return -1;
+    return backward_mb<double>(grad_out, cache, ops, ch, M, B, K, N, n_x, grad_C, grad_X, workspace, ws_bytes, d_err,
+                               (hipStream_t)stream, rows_ws, rows_bytes, X, C);
 }
 
 }  // extern "C"

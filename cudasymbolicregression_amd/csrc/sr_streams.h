@@ -76,6 +76,14 @@ class ForkScope {
     }
     hipStream_t side(int i) const { return f_->side[i]; }
 
+    // The caller's stream waits for the work queued so far on side stream i
+    // (its join event is recorded now; finish records it again at the end).
+    int handoff(int i) {
+        int r = (int)hipEventRecord(f_->ev_join[i], f_->side[i]);
+        if (!r) r = (int)hipStreamWaitEvent(s_, f_->ev_join[i], 0);
+        return r;
+    }
+
     int finish(int r) {
         if (!forked_) return r;
         forked_ = false;

@@ -101,6 +101,21 @@ int sr_backward_mb_rows_f64(double *grad_cache, const double *cache, const int64
                             int64_t M, int64_t B, int64_t N, int64_t n_x, double *grad_C, double *grad_X,
                             void *workspace, int64_t ws_bytes, void *rows_ws, int64_t rows_bytes, int32_t *d_err,
                             void *stream);
+/* sr_backward_mb_rows_* for a cache that is the output of the forward on X
+ * (N,n_x) and C (M,B): the gathered groups recompute their forward values
+ * from X and C, with the forward's own per-node arithmetic (the same bits),
+ * instead of reading them from the cache, so only grad_cache is gathered;
+ * groups without room in the rows still read the cache.  Same results as
+ * sr_backward_mb_rows_* on that cache.  (The reference's autograd backward,
+ * src/symbolic_evaluation.cpp:94-168, receives the cache its forward saved.) */
+int sr_backward_mb_rows_recompute_f32(float *grad_cache, const float *cache, const float *X, const float *C,
+                                      const int64_t *ops, const int64_t *ch, int64_t M, int64_t B, int64_t N,
+                                      int64_t n_x, float *grad_C, float *grad_X, void *workspace, int64_t ws_bytes,
+                                      void *rows_ws, int64_t rows_bytes, int32_t *d_err, void *stream);
+int sr_backward_mb_rows_recompute_f64(double *grad_cache, const double *cache, const double *X, const double *C,
+                                      const int64_t *ops, const int64_t *ch, int64_t M, int64_t B, int64_t N,
+                                      int64_t n_x, double *grad_C, double *grad_X, void *workspace, int64_t ws_bytes,
+                                      void *rows_ws, int64_t rows_bytes, int32_t *d_err, void *stream);
 
 /* K constant sets per expression: C (M,B,K), cache (M,N,B,K).  Replaces
  * evaluate_multiple_constant_backend and its kernels
@@ -134,6 +149,17 @@ int sr_multi_backward_mb_rows_f64(double *grad_cache, const double *cache, const
                                   int64_t M, int64_t B, int64_t N, int64_t n_x, int64_t K, double *grad_C,
                                   double *grad_X, void *workspace, int64_t ws_bytes, void *rows_ws,
                                   int64_t rows_bytes, int32_t *d_err, void *stream);
+/* As sr_backward_mb_rows_recompute_*, with C (M,B,K). */
+int sr_multi_backward_mb_rows_recompute_f32(float *grad_cache, const float *cache, const float *X, const float *C,
+                                            const int64_t *ops, const int64_t *ch, int64_t M, int64_t B, int64_t N,
+                                            int64_t n_x, int64_t K, float *grad_C, float *grad_X, void *workspace,
+                                            int64_t ws_bytes, void *rows_ws, int64_t rows_bytes, int32_t *d_err,
+                                            void *stream);
+int sr_multi_backward_mb_rows_recompute_f64(double *grad_cache, const double *cache, const double *X,
+                                            const double *C, const int64_t *ops, const int64_t *ch, int64_t M,
+                                            int64_t B, int64_t N, int64_t n_x, int64_t K, double *grad_C,
+                                            double *grad_X, void *workspace, int64_t ws_bytes, void *rows_ws,
+                                            int64_t rows_bytes, int32_t *d_err, void *stream);
 
 /* Fused root-only forward + backward over batch-first layouts.  Computes
  * what optimize's closure needs (symbolic_torch/optimization.py:63-101:

@@ -123,9 +123,11 @@ def test_row_buffer_forward_twice_in_a_row():
     assert np.array_equal(b, _written(lib, X2, ops2, ch2, C2, None), equal_nan=True)
 
 
-def _backward(lib, ops, ch, cache, g, n_x, K, rows_bytes, is64):
+def _backward(lib, ops, ch, cache, g, n_x, K, rows_bytes, is64, recompute=None):
     """(grad_C, grad_X, err) of the multi-constant backward, direct
-    (rows_bytes None) or through a row workspace of rows_bytes bytes."""
+    (rows_bytes None) or through a row workspace of rows_bytes bytes; with
+    recompute = (X, C), the entry point that recomputes the gathered groups'
+    forward values from the forward's inputs instead of reading the cache."""
     from cudasymbolicregression_amd import _lib
 
     M, N, B = cache.shape[0], cache.shape[1], cache.shape[2]
@@ -142,11 +144,17 @@ def _backward(lib, ops, ch, cache, g, n_x, K, rows_bytes, is64):
         rc = getattr(lib, f"sr_multi_backward_mb_{sfx}")(
             _lib.ptr(gg), _lib.ptr(cache), _lib.ptr(ops), _lib.ptr(ch), M, B, N, n_x, K, _lib.ptr(gC), _lib.ptr(gX),
             _lib.ptr(ws), ws.numel(), _lib.ptr(err), s)
-    else:
+    elif recompute is None:
         rws = torch.empty(rows_bytes, dtype=torch.uint8, device=dev) if rows_bytes > 0 else None  # 0: NULL
         rc = getattr(lib, f"sr_multi_backward_mb_rows_{sfx}")(
             _lib.ptr(gg), _lib.ptr(cache), _lib.ptr(ops), _lib.ptr(ch), M, B, N, n_x, K, _lib.ptr(gC), _lib.ptr(gX),
             _lib.ptr(ws), ws.numel(), _lib.ptr(rws), rows_bytes, _lib.ptr(err), s)
+    else:
+        X, C = recompute
+        rws = torch.empty(rows_bytes, dtype=torch.uint8, device=dev) if rows_bytes > 0 else None  # 0: NULL
+        rc = getattr(lib, f"sr_multi_backward_mb_rows_recompute_{sfx}")(
+            _lib.ptr(gg), _lib.ptr(cache), _lib.ptr(X), _lib.ptr(C), _lib.ptr(ops), _lib.ptr(ch), M, B, N, n_x, K,
+            _lib.ptr(gC), _lib.ptr(gX), _lib.ptr(ws), ws.numel(), _lib.ptr(rws), rows_bytes, _lib.ptr(err), s)
     _lib.check_status(rc, "backward")
     torch.cuda.synchronize()
     return gC.cpu().numpy(), gX.cpu().numpy(), int(err.item())
@@ -157,8 +165,9 @@ def _backward(lib, ops, ch, cache, g, n_x, K, rows_bytes, is64):
 def test_row_buffer_backward_equals_direct(dtype, M, K):
     """Gathered rows against the direct backward: grad_C, grad_X and the
     error code identical bit for bit, with the upstream gradient on every
-    live row; then with only a nonzero gradient on one NO_OP padding row,
-    which both report as 201."""
+    live row, also when the gathered groups recompute their forward values
+    from X and C instead of reading the cache; then with only a nonzero
+    gradient on one NO_OP padding row, which all report as 201."""
     from cudasymbolicregression_amd import _lib
 
     lib = _lib.hip()
@@ -180,10 +189,12 @@ def test_row_buffer_backward_equals_direct(dtype, M, K):
     sizes = (rec, _index_bytes(E) + 512 + some, _index_bytes(E) + 512, 0)
     ref = _backward(lib, ops, ch, cache, g, 3, K, None, is64)
     for rb in sizes:
-        got = _backward(lib, ops, ch, cache, g, 3, K, rb, is64)
-        same = all(np.array_equal(a, b, equal_nan=True) for a, b in zip(got[:2], ref[:2])) and got[2] == ref[2]
-        print(f"  rows workspace {rb} B: identical={same} err={got[2]}", flush=True)
-        assert same
+        for rec_src in (None, (X, C)):
+            got = _backward(lib, ops, ch, cache, g, 3, K, rb, is64, rec_src)
+            same = all(np.array_equal(a, b, equal_nan=True) for a, b in zip(got[:2], ref[:2])) and got[2] == ref[2]
+            print(f"  rows workspace {rb} B, recompute={rec_src is not None}: identical={same} err={got[2]}",
+                  flush=True)
+            assert same
     # only a gradient on a padding row of expression 3 (in the first group,
     # which fits in the recommended and the partial workspaces), on programs
     # without division: the backward reports the largest code, and a zero
@@ -199,9 +210,11 @@ def test_row_buffer_backward_equals_direct(dtype, M, K):
     ref2 = _backward(lib, ops, ch, cache, g2, 3, K, None, is64)
     assert ref2[2] == 201
     for rb in sizes:
-        got2 = _backward(lib, ops, ch, cache, g2, 3, K, rb, is64)
-        print(f"  NO_OP gradient, rows workspace {rb} B: err={got2[2]}", flush=True)
-        assert got2[2] == 201
+        for rec_src in (None, (X, C)):
+            got2 = _backward(lib, ops, ch, cache, g2, 3, K, rb, is64, rec_src)
+            print(f"  NO_OP gradient, rows workspace {rb} B, recompute={rec_src is not None}: err={got2[2]}",
+                  flush=True)
+            assert got2[2] == 201
 
 
 def test_api_row_workspaces_match_direct(monkeypatch):
