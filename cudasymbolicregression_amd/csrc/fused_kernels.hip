@@ -99,6 +99,7 @@ struct FArgs {
     const float *xt;        // X transposed: variable v's N values at xt + v * n_pad, zero-padded
     int64_t n_pad;          // N rounded up to a multiple of WIDE_PAD
     int vec;                // dy and y may be accessed 16 bytes at a time (N % 4 == 0, aligned)
+    double *gx_tot;         // grad_X: per-program running FP64 sums, WIDE_TOTW per program
 };
 
 // LT: most instructions of an expression in the bucket; SL: slot registers
@@ -125,17 +126,6 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
                       : reinterpret_cast<T *>(smem + gx_bytes) + (size_t)wave * f_tape_elems(LT, PPT);
         t.v = slot;
     }
-    // this wave's grad_X partial row, variable-major (element (n, v) at
-    // v * N + n, so a wave's 64 points are 64 consecutive words); each element
-    // is only ever touched by the lane that owns its point (n = lane mod 64),
-    // zeroing included
-    T *gxrow = nullptr;
-    if (want_gx) {
-        gxrow = gx_lds ? reinterpret_cast<T *>(smem) + (size_t)wave * gx_len
-                      : (T *)a.gx_partial + (a.gx_row0 + (int64_t)blockIdx.x * WPB + wave) * gx_len;
-        for (int v = 0; v < n_x; ++v)
-            for (int64_t nz = lane; nz < N; nz += WAVE) gxrow[(int64_t)v * N + nz] = T(0);
-    }
     const int64_t count = (int64_t)*a.list_count;
     const T *const X0 = X;
     // work: a fixed segment of the list (grad_X wanted), else chunks of CH
@@ -143,6 +133,18 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_code(FArgs a) {
     const int64_t gw = (int64_t)blockIdx.x * WPB + wave;
     int32_t seg_i = a.seg ? uniform_i32(a.seg[gw]) : 0;
     const int32_t seg_e = a.seg ? uniform_i32(a.seg[gw + 1]) : 0;
+    // this wave's grad_X partial row, variable-major (element (n, v) at
+    // v * N + n, so a wave's 64 points are 64 consecutive words); each element
+    // is only ever touched by the lane that owns its point (n = lane mod 64),
+    // zeroing included.  A row in the workspace whose segment is empty is
+    // left alone: the row sums skip it.
+    T *gxrow = nullptr;
+    if (want_gx && (gx_lds || seg_i < seg_e)) {
+        gxrow = gx_lds ? reinterpret_cast<T *>(smem) + (size_t)wave * gx_len
+                      : (T *)a.gx_partial + (a.gx_row0 + (int64_t)blockIdx.x * WPB + wave) * gx_len;
+        for (int v = 0; v < n_x; ++v)
+            for (int64_t nz = lane; nz < N; nz += WAVE) gxrow[(int64_t)v * N + nz] = T(0);
+    }
     constexpr int CH = fetch_chunk(LT);
     int32_t chunk = 0, left = 0;
     for (;;) {
@@ -535,6 +537,7 @@ constexpr int PREC_WORDS = 80;
 constexpr int PLAN_MAX_ROWS = 16;
 constexpr int PLAN_MAX_NC = 16;
 constexpr int WIDE_PAD = 1024;  // n_pad granularity: a multiple of every tile group (64 x PPT, PPT <= 16)
+constexpr int64_t WIDE_CHUNKS_PER_WAVE = 2;  // grad_X list chunks (and partial rows) per wave
 
 // One wave per program.  Programs that fail validation get their code
 // (flags[0], err_b) and no bucket (-1).  With `plan`, programs of at most
@@ -648,12 +651,28 @@ __host__ __device__ inline size_t wide_xblk_bytes(int n_x, int64_t n_pad) {
     return wide_xblk(n_x, n_pad) ? (size_t)n_x * n_pad * sizeof(float) : 0;
 }
 
+// Running FP64 sums of a program across the tile groups of a grad_X call
+// (k_fused_wide, group-major): its nc constants, then its loss, then its
+// error code, in a workspace of WIDE_TOTW doubles per program (nc <= 16).
+constexpr int WIDE_TOTW = 18;
+
 // LT: tape rows of the bucket (8 or 16); PPT: points per lane per dispatch
 // (a multiple of 4); WPB: waves per block.  Same per-point arithmetic and
 // reference rules as k_fused_code (fwd_ops, bwd_masked, the scatter modes),
 // so y and the error codes are the same bit for bit; the FP64 sums (loss,
 // grad_C) and the grad_X partial rows add the same terms in another order.
 // Variable adjoints of a tile group accumulate in registers (PM_VAR).
+//
+// Without grad_X a wave evaluates one program over all its tile groups at a
+// time.  With grad_X the list is cut into gx_rows chunks of consecutive
+// positions; a wave takes a chunk from the shared counter and runs it group
+// by group: every program of the chunk on tile group 0, then on group 1, and
+// so on, so the chunk's variable adjoints of a group are summed in registers
+// (program by program, in list order) and written once to the chunk's own
+// partial row, while each program carries its FP64 sums from group to group
+// in the workspace (a program's sums do not depend on its chunk).  Which
+// wave runs a chunk does not change its row, so grad_X is the same on every
+// call while the chunks balance the load.
 template <int LT, int PPT, int WPB>
 __global__ __launch_bounds__(WPB * 64) void k_fused_wide(FArgs a) {
     static_assert(PPT % 4 == 0, "points per lane come in quads");
@@ -675,13 +694,14 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_wide(FArgs a) {
     float *const grad_C = (float *)a.grad_C;
     const int64_t gx_len = N * n_x;
     const int64_t NP4 = NP / 4;
+    const int64_t GRP = (int64_t)WAVE * PPT;
     if (xblk) {  // the block's copy of X
         const float4 *src = reinterpret_cast<const float4 *>(a.xt);
         for (int64_t z = threadIdx.x; z < n_x * NP4; z += WPB * WAVE) xb4[z] = src[(z / NP4) * NP4 + z % NP4];
         __syncthreads();
     }
     // this wave's variables of one tile group in rows LT + v (no block copy)
-    const bool one_group = N <= (int64_t)WAVE * PPT;
+    const bool one_group = N <= GRP;
     auto put_var_rows = [&](int64_t base, int mask) {
         for (int v = 0; v < n_x; ++v) {
             if (!((mask >> v) & 1)) continue;
@@ -691,268 +711,199 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_wide(FArgs a) {
         }
     };
     if (!xblk && one_group) put_var_rows(0, (1 << n_x) - 1);
-    const int64_t count = (int64_t)*a.list_count;
-    // Work: with grad_X, the list is cut into gx_rows chunks of kc
-    // consecutive positions (trailing chunks may be empty); a wave takes a
-    // chunk from the shared counter and sums its programs' variable
-    // adjoints into the chunk's own partial row, variable-major (v * N + n),
-    // in list order.  Which wave runs a chunk does not change its row, so
-    // grad_X is the same on every call while the chunks balance the load.
-    // Without grad_X, waves take CH positions at a time.
-    const int64_t kc = want_gx ? (count + a.gx_rows - 1) / a.gx_rows : 0;
-    constexpr int CH = fetch_chunk(LT);
-    int64_t pos = 0, pos_end = 0;
-    float *gxrow = nullptr;
     const bool need_dy = a.mode == 1 || !a.fwd_only;
     const float inv_n = 1.0f / (float)N;
-    for (;;) {
-        if (pos >= pos_end) {
-            if (want_gx) {
-                const int64_t c = wave_fetch_add(a.work_counter);
-                if (kc == 0 || c * kc >= count) break;
-                pos = c * kc;
-                pos_end = pos + kc < count ? pos + kc : count;
-                gxrow = (float *)a.gx_partial + (a.gx_row0 + c) * gx_len;
-                if (vec) {
-                    for (int64_t z = lane; z < gx_len / 4; z += WAVE) reinterpret_cast<float4 *>(gxrow)[z] = f4_zero();
-                } else {
-                    for (int64_t z = lane; z < gx_len; z += WAVE) gxrow[z] = 0.0f;
-                }
-            } else {
-                const int64_t c = wave_fetch_add_n(a.work_counter, CH);
-                if (c >= count) break;
-                pos = c;
-                pos_end = c + CH < count ? c + CH : count;
-            }
-        }
-        const int64_t i = pos++;
-        const int64_t b = (int64_t)a.list[i];
-        const uint32_t *rec = a.plan + (size_t)b * PREC_WORDS;
-        const uint32_t rw = rec[lane];
-        const uint32_t hw = rec[64 + (lane & 15)];
+
+    // ---- one program on one tile group.  In: its plan words (rw, hw).  Out
+    // (added to): lg per-lane FP64 loss terms, cg[j] per-lane FP64 sums of
+    // the constants j < NCW used once, gcs lane j FP64 wave sums of the
+    // other constant adjoints, e the error code, gxr the variable adjoints
+    // of this group's points.
+    auto eval_group = [&](int64_t b, uint32_t rw, uint32_t hw, int64_t base, double &lg, double (&cg)[NCW],
+                          double &gcs, int32_t &e, float (&gxr)[NXR_ROWS][PPT]) {
         const uint32_t h0 = rl_u32(hw, 4);
         const int NI = (int)(h0 & 255u), nextra = (int)((h0 >> 8) & 255u), nc = (int)((h0 >> 16) & 255u);
         const int vmask = (int)(h0 >> 24);
-        const uint64_t cm0 = (uint64_t)rl_u32(hw, 5) | ((uint64_t)rl_u32(hw, 6) << 32);
-        const uint64_t cm1 = (uint64_t)rl_u32(hw, 7) | ((uint64_t)rl_u32(hw, 8) << 32);
         auto crow = [&](int j) -> int { return (int)((rl_u32(hw, j >> 2) >> (8 * (j & 3))) & 255u); };
         const int64_t yo = b * N;
-        double lacc = 0.0;
-        double cacc[NCW];
-#pragma unroll
-        for (int q = 0; q < NCW; ++q) cacc[q] = 0.0;
-        double gc = 0.0;  // lane j: FP64 gradient of constant j (nc <= 16)
-        int32_t e = 0;
-        for (int64_t base = 0; base < N; base += (int64_t)WAVE * PPT) {
-            // row r of this tile group: the wave's tape, or the block's X
-            auto row4 = [&](int r) -> float4 * {
-                return (xblk && r >= LT) ? xb4 + (int64_t)(r - LT) * NP4 + base / 4 : tape + (size_t)r * Q * WAVE;
-            };
-            // operands of instruction ii (a unary instruction reads no second row)
-            auto fetch = [&](uint32_t f, int ii, float (&a0)[PPT], float (&a1)[PPT]) {
-                const bool bin = pf_bin(f);
-                const bool c0 = pf_sc(f, 0), c1 = pf_sc(f, 1) || !bin;
-                const float4 *p0 = row4(pf_row(f, 0)), *p1 = row4(pf_row(f, 1));
-                const float s0 = __uint_as_float(rl_u32(rw, 32 + ii));
-                const float s1 = bin ? __uint_as_float(rl_u32(rw, 48 + ii)) : 0.0f;
-                float4 t0[Q], t1[Q];
-#pragma unroll
-                for (int q = 0; q < Q; ++q) {
-                    t0[q] = c0 ? f4_zero() : p0[q * WAVE + lane];
-                    t1[q] = c1 ? f4_zero() : p1[q * WAVE + lane];
-                }
-#pragma unroll
-                for (int q = 0; q < Q; ++q)
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        a0[4 * q + k] = c0 ? s0 : f4_get(t0[q], k);
-                        a1[4 * q + k] = c1 ? s1 : f4_get(t1[q], k);
-                    }
-            };
-            bool act[PPT];
-            int64_t n0[Q];
+        // row r of this tile group: the wave's tape, or the block's X
+        auto row4 = [&](int r) -> float4 * {
+            return (xblk && r >= LT) ? xb4 + (int64_t)(r - LT) * NP4 + base / 4 : tape + (size_t)r * Q * WAVE;
+        };
+        // operands of instruction ii (a unary instruction reads no second row)
+        auto fetch = [&](uint32_t f, int ii, float (&a0)[PPT], float (&a1)[PPT]) {
+            const bool bin = pf_bin(f);
+            const bool c0 = pf_sc(f, 0), c1 = pf_sc(f, 1) || !bin;
+            const float4 *p0 = row4(pf_row(f, 0)), *p1 = row4(pf_row(f, 1));
+            const float s0 = __uint_as_float(rl_u32(rw, 32 + ii));
+            const float s1 = bin ? __uint_as_float(rl_u32(rw, 48 + ii)) : 0.0f;
+            float4 t0[Q], t1[Q];
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
-                n0[q] = base + (int64_t)q * (WAVE * 4) + (int64_t)lane * 4;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) act[4 * q + k] = n0[q] + k < N;
+                t0[q] = c0 ? f4_zero() : p0[q * WAVE + lane];
+                t1[q] = c1 ? f4_zero() : p1[q * WAVE + lane];
             }
-            // upstream values (MODE 0) or targets (MODE 1), and this group's
-            // grad_X partial values of the used variables: loaded now, used
-            // after the forward pass and the reverse sweep
-            float4 dq[Q];
 #pragma unroll
-            for (int q = 0; q < Q; ++q) dq[q] = need_dy ? load_quad(dy + yo, n0[q], N, vec) : f4_zero();
-            float4 gxp[NXR_ROWS][Q];
-            float gxr[NXR_ROWS][PPT];
+            for (int q = 0; q < Q; ++q)
 #pragma unroll
-            for (int v = 0; v < NXR_ROWS; ++v) {
-#pragma unroll
-                for (int q = 0; q < Q; ++q)
-                    gxp[v][q] = (want_gx && v < n_x && ((vmask >> v) & 1)) ? load_quad(gxrow + (int64_t)v * N, n0[q], N, vec)
-                                                                          : f4_zero();
-#pragma unroll
-                for (int k = 0; k < PPT; ++k) gxr[v][k] = 0.0f;
-            }
-            if (!xblk && !one_group) put_var_rows(base, vmask);
-            for (int r = NI; r < NI + nextra; ++r) {
-                float4 *pr = row4(r);
-#pragma unroll
-                for (int q = 0; q < Q; ++q) pr[q * WAVE + lane] = f4_zero();
-            }
-            // ---- forward
-            for (int ii = 0; ii < NI; ++ii) {
-                const uint32_t f = rl_u32(rw, ii);
-                const int op = pf_op(f);
-                float a0[PPT], a1[PPT], r[PPT];
-                fetch(f, ii, a0, a1);
-                if (op == OP_COPY) {
-#pragma unroll
-                    for (int k = 0; k < PPT; ++k) r[k] = a0[k];
-                } else {
-                    fwd_ops<float, PPT>(op, a0, a1, r);
+                for (int k = 0; k < 4; ++k) {
+                    a0[4 * q + k] = c0 ? s0 : f4_get(t0[q], k);
+                    a1[4 * q + k] = c1 ? s1 : f4_get(t1[q], k);
                 }
-                float4 *pw = row4(ii);
+        };
+        bool act[PPT];
+        int64_t n0[Q];
 #pragma unroll
-                for (int q = 0; q < Q; ++q)
-                    pw[q * WAVE + lane] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
+        for (int q = 0; q < Q; ++q) {
+            n0[q] = base + (int64_t)q * (WAVE * 4) + (int64_t)lane * 4;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) act[4 * q + k] = n0[q] + k < N;
+        }
+        // upstream values (MODE 0) or targets (MODE 1): loaded now, used
+        // after the forward pass
+        float4 dq[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) dq[q] = need_dy ? load_quad(dy + yo, n0[q], N, vec) : f4_zero();
+#pragma unroll
+        for (int v = 0; v < NXR_ROWS; ++v)
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) gxr[v][k] = 0.0f;
+        if (!xblk && !one_group && !want_gx) put_var_rows(base, vmask);
+        for (int r = NI; r < NI + nextra; ++r) {
+            float4 *pr = row4(r);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) pr[q * WAVE + lane] = f4_zero();
+        }
+        // ---- forward
+        for (int ii = 0; ii < NI; ++ii) {
+            const uint32_t f = rl_u32(rw, ii);
+            const int op = pf_op(f);
+            float a0[PPT], a1[PPT], r[PPT];
+            fetch(f, ii, a0, a1);
+            if (op == OP_COPY) {
+#pragma unroll
+                for (int k = 0; k < PPT; ++k) r[k] = a0[k];
+            } else {
+                fwd_ops<float, PPT>(op, a0, a1, r);
             }
-            // ---- root value, loss and upstream gradient; the root's adjoint
-            //      replaces its value
+            float4 *pw = row4(ii);
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                pw[q * WAVE + lane] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
+        }
+        // ---- root value, loss and upstream gradient; the root's adjoint
+        //      replaces its value
+        {
+            float4 *pr = row4(NI - 1);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const float4 yq = pr[q * WAVE + lane];
+                if (y_out) store_quad(y_out + yo, n0[q], N, vec, yq);
+                float gq[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const float yh = f4_get(yq, k), dvk = f4_get(dq[q], k);
+                    float g0 = 0.0f;
+                    if (a.mode == 1) {
+                        const float d = yh - dvk;
+                        if (act[4 * q + k]) lg += (double)d * (double)d;
+                        g0 = (2.0f * d) * inv_n;
+                    } else if (!a.fwd_only) {
+                        g0 = dvk;
+                    }
+                    gq[k] = act[4 * q + k] ? g0 : 0.0f;
+                }
+                pr[q * WAVE + lane] = make_float4(gq[0], gq[1], gq[2], gq[3]);
+            }
+        }
+        if (a.fwd_only) return;
+        // ---- reverse sweep
+        for (int ii = NI - 1; ii >= 0; --ii) {
+            const uint32_t f = rl_u32(rw, ii), sw = rl_u32(rw, 16 + ii);
+            const int op = pf_op(f);
+            const bool binary = pf_bin(f);
+            float g[PPT], a0[PPT], a1[PPT], ge0[PPT], ge1[PPT];
             {
-                float4 *pr = row4(NI - 1);
+                const float4 *pg = row4(ps_grow(sw));
 #pragma unroll
                 for (int q = 0; q < Q; ++q) {
-                    const float4 yq = pr[q * WAVE + lane];
-                    if (y_out) store_quad(y_out + yo, n0[q], N, vec, yq);
-                    float gq[4];
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        const float yh = f4_get(yq, k), dvk = f4_get(dq[q], k);
-                        float g0 = 0.0f;
-                        if (a.mode == 1) {
-                            const float d = yh - dvk;
-                            if (act[4 * q + k]) lacc += (double)d * (double)d;
-                            g0 = (2.0f * d) * inv_n;
-                        } else if (!a.fwd_only) {
-                            g0 = dvk;
-                        }
-                        gq[k] = act[4 * q + k] ? g0 : 0.0f;
-                    }
-                    pr[q * WAVE + lane] = make_float4(gq[0], gq[1], gq[2], gq[3]);
+                    const float4 t = pg[q * WAVE + lane];
+                    g[4 * q] = t.x;
+                    g[4 * q + 1] = t.y;
+                    g[4 * q + 2] = t.z;
+                    g[4 * q + 3] = t.w;
                 }
             }
-            if (a.fwd_only) continue;
-            // ---- reverse sweep
-            for (int ii = NI - 1; ii >= 0; --ii) {
-                const uint32_t f = rl_u32(rw, ii), sw = rl_u32(rw, 16 + ii);
-                const int op = pf_op(f);
-                const bool binary = pf_bin(f);
-                float g[PPT], a0[PPT], a1[PPT], ge0[PPT], ge1[PPT];
-                {
-                    const float4 *pg = row4(ps_grow(sw));
+            fetch(f, ii, a0, a1);
+            bwd_masked<float, PPT>(op, g, a0, a1, act, ge0, ge1, e);
+            auto scatter = [&](int k, const float (&ge)[PPT]) {
+                const uint32_t mode = ps_mode(sw, k);
+                const int tg = ps_tgt(sw, k);
+                if (mode - 1u < 3u) {  // PM_WRITE, PM_CROW, PM_ACC: a tape row
+                    float4 *pt = row4(tg);
 #pragma unroll
                     for (int q = 0; q < Q; ++q) {
-                        const float4 t = pg[q * WAVE + lane];
-                        g[4 * q] = t.x;
-                        g[4 * q + 1] = t.y;
-                        g[4 * q + 2] = t.z;
-                        g[4 * q + 3] = t.w;
+                        const float4 o = mode == PM_ACC ? pt[q * WAVE + lane] : f4_zero();
+                        pt[q * WAVE + lane] = make_float4(o.x + ge[4 * q], o.y + ge[4 * q + 1], o.z + ge[4 * q + 2],
+                                                          o.w + ge[4 * q + 3]);
                     }
-                }
-                fetch(f, ii, a0, a1);
-                bwd_masked<float, PPT>(op, g, a0, a1, act, ge0, ge1, e);
-                auto scatter = [&](int k, const float (&ge)[PPT]) {
-                    const uint32_t mode = ps_mode(sw, k);
-                    const int tg = ps_tgt(sw, k);
-                    if (mode - 1u < 3u) {  // PM_WRITE, PM_CROW, PM_ACC: a tape row
-                        float4 *pt = row4(tg);
+                } else if (mode == PM_VAR) {  // a variable: its adjoint for grad_X
+                    if (want_gx) {
 #pragma unroll
-                        for (int q = 0; q < Q; ++q) {
-                            const float4 o = mode == PM_ACC ? pt[q * WAVE + lane] : f4_zero();
-                            pt[q * WAVE + lane] = make_float4(o.x + ge[4 * q], o.y + ge[4 * q + 1], o.z + ge[4 * q + 2],
-                                                              o.w + ge[4 * q + 3]);
-                        }
-                    } else if (mode == PM_VAR) {  // a variable: its adjoint for grad_X
-                        if (want_gx) {
+                        for (int v = 0; v < NXR_ROWS; ++v)
+                            if (v == tg)
 #pragma unroll
-                            for (int v = 0; v < NXR_ROWS; ++v)
-                                if (v == tg)
+                                for (int k2 = 0; k2 < PPT; ++k2) gxr[v][k2] += ge[k2];
+                    }
+                } else if (mode == PM_CONST) {  // a constant used several times: wave sum per use
+                    double gv = 0.0;
 #pragma unroll
-                                    for (int k2 = 0; k2 < PPT; ++k2) gxr[v][k2] += ge[k2];
-                        }
-                    } else if (mode == PM_CONST) {  // a constant used several times: wave sum per use
-                        double gv = 0.0;
-#pragma unroll
-                        for (int k2 = 0; k2 < PPT; ++k2) gv += (double)ge[k2];
-                        const double s = wave_sum_rows(gv, 4);
-                        if (lane == tg) gc += s;
-                    } else if (mode == PM_NOOP) {
-#pragma unroll
-                        for (int k2 = 0; k2 < PPT; ++k2)
-                            if (ge[k2] != 0.0f) e = e > ERR_GRAD_NO_OP ? e : ERR_GRAD_NO_OP;
-                    }  // PM_NONE adds nothing
-                };
-                scatter(0, ge0);
-                if (binary) scatter(1, ge1);
-            }
-            // ---- adjoints of the constants used once, from their rows, in FP64
-            for (int j = 0; j < nc; ++j) {
-                const int row = crow(j);
-                if (row == (int)NO_CROW) continue;
-                const float4 *pc = row4(row);
-                double gv = 0.0;
-#pragma unroll
-                for (int q = 0; q < Q; ++q) {
-                    const float4 t = pc[q * WAVE + lane];
-                    gv += (double)t.x;
-                    gv += (double)t.y;
-                    gv += (double)t.z;
-                    gv += (double)t.w;
-                }
-                if (j < NCW) {
-#pragma unroll
-                    for (int r = 0; r < NCW; ++r)
-                        if (r == j) cacc[r] += gv;
-                } else {
+                    for (int k2 = 0; k2 < PPT; ++k2) gv += (double)ge[k2];
                     const double s = wave_sum_rows(gv, 4);
-                    if (lane == j) gc += s;
-                }
+                    if (lane == tg) gcs += s;
+                } else if (mode == PM_NOOP) {
+#pragma unroll
+                    for (int k2 = 0; k2 < PPT; ++k2)
+                        if (ge[k2] != 0.0f) e = e > ERR_GRAD_NO_OP ? e : ERR_GRAD_NO_OP;
+                }  // PM_NONE adds nothing
+            };
+            scatter(0, ge0);
+            if (binary) scatter(1, ge1);
+        }
+        // ---- adjoints of the constants used once, from their rows, in FP64
+        for (int j = 0; j < nc; ++j) {
+            const int row = crow(j);
+            if (row == (int)NO_CROW) continue;
+            const float4 *pc = row4(row);
+            double gv = 0.0;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const float4 t = pc[q * WAVE + lane];
+                gv += (double)t.x;
+                gv += (double)t.y;
+                gv += (double)t.z;
+                gv += (double)t.w;
             }
-            // ---- variable adjoints of this tile group into the partial row
-            //      (a zero contribution is not added: the sum keeps a -0)
-            if (want_gx) {
+            if (j < NCW) {
 #pragma unroll
-                for (int v = 0; v < NXR_ROWS; ++v) {
-                    if (v >= n_x || !((vmask >> v) & 1)) continue;
-#pragma unroll
-                    for (int q = 0; q < Q; ++q) {
-                        const float4 o = gxp[v][q];
-                        const float t0 = gxr[v][4 * q], t1 = gxr[v][4 * q + 1], t2 = gxr[v][4 * q + 2],
-                                    t3 = gxr[v][4 * q + 3];
-                        const float4 sm = make_float4(t0 == 0.f ? o.x : o.x + t0, t1 == 0.f ? o.y : o.y + t1,
-                                                      t2 == 0.f ? o.z : o.z + t2, t3 == 0.f ? o.w : o.w + t3);
-                        store_quad(gxrow + (int64_t)v * N, n0[q], N, vec, sm);
-                    }
-                }
+                for (int r = 0; r < NCW; ++r)
+                    if (r == j) cg[r] += gv;
+            } else {
+                const double s = wave_sum_rows(gv, 4);
+                if (lane == j) gcs += s;
             }
         }
-        if (a.mode == 1 && a.loss_out) {
-            const double s = wave_sum_rows(lacc, 4);
-            if (lane == 0) a.loss_out[b] = s / (double)N;
-        }
+    };
+    // ---- a program's results: loss, grad_C at its node slots, error code
+    //      (gc: lane j its constant j's gradient; loss and e wave-uniform)
+    auto finish = [&](int64_t b, uint32_t hw, double loss_sum, double gc, int32_t e) {
+        if (a.mode == 1 && a.loss_out && lane == 0) a.loss_out[b] = loss_sum / (double)N;
         if (a.fwd_only) {
             if (lane == 0 && a.err_b) a.err_b[b] = 0;
-            continue;
+            return;
         }
-        const int ncr = nc < NCW ? nc : NCW;
-#pragma unroll
-        for (int r = 0; r < NCW; ++r) {
-            if (r < ncr) {
-                const double s = wave_sum_rows(cacc[r], 4);
-                if (lane == r) gc += s;
-            }
-        }
+        const uint64_t cm0 = (uint64_t)rl_u32(hw, 5) | ((uint64_t)rl_u32(hw, 6) << 32);
+        const uint64_t cm1 = (uint64_t)rl_u32(hw, 7) | ((uint64_t)rl_u32(hw, 8) << 32);
         // dL/dC back to node slots: constant node k takes its compact lane
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -967,6 +918,116 @@ __global__ __launch_bounds__(WPB * 64) void k_fused_wide(FArgs a) {
         if (lane == 0) {
             if (e) atomicMax(&a.flags[1], e);
             if (a.err_b) a.err_b[b] = e;
+        }
+    };
+
+    const int64_t count = (int64_t)*a.list_count;
+    if (!want_gx) {
+        // ---- one program over all its tile groups at a time, CH list
+        //      positions per ticket
+        constexpr int CH = fetch_chunk(LT);
+        for (;;) {
+            const int64_t c = wave_fetch_add_n(a.work_counter, CH);
+            if (c >= count) break;
+            const int64_t pe = c + CH < count ? c + CH : count;
+            for (int64_t i = c; i < pe; ++i) {
+                const int64_t b = (int64_t)a.list[i];
+                const uint32_t *rec = a.plan + (size_t)b * PREC_WORDS;
+                const uint32_t rw = rec[lane];
+                const uint32_t hw = rec[64 + (lane & 15)];
+                const int nc = (int)((rl_u32(hw, 4) >> 16) & 255u);
+                double lacc = 0.0, gc = 0.0;
+                double cacc[NCW];
+#pragma unroll
+                for (int q = 0; q < NCW; ++q) cacc[q] = 0.0;
+                int32_t e = 0;
+                float gxr[NXR_ROWS][PPT];
+                for (int64_t base = 0; base < N; base += GRP) eval_group(b, rw, hw, base, lacc, cacc, gc, e, gxr);
+                const double loss = a.mode == 1 ? wave_sum_rows(lacc, 4) : 0.0;
+#pragma unroll
+                for (int r = 0; r < NCW; ++r) {
+                    if (r < nc) {
+                        const double sr = wave_sum_rows(cacc[r], 4);
+                        if (lane == r) gc += sr;
+                    }
+                }
+                finish(b, hw, loss, gc, e);
+            }
+        }
+        return;
+    }
+    // ---- grad_X: chunks of kc list positions, group by group
+    const int64_t kc = (count + a.gx_rows - 1) / a.gx_rows;
+    const int64_t ngrp = (N + GRP - 1) / GRP;
+    for (;;) {
+        const int64_t c = wave_fetch_add(a.work_counter);
+        if (kc == 0 || c * kc >= count) break;
+        const int64_t p0 = c * kc, p1 = p0 + kc < count ? p0 + kc : count;
+        float *const gxrow = (float *)a.gx_partial + (a.gx_row0 + c) * gx_len;
+        for (int64_t gi = 0; gi < ngrp; ++gi) {
+            const int64_t base = gi * GRP;
+            if (!xblk && !one_group) put_var_rows(base, (1 << n_x) - 1);
+            float gxacc[NXR_ROWS][PPT];
+#pragma unroll
+            for (int v = 0; v < NXR_ROWS; ++v)
+#pragma unroll
+                for (int k = 0; k < PPT; ++k) gxacc[v][k] = 0.0f;
+            for (int64_t i = p0; i < p1; ++i) {
+                const int64_t b = (int64_t)a.list[i];
+                const uint32_t *rec = a.plan + (size_t)b * PREC_WORDS;
+                const uint32_t rw = rec[lane];
+                const uint32_t hw = rec[64 + (lane & 15)];
+                const int nc = (int)((rl_u32(hw, 4) >> 16) & 255u);
+                double *tot = a.gx_tot + b * WIDE_TOTW;
+                // this program's sums so far (loaded now, added to below):
+                // lanes 0..nc-1 its constants, lane nc its loss, nc + 1 its
+                // error code
+                const int nt = nc + 2;
+                const double prev = (gi > 0 && lane < nt) ? tot[lane] : 0.0;
+                double lg = 0.0, gcs = 0.0;
+                double cg[NCW];
+#pragma unroll
+                for (int q = 0; q < NCW; ++q) cg[q] = 0.0;
+                int32_t e = 0;
+                float gxr[NXR_ROWS][PPT];
+                eval_group(b, rw, hw, base, lg, cg, gcs, e, gxr);
+#pragma unroll
+                for (int v = 0; v < NXR_ROWS; ++v)
+#pragma unroll
+                    for (int k = 0; k < PPT; ++k) gxacc[v][k] += gxr[v][k];
+                // this group's sums, then the running ones
+#pragma unroll
+                for (int r = 0; r < NCW; ++r) {
+                    if (r < nc) {
+                        const double sr = wave_sum_rows(cg[r], 4);
+                        if (lane == r) gcs += sr;
+                    }
+                }
+                const double lsum = a.mode == 1 ? wave_sum_rows(lg, 4) : 0.0;
+                const int32_t eg = wave_max_err(e);
+                double run = lane == nc ? lsum : (lane == nc + 1 ? (double)eg : gcs);
+                if (gi > 0) run = lane == nc + 1 ? (prev > run ? prev : run) : prev + run;
+                if (gi + 1 < ngrp) {
+                    if (lane < nt) tot[lane] = run;
+                } else {
+                    const double loss = uniform_f64(rl(run, nc));
+                    const int32_t et = uniform_i32((int32_t)rl(run, nc + 1));
+                    finish(b, hw, loss, run, et);
+                }
+            }
+            // the chunk's variable adjoints of this group, written once
+            // (every point of every variable: a zero where none was added)
+#pragma unroll
+            for (int v = 0; v < NXR_ROWS; ++v) {
+                if (v >= n_x) continue;
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    const int64_t n0 = base + (int64_t)q * (WAVE * 4) + (int64_t)lane * 4;
+                    store_quad(gxrow + (int64_t)v * N, n0, N, vec,
+                               make_float4(gxacc[v][4 * q], gxacc[v][4 * q + 1], gxacc[v][4 * q + 2],
+                                           gxacc[v][4 * q + 3]));
+                }
+            }
         }
     }
 }
@@ -1007,10 +1068,13 @@ constexpr int ROW_CHUNK = 32;
 
 // The partial rows of each bucket launch: [row0, row0 + cap).  A launch of
 // k_fused_wide with grad_X fills only the first rows of its range (one per
-// non-empty list chunk); the other launches fill all of theirs.
+// non-empty list chunk); a launch with one row per wave in the workspace
+// fills the rows of the waves whose segment is not empty; a launch with one
+// row per block fills all of its rows.
 struct RowSel {
     int64_t row0[F_NBUCKETS], cap[F_NBUCKETS];
     int chunked[F_NBUCKETS];
+    const int32_t *seg[F_NBUCKETS];  // row w holds data iff seg[w] < seg[w + 1] (nullptr: every row)
     const int32_t *counts;  // bucket list lengths (device)
 };
 __device__ inline int64_t rows_used(const RowSel &rs, int k) {
@@ -1033,7 +1097,25 @@ __global__ void k_sum_rows_1(const T *__restrict__ part, int64_t P, int64_t len,
         const int64_t lo = r0 > rs.row0[k] ? r0 : rs.row0[k];
         const int64_t e = rs.row0[k] + rows_used(rs, k);
         const int64_t hi = r1 < e ? r1 : e;
-        if (lo < hi) s = sum_rows_in_order(part + i, lo, hi, len, s);
+        if (lo >= hi) continue;
+        if (rs.seg[k]) {
+            // rows of empty segments hold no data: they count as zeros (the
+            // FP64 sum starts at +0 and so never holds -0, and adding +0
+            // changes nothing)
+            const int32_t *sg = rs.seg[k] + (lo - rs.row0[k]);
+            constexpr int U = 16;
+            int64_t r = lo;
+            for (; r < hi; r += U) {
+                T v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    v[u] = (r + u < hi && sg[r - lo + u] < sg[r - lo + u + 1]) ? part[(r + u) * len + i] : T(0);
+#pragma unroll
+                for (int u = 0; u < U; ++u) s += (double)v[u];
+            }
+        } else {
+            s = sum_rows_in_order(part + i, lo, hi, len, s);
+        }
     }
     part64[(int64_t)blockIdx.y * len + i] = s;
 }
@@ -1191,9 +1273,11 @@ template <> struct FTab<double> {
 
 // Workspace: lists [5][B] int32, counters [5]+[5] | queue | segment starts
 // and segment queue workspace (grad_X wanted) | grad_X partial rows | FP64
-// chunk sums of the rows | global tape (bucket 4).
+// chunk sums of the rows | global tape (bucket 4) | plan records and
+// transposed X (k_fused_wide) | per-program running sums (k_fused_wide with
+// grad_X).
 struct FWs {
-    size_t q_off, seg_off, sq_off, gx_off, g64_off, gt_off, plan_off, xt_off, total;
+    size_t q_off, seg_off, sq_off, gx_off, g64_off, gt_off, plan_off, xt_off, tot_off, total;
     int64_t nrows[F_NBUCKETS];  // partial rows per bucket
     int chunked[F_NBUCKETS];    // k_fused_wide with grad_X: rows per list chunk
     bool wide;              // some bucket runs k_fused_wide: plans and xt are laid out
@@ -1227,11 +1311,13 @@ FWs fused_layout(int64_t N, int64_t n_x, int64_t B, int64_t M, bool want_gx, con
             const int mode = f.gxmode(want_gx, gx_len);
             const int grid = f.grid(mode, gx_len, ni, N);
             const int64_t W = (int64_t)grid * f.wpb;
-            // k_fused_wide: two list chunks per wave, each with its own row
+            // k_fused_wide: WIDE_CHUNKS_PER_WAVE list chunks per wave, each
+            // with its own row
             int64_t rows = mode == 1 ? grid : (mode == 2 ? W : 0);
             if (f.wide && mode == 2) {
                 const int64_t cap = (int64_t)GX_GLOBAL_BUDGET / (gx_len * (int64_t)sizeof(T) > 0 ? gx_len * (int64_t)sizeof(T) : 1);
-                rows = 2 * W < cap ? 2 * W : (cap > W ? cap : W);
+                const int64_t want = W * WIDE_CHUNKS_PER_WAVE;
+                rows = want < cap ? want : (cap > W ? cap : W);
             }
             if (var) {
                 w.grid[k] = grid;
@@ -1279,7 +1365,10 @@ FWs fused_layout(int64_t N, int64_t n_x, int64_t B, int64_t M, bool want_gx, con
         off = (off + 255) / 256 * 256;
         w.xt_off = off;
         off += (size_t)NXR_ROWS * wide_npad(N) * sizeof(float);
+        off = (off + 255) / 256 * 256;
     }
+    w.tot_off = off;
+    if (w.wide && want_gx) off += (size_t)B * WIDE_TOTW * sizeof(double);
     w.total = off + 256;
     return w;
 }
@@ -1391,6 +1480,7 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
         a.plan = (const uint32_t *)(ws + L.plan_off);
         a.xt = (const float *)(ws + L.xt_off);
         a.n_pad = n_pad;
+        a.gx_tot = want_gx ? (double *)(ws + L.tot_off) : nullptr;
         a.vec = (N % 4 == 0 && ((uintptr_t)dy & 15) == 0 && ((uintptr_t)y_out & 15) == 0) ? 1 : 0;
     }
     auto bucket_args = [&](int k) {
@@ -1444,6 +1534,7 @@ int fused_root_code(const T *X, int64_t N, int64_t n_x, const int64_t *ops, cons
             rs.row0[k] = L.row0[k];
             rs.cap[k] = L.nrows[k];
             rs.chunked[k] = L.chunked[k];
+            rs.seg[k] = (L.mode[k] == 2 && !L.chunked[k]) ? segs.seg[k] : nullptr;
         }
         rs.counts = counters;
         hipLaunchKernelGGL(k_sum_rows_1<T>, dim3(nb, (unsigned)chunks), dim3(256), 0, s, (const T *)(ws + L.gx_off),
