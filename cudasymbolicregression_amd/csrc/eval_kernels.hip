@@ -462,6 +462,89 @@ __global__ __launch_bounds__(BWD_WAVES * 64) void k_full_backward(
 constexpr int BWD_BATCH = 16;
 constexpr int64_t BWD_CHUNKS = 1;  // pipeline chunks of the gathered backward
 constexpr int BWR_WPE = 8;  // waves per SIMD the FP32 row-tape sweep's registers are sized for (FP64 would spill)
+constexpr int BWR_U = 2;    // point tiles the row-tape sweep runs at once (node by node)
+
+// Reverse sweep of one expression over BWR_U point tiles at once, on its
+// gathered rows: vr (values) and gr (adjoints, accumulated in place), row k
+// of a tile at k * N + n.  Node by node, each step touches the BWR_U tiles'
+// rows together, so their loads are in flight at once.  For each tile the
+// steps are backward_tile's, in the same node order, and a constant's FP64
+// sum takes its tiles in tile order, so every result equals tile-by-tile
+// sweeps bit for bit.  Variable adjoints are not summed here (var_adjoints
+// reads them back from the rows).  Rows are clamped below L (a valid program
+// only names children below their parent; an unvalidated one stays in
+// bounds).  Returns this lane's error code.
+template <typename T, int U>
+__device__ inline int32_t backward_rows_batch(const Program<T> &p, const T *__restrict__ vr, T *__restrict__ gr,
+                                              int64_t N, int L, const int64_t (&nu)[U], const bool (&act)[U],
+                                              GradAcc<T> &acc, double *cextra) {
+    const int lane = lane_id();
+    int32_t err = 0;
+    auto row = [&](int k) -> int64_t { return (int64_t)(k < L ? k : L - 1) * N; };
+    for (int k = L - 1; k >= 0; --k) {
+        const uint32_t w = p.word(k);
+        const int op = node_op(w);
+        if (op == UNKNOWN_LEAF || (op < SIN && op != NO_OP && op != LEARNABLE_CONSTANT) || op >= VAR_START_ID)
+            continue;  // hard constants, 0-valued leaves, variables: nothing to do
+        const int64_t rk = row(k);
+        T g[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) g[u] = gr[rk + nu[u]];
+        if (op == NO_OP) {
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (act[u] && g[u] != T(0)) err = err > ERR_GRAD_NO_OP ? err : ERR_GRAD_NO_OP;
+            continue;
+        }
+        if (op == LEARNABLE_CONSTANT) {
+            const int j = p.const_ordinal(k);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const double gv = (act[u] && g[u] != T(0)) ? (double)g[u] : 0.0;
+                if (j < NCREG) {
+#pragma unroll
+                    for (int q = 0; q < NCREG; ++q)
+                        if (q == j) acc.cacc[q] += gv;
+                } else {
+                    const double sw = wave_sum(gv);
+                    if (lane == 0) cextra[k] += sw;
+                }
+            }
+            continue;
+        }
+        const int c0 = node_c0(w), c1 = node_c1(w);
+        const int ar = arity_of(op);
+        const int64_t r0 = row(c0), r1 = row(c1);
+        T a0[U], a1[U], o0[U], o1[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            a0[u] = vr[r0 + nu[u]];
+            a1[u] = ar == 2 ? vr[r1 + nu[u]] : T(0);
+            o0[u] = gr[r0 + nu[u]];
+            o1[u] = ar == 2 ? gr[r1 + nu[u]] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            T go0 = T(0), go1 = T(0);
+            bool add0, add1;
+            const int32_t e = bwd_node<T>(op, g[u], a0[u], a1[u], go0, add0, go1, add1);
+            if (act[u] && e > err) err = e;
+            add0 = act[u] && add0;
+            add1 = act[u] && add1 && ar == 2;
+            if (ar == 2 && r0 == r1) {
+                // one row takes both adds, first operand first
+                T v = o0[u];
+                if (add0) v += go0;
+                if (add1) v += go1;
+                if (add0 || add1) gr[r0 + nu[u]] = v;
+            } else {
+                if (add0) gr[r0 + nu[u]] = o0[u] + go0;
+                if (add1) gr[r1 + nu[u]] = o1[u] + go1;
+            }
+        }
+    }
+    return err;
+}
 
 // Backward gather (with a row-buffer workspace).  The grouped backward
 // below reads the upstream gradient and the cached values per (node, point)
@@ -596,6 +679,46 @@ k_full_backward_grp(
     const int Lr = (from_rows && bvalid) ? (p.L < (int)fr.Lw[e_id] ? p.L : (int)fr.Lw[e_id]) : 0;
     __syncthreads();
     const int Lel = Ls[el];
+    if constexpr (ROWS && !MANYX) {
+        // BWR_U tiles at a time, node by node (backward_rows_batch)
+        T *gr0 = reinterpret_cast<T *>(fr.rows) + rbase;
+        const T *vr0 = vrows + rbase;
+        for (int64_t t0 = 0; t0 < n_tiles; t0 += BWR_U) {
+            int64_t nu[BWR_U];
+            bool au[BWR_U];
+#pragma unroll
+            for (int u = 0; u < BWR_U; ++u) {
+                const int64_t n = (t0 + u) * TILE + lane;
+                au[u] = bvalid && t0 + u < n_tiles && n < N;
+                nu[u] = au[u] ? n : 0;
+            }
+            if (Lr > 0) {
+                const int32_t e = backward_rows_batch<T, BWR_U>(p, vr0, gr0, N, Lr, nu, au, acc, cextra);
+                err = e > err ? e : err;
+            }
+            if (want_gx) {
+#pragma unroll
+                for (int u = 0; u < BWR_U; ++u) {
+                    if (t0 + u >= n_tiles) break;  // uniform over the block
+                    T gxl[NXREG];
+                    auto adj = [&](int k) -> T & { return gr0[(int64_t)(k < Lr ? k : Lr - 1) * N + nu[u]]; };
+                    var_adjoints<T>(p, adj, Lr, au[u], 0, gxl);
+#pragma unroll
+                    for (int q = 0; q < NXREG; ++q) gxs[(wave * WAVE + lane) * NXREG + q] = gxl[q];
+                    __syncthreads();
+                    const int64_t n = (t0 + u) * TILE + lane;
+                    if (wave == 0 && n < N) {
+                        for (int v = 0; v < n_x; ++v) {
+                            T sum = gxs[(0 * WAVE + lane) * NXREG + v];
+                            for (int w2 = 1; w2 < BWD_WAVES; ++w2) sum += gxs[(w2 * WAVE + lane) * NXREG + v];
+                            gx_partial[blk * N * n_x + n * n_x + v] = sum;
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+    } else
     for (int64_t tt = 0; tt < n_tiles; ++tt) {
         if constexpr (!ROWS) {
         // cooperative load, expression index fastest
