@@ -470,7 +470,8 @@ constexpr int BWR_U = 2;    // point tiles the row-tape sweep runs at once (node
 // rows together, so their loads are in flight at once.  For each tile the
 // steps are backward_tile's, in the same node order, and a constant's FP64
 // sum takes its tiles in tile order, so every result equals tile-by-tile
-// sweeps bit for bit.  Variable adjoints are not summed here (var_adjoints
+// sweeps (as values: a NaN's payload may differ, the compiler being free to
+// order the operands of an addition).  Variable adjoints are not summed here (var_adjoints
 // reads them back from the rows).  Rows are clamped below L (a valid program
 // only names children below their parent; an unvalidated one stays in
 // bounds).  Returns this lane's error code.
