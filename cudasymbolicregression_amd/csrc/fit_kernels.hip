@@ -91,11 +91,18 @@ struct FitArgs {
 };
 
 // FP64 inner product over the K live slots; lanes past K never contribute.
+// With K <= 4 only the first quad holds terms: its two quad steps and one
+// + 0.0 give the bits of the full row reduction (whose other steps add +0).
 template <int SL>
 __device__ inline double sdot(const SVec<float, SL> &a, const SVec<float, SL> &b, int K) {
     const int lane = lane_id();
     double s = (lane < K) ? (double)a.v[0] * (double)b.v[0] : 0.0;
     if (SL > 1 && lane + WAVE < K) s += (double)a.v[SL - 1] * (double)b.v[SL - 1];
+    if (K <= 4) {
+        s += dpp_f64<DPP_QUAD_SWAP1>(s);
+        s += dpp_f64<DPP_QUAD_SWAP2>(s);
+        return uniform_f64(rl(s, 0) + 0.0);
+    }
     return wave_sum_rows(s, K <= 16 ? 1 : 4);
 }
 
@@ -634,18 +641,28 @@ void k_fit(FitArgs a) {
                 }
                 if (__ballot(notnan) == 0ull) break;
             }
-            // two-loop recursion, newest pair first; alpha_j kept in lane j
-            SVec<float, SL> q = g;
-            double alph_lane = 0.0;
-            for (int jj = nh - 1; jj >= 0; --jj) {
+            // two-loop recursion, newest pair first; alpha_j kept in lane j.
+            // Each pair's history rows are read one step ahead, so their LDS
+            // latency overlaps the previous step's inner product.
+            auto load_pair = [&](int jj, SVec<float, SL> &sj, SVec<float, SL> &yj, double &rj) {
                 const int sl = head + jj < hist ? head + jj : head + jj - hist;
-                SVec<float, SL> sj, yj;
 #pragma unroll
                 for (int h = 0; h < SL; ++h) {
                     sj.v[h] = Sh[((size_t)sl * SL + h) * WAVE + lane];
                     yj.v[h] = Yh[((size_t)sl * SL + h) * WAVE + lane];
                 }
-                const double aj = uniform_f64(rho_l[sl]) * sdot<SL>(sj, q, K);
+                rj = rho_l[sl];
+            };
+            SVec<float, SL> q = g;
+            double alph_lane = 0.0;
+            SVec<float, SL> sn, yn;
+            double rn = 0.0;
+            if (nh > 0) load_pair(nh - 1, sn, yn, rn);
+            for (int jj = nh - 1; jj >= 0; --jj) {
+                const SVec<float, SL> sj = sn, yj = yn;
+                const double rj = rn;
+                if (jj > 0) load_pair(jj - 1, sn, yn, rn);
+                const double aj = uniform_f64(rj) * sdot<SL>(sj, q, K);
                 if (lane == jj) alph_lane = aj;
                 const float af = (float)aj;
 #pragma unroll
@@ -653,25 +670,18 @@ void k_fit(FitArgs a) {
             }
             SVec<float, SL> z = q;
             if (nh > 0) {
-                const int sl = head + nh - 1 < hist ? head + nh - 1 : head + nh - 1 - hist;
-                SVec<float, SL> sn, yn;
-#pragma unroll
-                for (int h = 0; h < SL; ++h) {
-                    sn.v[h] = Sh[((size_t)sl * SL + h) * WAVE + lane];
-                    yn.v[h] = Yh[((size_t)sl * SL + h) * WAVE + lane];
-                }
-                const float gamma = (float)(sdot<SL>(sn, yn, K) / (sdot<SL>(yn, yn, K) + a.eps));
+                SVec<float, SL> s_new, y_new;  // the newest pair
+                double r_new;
+                load_pair(nh - 1, s_new, y_new, r_new);
+                load_pair(0, sn, yn, rn);
+                const float gamma = (float)(sdot<SL>(s_new, y_new, K) / (sdot<SL>(y_new, y_new, K) + a.eps));
 #pragma unroll
                 for (int h = 0; h < SL; ++h) z.v[h] = gamma * q.v[h];
                 for (int jj = 0; jj < nh; ++jj) {
-                    const int sj_ = head + jj < hist ? head + jj : head + jj - hist;
-                    SVec<float, SL> sj, yj;
-#pragma unroll
-                    for (int h = 0; h < SL; ++h) {
-                        sj.v[h] = Sh[((size_t)sj_ * SL + h) * WAVE + lane];
-                        yj.v[h] = Yh[((size_t)sj_ * SL + h) * WAVE + lane];
-                    }
-                    const double beta = uniform_f64(rho_l[sj_]) * sdot<SL>(yj, z, K);
+                    const SVec<float, SL> sj = sn, yj = yn;
+                    const double rj = rn;
+                    if (jj + 1 < nh) load_pair(jj + 1, sn, yn, rn);
+                    const double beta = uniform_f64(rj) * sdot<SL>(yj, z, K);
                     const float coef = (float)(uniform_f64(rl(alph_lane, jj)) - beta);
 #pragma unroll
                     for (int h = 0; h < SL; ++h) z.v[h] = z.v[h] + sj.v[h] * coef;
