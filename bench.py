@@ -671,7 +671,7 @@ def main() -> int:
     ap.add_argument("--problems", type=int, default=16384)
     ap.add_argument("--seed", type=int, default=5, help="test problems (training uses seed - 1)")
     ap.add_argument("--train-tuples", type=int, default=2_000_000)
-    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--cpu-sample", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-evaluate", action="store_true")

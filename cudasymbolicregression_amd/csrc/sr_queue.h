@@ -86,14 +86,26 @@ __device__ inline void scan_runs(const V *__restrict__ runval, int64_t R, int NB
     }
 }
 
+// With segments it also writes each segment list's entries that pass 3 does
+// not: seg[k][0] = 0, and every entry of an empty bucket's list (pass 3
+// writes entries 1..W[k] of a non-empty one, whose items cover every
+// boundary).
 __global__ __launch_bounds__(256) void k_queue_scan(const int32_t *__restrict__ runcnt, int64_t R, int NB,
                                                    int32_t *__restrict__ runoff, int32_t *__restrict__ counts,
                                                    const int64_t *__restrict__ runcost, int64_t *__restrict__ runcoff,
-                                                   int64_t *__restrict__ ctotal) {
+                                                   int64_t *__restrict__ ctotal, QueueSegs segs) {
     __shared__ int32_t wtot[256 / WAVE];
     __shared__ int64_t wtot64[256 / WAVE];
     scan_runs<int32_t>(runcnt, R, NB, runoff, counts, wtot);
-    if (runcost) scan_runs<int64_t>(runcost, R, NB, runcoff, ctotal, wtot64);
+    if (runcost) {
+        scan_runs<int64_t>(runcost, R, NB, runcoff, ctotal, wtot64);
+        // (scan_runs ends with a barrier after thread 255 wrote the totals)
+        for (int k = 0; k < NB; ++k) {
+            if (segs.seg[k] == nullptr) continue;
+            const int64_t n = ctotal[k] == 0 ? segs.W[k] + 1 : 1;
+            for (int64_t g = threadIdx.x; g < n; g += 256) segs.seg[k][g] = 0;
+        }
+    }
 }
 
 // Pass 1: one wave per run, from the buckets in bucket_of (and the costs).
@@ -155,11 +167,6 @@ __global__ __launch_bounds__(256) void k_queue_scatter(const int8_t *__restrict_
     }
 }
 
-__global__ void k_queue_fill_i32(int32_t *p, int32_t v, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
 }  // namespace
 
 // Passes 1-3, after the caller's classification kernel filled bucket_of
@@ -182,15 +189,11 @@ inline void queue_finish(void *ws, int64_t B, int NB, int32_t *lists, int32_t *c
         runcoff = runcost + (size_t)R * NB;
         ctotal = runcoff + (size_t)R * NB;
         sg = *segs;
-        for (int k = 0; k < NB; ++k) {
-            const int64_t n = sg.W[k] + 1;
-            hipLaunchKernelGGL(k_queue_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sg.seg[k], 0, n);
-        }
     }
     hipLaunchKernelGGL(k_queue_count, dim3((unsigned)((B + 4 * QRUN - 1) / (4 * QRUN))), dim3(256), 0, s, bucket_of,
                        B, NB, runcnt, cost_of, runcost);
     hipLaunchKernelGGL(k_queue_scan, dim3(1), dim3(256), 0, s, runcnt, R, NB, runoff, counts, runcost, runcoff,
-                       ctotal);
+                       ctotal, sg);
     hipLaunchKernelGGL(k_queue_scatter, dim3((unsigned)((B + 4 * QRUN - 1) / (4 * QRUN))), dim3(256), 0, s,
                        bucket_of, runoff, B, NB, lists, cost_of, runcoff, ctotal, sg);
 }
