@@ -55,6 +55,7 @@ namespace {
 
 constexpr int MHMAX = 16;      // largest supported history_size
 constexpr int FIT_NCREG = 8;   // constants accumulated in per-lane FP64 registers
+constexpr int FIT_NB = 5;      // tape-row buckets
 
 struct FitArgs {
     const float *X;
@@ -88,6 +89,11 @@ struct FitArgs {
     float *gtape;            // global tape slots (programs longer than 64 nodes)
     int stationary_exit;     // end a problem once its iteration is an exact fixed point
     int starts;              // fits per problem: item b = problem * starts + start
+    // Blocks past grid_cap (0: none) leave at once when the other buckets
+    // hold at least 1/32 as many problems as this one (counts_all: the
+    // problems per bucket)
+    int grid_cap;
+    const int32_t *counts_all;
 };
 
 // FP64 inner product over the K live slots; lanes past K never contribute.
@@ -378,6 +384,14 @@ void k_fit(FitArgs a) {
     // every listed problem expands into `starts` consecutive work items, so
     // the fits of one problem run side by side over the same program,
     // targets and points
+    if (a.grid_cap > 0 && (int)blockIdx.x >= a.grid_cap) {
+        // The short-program bucket holds most problems; with its full
+        // persistent grid its waves take the issue slots that the longer
+        // buckets' slowest problems (which set the call's duration) need.
+        int64_t other = 0;
+        for (int k = 1; k < FIT_NB; ++k) other += a.counts_all[k];
+        if (other * 32 >= (int64_t)a.counts_all[0]) return;  // uniform over the block
+    }
     const int64_t count = (int64_t)*a.list_count * a.starts;
     const double c1 = 1e-4, c2 = 0.9;
 
@@ -758,6 +772,7 @@ void k_fit(FitArgs a) {
 // adjoint rows) and live slot count K: rows <= 8, <= 16, <= 32, <= 64 (with
 // K <= 64), and the rest.
 constexpr int NB_CLASSIFY = 5;  // buckets of k_classify (== NBUCKETS below)
+static_assert(NB_CLASSIFY == FIT_NB, "k_classify and the fit kernel disagree on the buckets");
 
 // One wave per problem (the program row read coalesced); the run counts of
 // the deterministic bucket lists follow in k_queue_count (sr_queue.h).
@@ -942,6 +957,9 @@ int fit_impl(const float *X, const float *Y, int64_t N, int64_t n_x, const int64
         r.list = lists + (size_t)k * B;
         r.list_count = counters + k;
         r.work_counter = counters + NBUCKETS + k;
+        // the short-program bucket runs one block per CU beside longer ones
+        r.grid_cap = k == 0 ? num_cus() : 0;
+        r.counts_all = counters;
         return r;
     };
     int r = (int)hipGetLastError();
