@@ -91,7 +91,7 @@ LOSS_SLOTS = 5  # d = y - Y, d*d, accumulate, 2*d, *1/N
 FP32_PEAK_TFLOPS = 157.3
 VALU_LANE_PEAK = 256 * 4 * 16 * 2.4e9  # unpacked FP32 lane-instructions/s (3.93e13)
 HBM_PEAK_GBS = 8000.0
-FIT_TRAFFIC_PROFILED = 99.1e6  # HBM bytes per fit call, rocprofv3 FETCH_SIZE/WRITE_SIZE (calibrated)
+FIT_TRAFFIC_PROFILED = 99.0e6  # HBM bytes per fit call, rocprofv3 FETCH_SIZE/WRITE_SIZE (calibrated)
 EVAL_TRAFFIC_PROFILED = 1150.5e6  # HBM bytes per fused evaluate call (65536 x 1024), same counters
 
 
